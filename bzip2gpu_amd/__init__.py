@@ -1,0 +1,29 @@
+"""bzip2gpu_amd - bzip2 block compression on AMD MI355X (gfx950).
+
+The compute path is libbzg.so: hand-written HIP kernels behind the C ABI in
+include/bzg.h.  This package is a thin ctypes layer over it for Python
+callers, tests and benchmarks.  There is no CPU fallback: if the library or
+a GPU is missing, calls raise.
+"""
+from ._lib import (  # noqa: F401
+    BzgError,
+    REF_BLOCK_BYTES,
+    build,
+    bwt,
+    compress,
+    compress_batch_device,
+    compress_bound,
+    compress_plain,
+    crc32,
+    histogram,
+    huff_build,
+    huff_encode,
+    lib,
+    library_path,
+    last_stage_times,
+    mtf,
+    rle1,
+    rle2,
+    symbols_table,
+    version,
+)

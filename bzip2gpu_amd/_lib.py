@@ -1,0 +1,248 @@
+"""ctypes binding of libbzg.so (C ABI: include/bzg.h).
+
+Stage functions take and return torch tensors on the current HIP device,
+mirroring the reference's stage API (include/compression.h of
+SebastianBoteroLeonik/bzip2gpu); whole-path functions take and return bytes.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "libbzg.so")
+REF_BLOCK_BYTES = 900000
+
+_lib = None
+
+
+class BzgError(RuntimeError):
+    pass
+
+
+def build(jobs: int = 8) -> None:
+    """Compiles libbzg.so and the bzip2gpu CLI for gfx950 (in-tree)."""
+    subprocess.run(["make", "-s", "-C", os.path.join(_HERE, "csrc"), f"-j{jobs}"], check=True)
+
+
+def library_path() -> str:
+    return _SO
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_SO):
+        raise BzgError(f"{_SO} is not built; run bzip2gpu_amd.build() (hipcc, gfx950)")
+    L = ctypes.CDLL(_SO)
+    P, i32, i64, u32, sz = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32, ctypes.c_size_t
+    I32P, I64P, U32P, SZP = (ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
+                             ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_size_t))
+    sig = {
+        "bzg_version": ([], ctypes.c_char_p),
+        "bzg_last_error": ([], ctypes.c_char_p),
+        "bzg_init": ([ctypes.c_int], i32),
+        "bzg_compress_bound": ([sz, i32], sz),
+        "bzg_compress_buffer": ([P, sz, ctypes.c_int, i32, P, sz, SZP], i32),
+        "bzg_compress_plain": ([P, sz, ctypes.c_int, i32, P, sz, SZP], i32),
+        "bzg_compress_batch_device": ([P, i32, P, P, P, i32, ctypes.POINTER(ctypes.c_void_p), P, P, P], i32),
+        "bzg_last_stage_times": ([P, I32P], i32),
+        "bzg_crc32": ([P, i32, U32P, P], i32),
+        "bzg_rle1_bound": ([i32], i32),
+        "bzg_rle1": ([P, i32, P, I32P, P], i32),
+        "bzg_bwt": ([P, i32, P, I32P, P], i32),
+        "bzg_symbols_table": ([P, i32, P, I32P, P], i32),
+        "bzg_mtf": ([P, P, i32, P, I32P, P, I32P, P], i32),
+        "bzg_rle2": ([P, i32, i32, P, P, P], i32),
+        "bzg_histogram": ([P, i32, P, P], i32),
+        "bzg_huff_build": ([P, i32, i32, P, P, P, I32P, I32P, P], i32),
+        "bzg_huff_encode": ([P, i32, P, P, P, i32, i32, P, I64P, P], i32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _check(st: int, what: str) -> None:
+    if st != 0:
+        msg = lib().bzg_last_error().decode(errors="replace")
+        raise BzgError(f"{what} failed (status {st}): {msg}")
+
+
+def version() -> str:
+    return lib().bzg_version().decode()
+
+
+def _np_u8(data) -> np.ndarray:
+    if isinstance(data, (bytes, bytearray, memoryview)):
+        return np.frombuffer(bytes(data), dtype=np.uint8)
+    return np.ascontiguousarray(data, dtype=np.uint8)
+
+
+def compress_bound(n: int, block_bytes: int = REF_BLOCK_BYTES) -> int:
+    return int(lib().bzg_compress_bound(n, block_bytes))
+
+
+def _compress(fn, data, level, block_bytes) -> bytes:
+    d = _np_u8(data)
+    cap = compress_bound(d.size, block_bytes)
+    out = np.empty(cap, dtype=np.uint8)
+    n = ctypes.c_size_t(0)
+    st = fn(d.ctypes.data_as(ctypes.c_void_p) if d.size else None, d.size, level, block_bytes,
+            out.ctypes.data_as(ctypes.c_void_p), cap, ctypes.byref(n))
+    _check(st, fn.__name__)
+    return out[: n.value].tobytes()
+
+
+def compress(data, level: int = 9, block_bytes: int = REF_BLOCK_BYTES) -> bytes:
+    """Batched device pipeline (all blocks at once, device-side assembly)."""
+    return _compress(lib().bzg_compress_buffer, data, level, block_bytes)
+
+
+def compress_plain(data, level: int = 9, block_bytes: int = REF_BLOCK_BYTES) -> bytes:
+    """Plain path: one block per pipeline run, host-side serialisation."""
+    return _compress(lib().bzg_compress_plain, data, level, block_bytes)
+
+
+def compress_batch_device(d_in, offsets, lengths, levels, block_bytes: int = 0, stream=None):
+    """Many independent inputs already on the device (torch uint8 tensor).
+
+    Returns (device_out_ptr, byte_offsets np.int64[n], sizes np.int64[n]).
+    The output buffer belongs to the library and is valid until the next call.
+    """
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    ln = np.ascontiguousarray(lengths, dtype=np.uint64)
+    lv = np.ascontiguousarray(levels, dtype=np.int32)
+    n = off.size
+    bo = np.zeros(n, dtype=np.int64)
+    by = np.zeros(n, dtype=np.int64)
+    dout = ctypes.c_void_p(0)
+    st = lib().bzg_compress_batch_device(
+        ctypes.c_void_p(d_in.data_ptr()), n, off.ctypes.data_as(ctypes.c_void_p),
+        ln.ctypes.data_as(ctypes.c_void_p), lv.ctypes.data_as(ctypes.c_void_p), block_bytes,
+        ctypes.byref(dout), bo.ctypes.data_as(ctypes.c_void_p), by.ctypes.data_as(ctypes.c_void_p),
+        ctypes.c_void_p(stream) if stream else None)
+    _check(st, "bzg_compress_batch_device")
+    return dout.value, bo, by
+
+
+def last_stage_times():
+    ms = (ctypes.c_float * 6)()
+    r = ctypes.c_int32(0)
+    _check(lib().bzg_last_stage_times(ms, ctypes.byref(r)), "bzg_last_stage_times")
+    keys = ["front", "bwt", "mtf_rle2", "huffman", "pack", "assemble"]
+    return dict(zip(keys, list(ms))), int(r.value)
+
+
+# ------------------------------------------------------------- stage level
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise BzgError("no HIP device available")
+    return torch
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() else None
+
+
+def _stream():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def crc32(d_in) -> int:
+    c = ctypes.c_uint32(0)
+    _check(lib().bzg_crc32(_ptr(d_in), d_in.numel(), ctypes.byref(c), _stream()), "bzg_crc32")
+    return int(c.value)
+
+
+def rle1(d_in):
+    torch = _torch()
+    n = d_in.numel()
+    out = torch.empty(max(1, lib().bzg_rle1_bound(n)), dtype=torch.uint8, device=d_in.device)
+    m = ctypes.c_int32(0)
+    _check(lib().bzg_rle1(_ptr(d_in), n, _ptr(out), ctypes.byref(m), _stream()), "bzg_rle1")
+    return out[: m.value]
+
+
+def bwt(d_in):
+    torch = _torch()
+    m = d_in.numel()
+    sa = torch.empty(max(1, m), dtype=torch.int32, device=d_in.device)
+    r = ctypes.c_int32(0)
+    _check(lib().bzg_bwt(_ptr(d_in), m, _ptr(sa), ctypes.byref(r), _stream()), "bzg_bwt")
+    return sa[:m], int(r.value)
+
+
+def symbols_table(d_in):
+    tab = np.zeros(256, dtype=np.uint8)
+    c = ctypes.c_int32(0)
+    _check(lib().bzg_symbols_table(_ptr(d_in), d_in.numel(), tab.ctypes.data_as(ctypes.c_void_p),
+                                   ctypes.byref(c), _stream()), "bzg_symbols_table")
+    return tab[: c.value].copy()
+
+
+def mtf(d_rle1, d_sa):
+    torch = _torch()
+    m = d_rle1.numel()
+    out = torch.empty(max(1, m), dtype=torch.uint8, device=d_rle1.device)
+    op = ctypes.c_int32(0)
+    nu = ctypes.c_int32(0)
+    pres = np.zeros(256, dtype=np.uint8)
+    _check(lib().bzg_mtf(_ptr(d_rle1), _ptr(d_sa), m, _ptr(out), ctypes.byref(op),
+                         pres.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nu), _stream()), "bzg_mtf")
+    return out[:m], int(op.value), int(nu.value), pres.astype(bool)
+
+
+def rle2(d_mtf, table_size: int):
+    torch = _torch()
+    m = d_mtf.numel()
+    out = torch.empty(m + 2, dtype=torch.int16, device=d_mtf.device)
+    n = torch.zeros(1, dtype=torch.int32, device=d_mtf.device)
+    _check(lib().bzg_rle2(_ptr(d_mtf), m, table_size, _ptr(out), _ptr(n), _stream()), "bzg_rle2")
+    s = int(n.item())
+    return out[:s]
+
+
+def histogram(d_sym, d_freq=None):
+    torch = _torch()
+    if d_freq is None:
+        d_freq = torch.zeros(258, dtype=torch.int32, device=d_sym.device)
+    _check(lib().bzg_histogram(_ptr(d_sym), d_sym.numel(), _ptr(d_freq), _stream()), "bzg_histogram")
+    return d_freq
+
+
+def huff_build(d_sym, alphabet_size: int):
+    s = d_sym.numel()
+    ln = np.zeros((6, 258), dtype=np.uint8)
+    cd = np.zeros((6, 258), dtype=np.int32)
+    sel = np.zeros(s // 50 + 2, dtype=np.uint8)
+    ng = ctypes.c_int32(0)
+    ns = ctypes.c_int32(0)
+    _check(lib().bzg_huff_build(_ptr(d_sym), s, alphabet_size, ln.ctypes.data_as(ctypes.c_void_p),
+                                cd.ctypes.data_as(ctypes.c_void_p), sel.ctypes.data_as(ctypes.c_void_p),
+                                ctypes.byref(ng), ctypes.byref(ns), _stream()), "bzg_huff_build")
+    return ln, cd, sel[: ns.value].copy(), int(ng.value)
+
+
+def huff_encode(d_sym, ln, cd, sel, n_groups: int):
+    torch = _torch()
+    s = d_sym.numel()
+    ln = np.ascontiguousarray(ln, dtype=np.uint8)
+    cd = np.ascontiguousarray(cd, dtype=np.int32)
+    d_sel = torch.from_numpy(np.ascontiguousarray(sel, dtype=np.uint8)).to(d_sym.device)
+    words = torch.zeros(s * 17 // 32 + 4, dtype=torch.int32, device=d_sym.device)
+    bits = ctypes.c_int64(0)
+    _check(lib().bzg_huff_encode(_ptr(d_sym), s, ln.ctypes.data_as(ctypes.c_void_p),
+                                 cd.ctypes.data_as(ctypes.c_void_p), _ptr(d_sel), int(d_sel.numel()),
+                                 n_groups, _ptr(words), ctypes.byref(bits), _stream()), "bzg_huff_encode")
+    nw = (bits.value + 31) // 32
+    return words[:nw], int(bits.value)

@@ -1,0 +1,633 @@
+// bwt.hip - BWT suffix array of the cyclic rotations of every RLE1 block in
+// a batch, by prefix doubling with early exit.
+//
+// Reference: src/compression/fbwt.cu:13-95 sorts ALL m (rank[SA[i]],
+// rank[(SA[i]+k) mod m]) pairs with a 64-bit-key radix sort in each of
+// ceil(log2 m) rounds.  Here, after an initial sort on the first 4 bytes,
+// a round only touches rotations whose group (equal prefix so far) is not
+// yet a singleton, and sorts each such group by one 21-bit key:
+//     key(i) = RK[(SA[i] + P) mod m]
+// where RK[p] is the position of the first row of p's group (the group
+// "head") and P is the prefix length the groups currently represent.  A
+// group whose block already has P >= m holds identical rotations; it is
+// sorted by start index instead (the reference's stable sort from the
+// identity gives the same order, tests/compression/fbwt.cu:22).
+//
+// Work per round:
+//   plan      fixed-grid scans over the group list: prefix sums of small
+//             group lengths (packed 2048-element LDS sorts) and of large
+//             group tile counts.
+//   keys      gather key(i) for every row of every listed group.
+//   small     one workgroup per pack: LDS radix sort of (group, key)
+//             composite keys, new heads by a workgroup max-scan, RK/SA
+//             written, subgroups of size > 1 appended to the next list.
+//   large     segmented LSD radix sort over 4096-element tiles of the large
+//             groups (histogram -> per-group scan -> stable scatter), then
+//             boundary summaries and heads, as above.
+#include "pipeline.h"
+
+namespace bzg {
+
+// -------------------------------------------------------------- helpers
+// Stable rank of this lane's digit among the items this wave has ranked so
+// far; cnt = this wave's 256 counters in LDS.
+__device__ inline uint32_t wave_rank_digit(uint32_t digit, bool valid, uint32_t* cnt) {
+  uint64_t peers = ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; b++) {
+    bool bit = (digit >> b) & 1u;
+    uint64_t m = ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  uint32_t r = (uint32_t)mbcnt(peers);
+  uint32_t base = 0;
+  if (valid) {
+    base = cnt[digit];
+    int leader = __builtin_ctzll(peers);
+    if (lane_id() == leader) cnt[digit] = base + (uint32_t)__popcll(peers);
+  }
+  return base + r;
+}
+
+__device__ inline int bits_for(uint32_t v) {  // bits to hold 0..v
+  return v == 0 ? 0 : 32 - __clz(v);
+}
+
+// ------------------------------------------------------------------ init
+__global__ void __launch_bounds__(256) k_bwt_init(Dev d) {
+  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
+  const int b = d.tc_blk[t];
+  const BlockDesc D = d.desc[b];
+  const int m = d.st[b].m;
+  const int k = t - D.tile_c0;
+  for (int e = threadIdx.x; e < kTile; e += 256) {
+    int i = k * kTile + e;
+    if (i < m) d.SA[D.off + i] = (uint32_t)i;
+  }
+  if (k == 0 && threadIdx.x == 0) {
+    if (m > 1) {
+      int idx = atomicAdd(&d.ctr[CTR_LIST0], 1);
+      d.seg[0][idx] = Seg{b, 0, m, 0};
+    } else {
+      d.RK[D.off] = 0;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ plan
+struct PlanVals { int a, b, c; };
+__device__ inline PlanVals plan_vals(const Seg& g) {
+  bool small = g.len <= kHalf;
+  return PlanVals{small ? g.len : 0, small ? 0 : tiles_for(g.len), small ? 0 : 1};
+}
+
+__global__ void __launch_bounds__(256) k_plan_scan1(Dev d, int cur) {
+  __shared__ int32_t tmp[3][8];
+  const int n = d.ctr[cur];
+  const int per = (n + kScanWG - 1) / kScanWG;
+  const int i0 = blockIdx.x * per, i1 = min(n, i0 + per);
+  int sa = 0, sb = 0, sc = 0;
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) {
+    PlanVals v = plan_vals(d.seg[cur][i]);
+    sa += v.a; sb += v.b; sc += v.c;
+  }
+  sa = wave_sum(sa); sb = wave_sum(sb); sc = wave_sum(sc);
+  if (lane_id() == 0) { tmp[0][threadIdx.x >> 6] = sa; tmp[1][threadIdx.x >> 6] = sb; tmp[2][threadIdx.x >> 6] = sc; }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    int s = 0;
+    for (int w = 0; w < 4; w++) s += tmp[threadIdx.x][w];
+    d.scan_part[threadIdx.x * kScanWG + blockIdx.x] = s;
+  }
+}
+
+__global__ void __launch_bounds__(kScanWG) k_plan_scan2(Dev d, int cur) {
+  __shared__ int32_t tmp[kScanWG / 64 + 1];
+  for (int q = 0; q < 3; q++) {
+    int v = d.scan_part[q * kScanWG + threadIdx.x];
+    int tot;
+    int ex = wg_excl_sum<kScanWG, int>(v, tmp, &tot);
+    d.scan_part[q * kScanWG + threadIdx.x] = ex;
+    if (threadIdx.x == 0) d.ctr[CTR_SMALL + q] = tot;
+  }
+  if (threadIdx.x == 0) d.ctr[CTR_NSEG] = d.ctr[cur];
+}
+
+__global__ void __launch_bounds__(256) k_plan_scan3(Dev d, int cur) {
+  __shared__ int32_t tmp[8];
+  const int n = d.ctr[cur];
+  const int per = (n + kScanWG - 1) / kScanWG;
+  const int i0 = blockIdx.x * per, i1 = min(n, i0 + per);
+  int ra = d.scan_part[0 * kScanWG + blockIdx.x];
+  int rb = d.scan_part[1 * kScanWG + blockIdx.x];
+  int rc = d.scan_part[2 * kScanWG + blockIdx.x];
+  for (int base = i0; base < i1; base += 256) {
+    int i = base + threadIdx.x;
+    PlanVals v{0, 0, 0};
+    if (i < i1) v = plan_vals(d.seg[cur][i]);
+    int ta, tb, tc;
+    int ea = wg_excl_sum<256, int>(v.a, tmp, &ta);
+    int eb = wg_excl_sum<256, int>(v.b, tmp, &tb);
+    int ec = wg_excl_sum<256, int>(v.c, tmp, &tc);
+    if (i < i1) { d.pa[i] = ra + ea; d.pb[i] = rb + eb; d.pc[i] = rc + ec; }
+    ra += ta; rb += tb; rc += tc;
+  }
+}
+
+void launch_bwt_plan(const Dev& d, int cur, int /*nseg*/, hipStream_t s) {
+  k_plan_scan1<<<kScanWG, 256, 0, s>>>(d, cur);
+  k_plan_scan2<<<1, kScanWG, 0, s>>>(d, cur);
+  k_plan_scan3<<<kScanWG, 256, 0, s>>>(d, cur);
+}
+
+// pack k = small groups whose pa lies in [k*kHalf, (k+1)*kHalf)
+__global__ void __launch_bounds__(256) k_mk_packs(Dev d, int npacks, int nseg) {
+  int k = blockIdx.x * 256 + threadIdx.x;
+  if (k > npacks) return;
+  if (k == npacks) { d.packs[k] = nseg; return; }
+  int target = k * kHalf;
+  int lo = 0, hi = nseg;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (d.pa[mid] < target) lo = mid + 1; else hi = mid;
+  }
+  d.packs[k] = lo;
+}
+
+__global__ void __launch_bounds__(256) k_mk_large(Dev d, int cur, int nseg) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i == 0) d.ctr[cur ^ 1] = 0;  // next list starts empty
+  if (i >= nseg) return;
+  Seg g = d.seg[cur][i];
+  if (g.len <= kHalf) return;
+  d.llist[d.pc[i]] = i;
+  int nt = tiles_for(g.len), t0 = d.pb[i];
+  for (int t = 0; t < nt; t++) {
+    d.ltile_seg[t0 + t] = i;
+    d.ltile_pos[t0 + t] = t * kTile;
+  }
+}
+
+// ------------------------------------------------------------ pack loading
+struct PackLds {
+  int32_t start[1024];
+  int32_t len[1024];
+  int32_t blk[1024];
+  int32_t lofs[1024];
+  int32_t count;
+  int32_t total;
+  int32_t tmp[8];
+};
+
+// Loads the small groups of pack k into LDS (a pack holds <= 1023 groups:
+// each has >= 2 rows and the pack <= kPack-1 rows).
+__device__ inline void load_pack(const Dev& d, int cur, int k, PackLds& P) {
+  const int i0 = d.packs[k], i1 = d.packs[k + 1];
+  int count = 0;
+  for (int base = i0; base < i1; base += 256) {
+    int i = base + threadIdx.x;
+    Seg g{0, 0, 0, 0};
+    bool small = false;
+    if (i < i1) { g = d.seg[cur][i]; small = g.len <= kHalf; }
+    int tot;
+    int ex = wg_excl_sum<256, int>(small ? 1 : 0, P.tmp, &tot);
+    if (small) {
+      int idx = count + ex;
+      P.start[idx] = g.start;
+      P.len[idx] = g.len;
+      P.blk[idx] = g.blk;
+      P.lofs[idx] = d.pa[i];
+    }
+    count += tot;
+  }
+  __syncthreads();
+  if (count > 0) {
+    const int base_pa = P.lofs[0];
+    __syncthreads();
+    for (int j = threadIdx.x; j < count; j += 256) P.lofs[j] -= base_pa;
+    __syncthreads();
+    if (threadIdx.x == 0) P.total = P.lofs[count - 1] + P.len[count - 1];
+  }
+  if (threadIdx.x == 0) P.count = count;
+  __syncthreads();
+}
+
+__device__ inline int find_seg(const PackLds& P, int e) {  // last j with lofs[j] <= e
+  int lo = 0, hi = P.count - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (P.lofs[mid] <= e) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// key of the row at block-local SA index j
+__device__ inline uint32_t make_key(const Dev& d, uint64_t off, int m, int j, int round, int P) {
+  uint32_t v = d.SA[off + j];
+  if (round == 0) {
+    uint32_t k = 0;
+    uint32_t p = v;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      k = (k << 8) | d.T[off + p];
+      p = p + 1 == (uint32_t)m ? 0 : p + 1;
+    }
+    return k;
+  }
+  if (P >= m) return v;  // identical rotations: order by start index
+  uint32_t p = v + (uint32_t)P;
+  if (p >= (uint32_t)m) p -= (uint32_t)m;
+  return d.RK[off + p];
+}
+
+// ------------------------------------------------------------------- keys
+__global__ void __launch_bounds__(256) k_keys(Dev d, int cur, int round, int P, int npacks) {
+  __shared__ PackLds L;
+  const int w = blockIdx.x;
+  if (w < npacks) {
+    load_pack(d, cur, w, L);
+    if (L.count == 0) return;
+    const int total = L.total;
+    for (int j = threadIdx.x; j < L.count; j += 256)
+      atomicMax(&d.st[L.blk[j]].rounds, round + 1);
+    for (int e = threadIdx.x; e < total; e += 256) {
+      int s = find_seg(L, e);
+      int b = L.blk[s];
+      const BlockDesc& D = d.desc[b];
+      int m = d.st[b].m;
+      int j = L.start[s] + (e - L.lofs[s]);
+      d.KEY[D.off + j] = make_key(d, D.off, m, j, round, P);
+    }
+  } else {
+    const int t = w - npacks;
+    const int si = d.ltile_seg[t];
+    const Seg g = d.seg[cur][si];
+    const int pos = d.ltile_pos[t];
+    const uint64_t off = d.desc[g.blk].off;
+    const int m = d.st[g.blk].m;
+    if (pos == 0 && threadIdx.x == 0) atomicMax(&d.st[g.blk].rounds, round + 1);
+    const int n = min(kTile, g.len - pos);
+    for (int e = threadIdx.x; e < n; e += 256) {
+      int j = g.start + pos + e;
+      d.KEY[off + j] = make_key(d, off, m, j, round, P);
+    }
+  }
+}
+
+// Appends groups to the next list: each thread has 0/1 group to emit.
+__device__ inline void emit_groups(const Dev& d, int nxt, bool has, Seg g, int32_t* tmp, int32_t* s_base) {
+  int tot;
+  int ex = wg_excl_sum<256, int>(has ? 1 : 0, tmp, &tot);
+  if (threadIdx.x == 0 && tot > 0) *s_base = atomicAdd(&d.ctr[nxt], tot);
+  __syncthreads();
+  if (has) {
+    int idx = *s_base + ex;
+    if (idx < d.seg_cap) d.seg[nxt][idx] = g;
+    else d.ctr[CTR_ERR] = 1;
+  }
+  __syncthreads();
+}
+
+// ----------------------------------------------------------- small groups
+// One workgroup sorts a pack of <= kPack rows in LDS.
+__global__ void __launch_bounds__(256) k_sort_small(Dev d, int cur, int keybits) {
+  __shared__ PackLds L;
+  __shared__ uint64_t K0[kPack], K1[kPack];
+  __shared__ uint32_t V0[kPack], V1[kPack];
+  __shared__ uint32_t cnt[4][256];
+  __shared__ int32_t s_base;
+  __shared__ int32_t tmp[8];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
+  const int k = blockIdx.x;
+  const int nxt = cur ^ 1;
+  load_pack(d, cur, k, L);
+  if (L.count == 0) return;
+  const int total = L.total;
+  // load composite keys (group-in-pack << keybits | key) and SA values
+  for (int e = tid; e < kPack; e += 256) {
+    if (e < total) {
+      int s = find_seg(L, e);
+      uint64_t off = d.desc[L.blk[s]].off;
+      int j = L.start[s] + (e - L.lofs[s]);
+      K0[e] = ((uint64_t)s << keybits) | d.KEY[off + j];
+      V0[e] = d.SA[off + j];
+    }
+  }
+  __syncthreads();
+  const int nbits = keybits + bits_for((uint32_t)(L.count - 1));
+  const int npass = (nbits + 7) / 8;
+  uint64_t* Ks = K0; uint64_t* Kd = K1;
+  uint32_t* Vs = V0; uint32_t* Vd = V1;
+  for (int pass = 0; pass < npass; pass++) {
+    const int shift = pass * 8;
+    cnt[0][tid] = 0; cnt[1][tid] = 0; cnt[2][tid] = 0; cnt[3][tid] = 0;
+    __syncthreads();
+    uint32_t rk[8], dg[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      int e = wv * 512 + q * 64 + lane;
+      bool valid = e < total;
+      uint32_t digit = valid ? (uint32_t)(Ks[e] >> shift) & 255u : 0u;
+      dg[q] = digit;
+      rk[q] = wave_rank_digit(digit, valid, cnt[wv]);
+    }
+    __syncthreads();
+    {
+      uint32_t c0 = cnt[0][tid], c1 = cnt[1][tid], c2 = cnt[2][tid], c3 = cnt[3][tid];
+      int tot;
+      int base = wg_excl_sum<256, int>((int)(c0 + c1 + c2 + c3), tmp, &tot);
+      cnt[0][tid] = base;
+      cnt[1][tid] = base + c0;
+      cnt[2][tid] = base + c0 + c1;
+      cnt[3][tid] = base + c0 + c1 + c2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      int e = wv * 512 + q * 64 + lane;
+      if (e < total) {
+        uint32_t dst = cnt[wv][dg[q]] + rk[q];
+        Kd[dst] = Ks[e];
+        Vd[dst] = Vs[e];
+      }
+    }
+    __syncthreads();
+    uint64_t* tk = Ks; Ks = Kd; Kd = tk;
+    uint32_t* tv = Vs; Vs = Vd; Vd = tv;
+  }
+  // heads: thread owns rows [8*tid, 8*tid+8)
+  const int e0 = tid * 8;
+  int lastB = -1;
+  uint32_t bm = 0;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    int e = e0 + q;
+    if (e < total && (e == 0 || Ks[e] != Ks[e - 1])) { bm |= 1u << q; lastB = e; }
+  }
+  int incl = wg_incl_max<256, int>(lastB, tmp, -1);
+  __shared__ int32_t xs[256];
+  xs[tid] = incl;
+  __syncthreads();
+  int run = tid > 0 ? xs[tid - 1] : -1;
+  bool has = false;
+  Seg ng{0, 0, 0, 0};
+  int head_of[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    if (bm & (1u << q)) run = e0 + q;
+    head_of[q] = run;
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    int e = e0 + q;
+    if (e < total) {
+      int s = (int)(Ks[e] >> keybits);
+      int b = L.blk[s];
+      uint64_t off = d.desc[b].off;
+      int j = L.start[s] + (e - L.lofs[s]);
+      int hj = L.start[s] + (head_of[q] - L.lofs[s]);
+      uint32_t v = Vs[e];
+      d.SA[off + j] = v;
+      d.RK[off + v] = (uint32_t)hj;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    int e = e0 + q;
+    has = false;
+    if (e < total) {
+      bool end = (e == total - 1) || (Ks[e + 1] != Ks[e]);
+      int glen = e - head_of[q] + 1;
+      if (end && glen > 1) {
+        int s = (int)(Ks[e] >> keybits);
+        has = true;
+        ng = Seg{L.blk[s], L.start[s] + (head_of[q] - L.lofs[s]), glen, 0};
+      }
+    }
+    emit_groups(d, nxt, has, ng, tmp, &s_base);
+  }
+}
+
+// ----------------------------------------------------------- large groups
+__global__ void __launch_bounds__(256) k_lhist(Dev d, int cur, int shift, int src) {
+  __shared__ uint32_t h[256];
+  const int t = blockIdx.x;
+  const Seg g = d.seg[cur][d.ltile_seg[t]];
+  const int pos = d.ltile_pos[t];
+  const uint64_t off = d.desc[g.blk].off;
+  const uint32_t* K = src ? d.KEY2 : d.KEY;
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int n = min(kTile, g.len - pos);
+  for (int e = threadIdx.x; e < n; e += 256)
+    atomicAdd(&h[(K[off + g.start + pos + e] >> shift) & 255u], 1u);
+  __syncthreads();
+  d.hist[(int64_t)t * 256 + threadIdx.x] = h[threadIdx.x];
+}
+
+// per large group: hist[t][digit] -> absolute destination base
+__global__ void __launch_bounds__(256) k_lscan(Dev d, int cur) {
+  __shared__ int32_t tmp[8];
+  const int si = d.llist[blockIdx.x];
+  const Seg g = d.seg[cur][si];
+  const int t0 = d.pb[si], nt = tiles_for(g.len);
+  const int dg = threadIdx.x;
+  uint32_t run = 0;
+  for (int t = 0; t < nt; t++) {
+    uint32_t* p = &d.hist[(int64_t)(t0 + t) * 256 + dg];
+    uint32_t v = *p;
+    *p = run;
+    run += v;
+  }
+  int tot;
+  int base = wg_excl_sum<256, int>((int)run, tmp, &tot);
+  uint32_t add = (uint32_t)(g.start + base);
+  for (int t = 0; t < nt; t++) d.hist[(int64_t)(t0 + t) * 256 + dg] += add;
+}
+
+__global__ void __launch_bounds__(256) k_lscatter(Dev d, int cur, int shift, int src) {
+  __shared__ uint32_t cnt[4][256];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
+  const int t = blockIdx.x;
+  const Seg g = d.seg[cur][d.ltile_seg[t]];
+  const int pos = d.ltile_pos[t];
+  const uint64_t off = d.desc[g.blk].off;
+  const uint32_t* Ks = src ? d.KEY2 : d.KEY;
+  const uint32_t* Ss = src ? d.SA2 : d.SA;
+  uint32_t* Kd = src ? d.KEY : d.KEY2;
+  uint32_t* Sd = src ? d.SA : d.SA2;
+  const int n = min(kTile, g.len - pos);
+  const uint64_t base = off + g.start + pos;
+  cnt[0][tid] = 0; cnt[1][tid] = 0; cnt[2][tid] = 0; cnt[3][tid] = 0;
+  __syncthreads();
+  uint32_t key[16], val[16], rk[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    int e = wv * 1024 + q * 64 + lane;
+    bool valid = e < n;
+    key[q] = valid ? Ks[base + e] : 0u;
+    val[q] = valid ? Ss[base + e] : 0u;
+    rk[q] = wave_rank_digit((key[q] >> shift) & 255u, valid, cnt[wv]);
+  }
+  __syncthreads();
+  {
+    uint32_t c0 = cnt[0][tid], c1 = cnt[1][tid], c2 = cnt[2][tid];
+    uint32_t gb = d.hist[(int64_t)t * 256 + tid];
+    cnt[0][tid] = gb;
+    cnt[1][tid] = gb + c0;
+    cnt[2][tid] = gb + c0 + c1;
+    cnt[3][tid] = gb + c0 + c1 + c2;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    int e = wv * 1024 + q * 64 + lane;
+    if (e < n) {
+      uint32_t dst = cnt[wv][(key[q] >> shift) & 255u] + rk[q];
+      Kd[off + dst] = key[q];
+      Sd[off + dst] = val[q];
+    }
+  }
+}
+
+// first / last group boundary (seg-relative) of each large tile
+__global__ void __launch_bounds__(256) k_lsum(Dev d, int cur, int fin) {
+  __shared__ int32_t tmp[8];
+  const int t = blockIdx.x;
+  const Seg g = d.seg[cur][d.ltile_seg[t]];
+  const int pos = d.ltile_pos[t];
+  const uint64_t base = d.desc[g.blk].off + g.start;
+  const uint32_t* K = fin ? d.KEY2 : d.KEY;
+  const int n = min(kTile, g.len - pos);
+  int fb = 0x7fffffff, lb = -1;
+  for (int e = threadIdx.x; e < n; e += 256) {
+    int p = pos + e;
+    if (p == 0 || K[base + p] != K[base + p - 1]) { fb = min(fb, p); lb = max(lb, p); }
+  }
+  fb = wave_min(fb);
+  lb = wave_max(lb);
+  if (lane_id() == 0) { tmp[threadIdx.x >> 6] = fb; tmp[4 + (threadIdx.x >> 6)] = lb; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int f = tmp[0], l2 = tmp[4];
+    for (int w = 1; w < 4; w++) { f = min(f, tmp[w]); l2 = max(l2, tmp[4 + w]); }
+    d.lsum[2 * t] = f;
+    d.lsum[2 * t + 1] = l2;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
+  __shared__ int32_t tmp[8];
+  __shared__ int32_t s_carry, s_base;
+  __shared__ int32_t xs[256];
+  const int tid = threadIdx.x;
+  const int t = blockIdx.x;
+  const int si = d.ltile_seg[t];
+  const Seg g = d.seg[cur][si];
+  const int pos = d.ltile_pos[t];
+  const int kt = pos / kTile;           // tile index within the group
+  const int t0 = t - kt;
+  const uint64_t off = d.desc[g.blk].off;
+  const uint64_t base = off + g.start;
+  const uint32_t* K = fin ? d.KEY2 : d.KEY;
+  const uint32_t* S = fin ? d.SA2 : d.SA;
+  const int n = min(kTile, g.len - pos);
+  const int nxt = cur ^ 1;
+  // carry: last boundary before this tile (largest lb of earlier tiles)
+  {
+    int lbv = -1;
+    for (int j = kt - 1 - tid; j >= 0; j -= 256) {
+      int v = d.lsum[2 * (t0 + j) + 1];
+      if (v >= 0) { lbv = v; break; }
+    }
+    lbv = wave_max(lbv);
+    if (lane_id() == 0) tmp[tid >> 6] = lbv;
+    __syncthreads();
+    if (tid == 0) s_carry = max(max(tmp[0], tmp[1]), max(tmp[2], tmp[3]));
+    __syncthreads();
+  }
+  // thread owns rows [16*tid, 16*tid+16) of the tile
+  const int e0 = tid * 16;
+  uint32_t kk[17];
+  uint32_t bm = 0;
+  int lastB = -1;
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    int e = e0 + q;
+    int p = pos + e;
+    kk[q] = e < n ? K[base + p] : 0u;
+    if (e < n) {
+      uint32_t prev = q > 0 ? kk[q - 1] : (p > 0 ? K[base + p - 1] : 0u);
+      if (p == 0 || kk[q] != prev) { bm |= 1u << q; lastB = p; }
+    }
+  }
+  {
+    int e = e0 + 16;
+    int p = pos + e;
+    kk[16] = (e0 + 15 < n && p < g.len) ? K[base + p] : 0u;
+  }
+  int incl = wg_incl_max<256, int>(lastB, tmp, s_carry);
+  xs[tid] = incl;
+  __syncthreads();
+  int run = tid > 0 ? xs[tid - 1] : s_carry;
+  run = max(run, s_carry);
+  bool has = false;
+  Seg ng{0, 0, 0, 0};
+  int head_of[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    if (bm & (1u << q)) run = pos + e0 + q;
+    head_of[q] = run;
+  }
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    int e = e0 + q;
+    if (e < n) {
+      int p = pos + e;
+      uint32_t v = S[base + p];
+      if (fin) d.SA[base + p] = v;
+      d.RK[off + v] = (uint32_t)(g.start + head_of[q]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    int e = e0 + q;
+    has = false;
+    if (e < n) {
+      int p = pos + e;
+      bool end = (p == g.len - 1) || (kk[q + 1] != kk[q]);
+      int glen = p - head_of[q] + 1;
+      if (end && glen > 1) {
+        has = true;
+        ng = Seg{g.blk, g.start + head_of[q], glen, 0};
+      }
+    }
+    emit_groups(d, nxt, has, ng, tmp, &s_base);
+  }
+}
+
+void launch_bwt_init(const Dev& d, hipStream_t s) {
+  k_bwt_init<<<d.ntile_c, 256, 0, s>>>(d);
+}
+
+void launch_bwt_round(const Dev& d, int cur, int round, int32_t P, int key_bits,
+                      int32_t npacks, int32_t ntiles, int32_t nlarge, int32_t nseg,
+                      hipStream_t s) {
+  k_mk_packs<<<(npacks + 1 + 255) / 256, 256, 0, s>>>(d, npacks, nseg);
+  k_mk_large<<<(nseg + 255) / 256, 256, 0, s>>>(d, cur, nseg);
+  if (npacks + ntiles > 0) k_keys<<<npacks + ntiles, 256, 0, s>>>(d, cur, round, P, npacks);
+  if (npacks > 0) k_sort_small<<<npacks, 256, 0, s>>>(d, cur, key_bits);
+  if (ntiles > 0) {
+    const int npass = (key_bits + 7) / 8;
+    for (int p = 0; p < npass; p++) {
+      k_lhist<<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
+      k_lscan<<<nlarge, 256, 0, s>>>(d, cur);
+      k_lscatter<<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
+    }
+    const int fin = npass & 1;
+    k_lsum<<<ntiles, 256, 0, s>>>(d, cur, fin);
+    k_lheads<<<ntiles, 256, 0, s>>>(d, cur, fin);
+  }
+}
+
+}  // namespace bzg

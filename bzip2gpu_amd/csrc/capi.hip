@@ -1,0 +1,554 @@
+// capi.hip - the extern "C" boundary (include/bzg.h) over the batch engine.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/bzg.h"
+#include "engine.h"
+
+using namespace bzg;
+
+namespace {
+
+thread_local std::string g_err;
+thread_local int g_dev = 0;
+
+struct Ctx {
+  explicit Ctx(int dev) : eng(dev), stage(dev), device(dev) {}
+  Engine eng;      // batched / plain paths
+  Engine stage;    // per-stage entry points
+  int device;
+  std::mutex mu;
+  uint8_t* d_in = nullptr;
+  size_t d_in_cap = 0;
+  hipStream_t stream = nullptr;
+  StageTimes last{};
+};
+
+std::mutex g_mu;
+std::unique_ptr<Ctx> g_ctx[64];
+
+bzg_status fail(bzg_status code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+bzg_status hip_fail(int e, const char* where) {
+  if (e == (int)hipErrorIllegalState)
+    return fail(BZG_ERR_INTERNAL, std::string(where) + ": device-side invariant failed");
+  return fail(BZG_ERR_HIP, std::string(where) + ": " + hipGetErrorString((hipError_t)e));
+}
+
+Ctx* ctx_for(int dev, bzg_status* st) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    *st = fail(BZG_ERR_HIP, "no HIP device available");
+    return nullptr;
+  }
+  if (dev < 0 || dev >= n || dev >= 64) {
+    *st = fail(BZG_ERR_INVALID, "device index out of range");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_ctx[dev]) {
+    if (hipSetDevice(dev) != hipSuccess) {
+      *st = fail(BZG_ERR_HIP, "hipSetDevice failed");
+      return nullptr;
+    }
+    g_ctx[dev].reset(new Ctx(dev));
+    if (hipStreamCreateWithFlags(&g_ctx[dev]->stream, hipStreamNonBlocking) != hipSuccess) {
+      g_ctx[dev].reset();
+      *st = fail(BZG_ERR_HIP, "hipStreamCreate failed");
+      return nullptr;
+    }
+  }
+  *st = BZG_OK;
+  return g_ctx[dev].get();
+}
+
+int ensure_input(Ctx* c, size_t bytes) {
+  if (bytes <= c->d_in_cap) return 0;
+  if (c->d_in) {
+    hipError_t e = hipFree(c->d_in);
+    if (e != hipSuccess) return (int)e;
+  }
+  c->d_in = nullptr;
+  c->d_in_cap = 0;
+  size_t want = bytes + bytes / 8 + 256;
+  hipError_t e = hipMalloc(&c->d_in, want);
+  if (e != hipSuccess) return (int)e;
+  c->d_in_cap = want;
+  return 0;
+}
+
+// MSB-first host bit writer (the reference's BitWriter, compress.cu:26-58).
+struct HostBits {
+  std::vector<uint8_t>& out;
+  uint64_t buf = 0;
+  int n = 0;
+  explicit HostBits(std::vector<uint8_t>& o) : out(o) {}
+  void put(uint32_t v, int k) {
+    if (k == 0) return;
+    v &= (uint32_t)((1ull << k) - 1);
+    buf = (buf << k) | v;
+    n += k;
+    while (n >= 8) {
+      n -= 8;
+      out.push_back((uint8_t)(buf >> n));
+    }
+  }
+  void put_words(const std::vector<uint32_t>& w, int64_t bits) {
+    int64_t rem = bits;
+    for (size_t i = 0; i < w.size() && rem > 0; i++) {
+      int take = rem < 32 ? (int)rem : 32;
+      put(w[i] >> (32 - take), take);
+      rem -= take;
+    }
+  }
+  void pad() {
+    if (n > 0) put(0, 8 - n);
+  }
+};
+
+int clamp_level(int level) { return level < 1 ? 1 : (level > 9 ? 9 : level); }
+
+}  // namespace
+
+extern "C" {
+
+const char* bzg_version(void) { return "bzip2gpu_amd 0.1 (gfx950)"; }
+
+const char* bzg_last_error(void) { return g_err.c_str(); }
+
+bzg_status bzg_init(int device) {
+  bzg_status st;
+  Ctx* c = ctx_for(device, &st);
+  if (!c) return st;
+  g_dev = device;
+  return BZG_OK;
+}
+
+size_t bzg_compress_bound(size_t len, int32_t block_bytes) {
+  if (block_bytes <= 0 || block_bytes > BZG_REF_BLOCK_BYTES) block_bytes = BZG_REF_BLOCK_BYTES;
+  size_t nb = (len + (size_t)block_bytes - 1) / (size_t)block_bytes;
+  int32_t cap = cap_for(block_bytes);
+  size_t per = (size_t)(hdr_words_for(cap) + pay_words_for(cap)) * 4;
+  return nb * per + 64;
+}
+
+bzg_status bzg_compress_buffer(const uint8_t* h_in, size_t len, int level, int32_t block_bytes,
+                               uint8_t* h_out, size_t out_cap, size_t* out_len) {
+  if ((!h_in && len) || !h_out || !out_len) return fail(BZG_ERR_INVALID, "null pointer");
+  if (block_bytes <= 0 || block_bytes > BZG_REF_BLOCK_BYTES) return fail(BZG_ERR_INVALID, "block_bytes out of range");
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(BZG_ERR_HIP, "hipSetDevice failed");
+  int e = ensure_input(c, len + 16);
+  if (e) return hip_fail(e, "input buffer");
+  if (len) {
+    hipError_t he = hipMemcpyAsync(c->d_in, h_in, len, hipMemcpyHostToDevice, c->stream);
+    if (he != hipSuccess) return hip_fail((int)he, "H2D");
+  }
+  std::vector<StreamSpec> ss{StreamSpec{0, len, clamp_level(level), block_bytes}};
+  if ((e = c->eng.plan(ss))) return hip_fail(e, "plan");
+  if ((e = c->eng.setup(c->d_in, c->stream))) return hip_fail(e, "setup");
+  if ((e = c->eng.run(c->stream, true, &c->last))) return hip_fail(e, "run");
+  std::vector<StreamState> lay;
+  if ((e = c->eng.stream_layout(lay, c->stream))) return hip_fail(e, "layout");
+  size_t n = (size_t)lay[0].bytes;
+  *out_len = n;
+  if (n > out_cap) return fail(BZG_ERR_CAPACITY, "output buffer too small");
+  hipError_t he = hipMemcpyAsync(h_out, c->eng.out_buffer() + lay[0].byte_off, n, hipMemcpyDeviceToHost, c->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+  if (he != hipSuccess) return hip_fail((int)he, "D2H");
+  return BZG_OK;
+}
+
+bzg_status bzg_compress_plain(const uint8_t* h_in, size_t len, int level, int32_t block_bytes,
+                              uint8_t* h_out, size_t out_cap, size_t* out_len) {
+  if ((!h_in && len) || !h_out || !out_len) return fail(BZG_ERR_INVALID, "null pointer");
+  if (block_bytes <= 0 || block_bytes > BZG_REF_BLOCK_BYTES) return fail(BZG_ERR_INVALID, "block_bytes out of range");
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(BZG_ERR_HIP, "hipSetDevice failed");
+  level = clamp_level(level);
+  std::vector<uint8_t> out;
+  out.reserve(len / 2 + 64);
+  HostBits bw(out);
+  bw.put(0x42, 8); bw.put(0x5A, 8); bw.put(0x68, 8); bw.put((uint32_t)(0x30 + level), 8);
+  uint32_t combined = 0;
+  int e = ensure_input(c, (size_t)block_bytes + 16);
+  if (e) return hip_fail(e, "input buffer");
+  BlockResult r;
+  for (size_t off = 0; off < len; off += (size_t)block_bytes) {
+    size_t n = len - off < (size_t)block_bytes ? len - off : (size_t)block_bytes;
+    hipError_t he = hipMemcpyAsync(c->d_in, h_in + off, n, hipMemcpyHostToDevice, c->stream);
+    if (he != hipSuccess) return hip_fail((int)he, "H2D");
+    std::vector<StreamSpec> ss{StreamSpec{0, n, level, block_bytes}};
+    if ((e = c->eng.plan(ss))) return hip_fail(e, "plan");
+    if ((e = c->eng.setup(c->d_in, c->stream))) return hip_fail(e, "setup");
+    if ((e = c->eng.run(c->stream, false, nullptr))) return hip_fail(e, "run");
+    if ((e = c->eng.block_result(0, r, c->stream))) return hip_fail(e, "block result");
+    combined = ((combined << 1) | (combined >> 31)) ^ r.crc;
+    bw.put_words(r.hdr, r.hdr_bits);
+    bw.put_words(r.words, r.bits);
+  }
+  bw.put(0x17, 8); bw.put(0x72, 8); bw.put(0x45, 8); bw.put(0x38, 8); bw.put(0x50, 8); bw.put(0x90, 8);
+  bw.put(combined, 32);
+  bw.pad();
+  *out_len = out.size();
+  if (out.size() > out_cap) return fail(BZG_ERR_CAPACITY, "output buffer too small");
+  memcpy(h_out, out.data(), out.size());
+  return BZG_OK;
+}
+
+bzg_status bzg_compress_batch_device(const uint8_t* d_in, int32_t n_inputs, const uint64_t* in_off,
+                                     const uint64_t* lens, const int32_t* levels, int32_t block_bytes,
+                                     uint8_t** d_out, int64_t* h_byte_off, int64_t* h_bytes, void* stream) {
+  if (n_inputs < 0 || (n_inputs > 0 && (!d_in || !in_off || !lens || !levels)) || !d_out || !h_byte_off || !h_bytes)
+    return fail(BZG_ERR_INVALID, "null pointer");
+  if (block_bytes > BZG_REF_BLOCK_BYTES) return fail(BZG_ERR_INVALID, "block_bytes out of range");
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(BZG_ERR_HIP, "hipSetDevice failed");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  std::vector<StreamSpec> ss;
+  ss.reserve(n_inputs);
+  for (int i = 0; i < n_inputs; i++) {
+    int lv = clamp_level(levels[i]);
+    ss.push_back(StreamSpec{in_off[i], lens[i], lv, block_bytes > 0 ? block_bytes : 100000 * lv});
+  }
+  int e;
+  if ((e = c->eng.plan(ss))) return hip_fail(e, "plan");
+  if ((e = c->eng.setup(d_in, s))) return hip_fail(e, "setup");
+  if ((e = c->eng.run(s, true, &c->last))) return hip_fail(e, "run");
+  std::vector<StreamState> lay;
+  if ((e = c->eng.stream_layout(lay, s))) return hip_fail(e, "layout");
+  for (int i = 0; i < n_inputs; i++) {
+    h_byte_off[i] = lay[i].byte_off;
+    h_bytes[i] = lay[i].bytes;
+  }
+  *d_out = c->eng.out_buffer();
+  return BZG_OK;
+}
+
+bzg_status bzg_last_stage_times(float* h_ms6, int32_t* h_rounds) {
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  if (h_ms6) {
+    h_ms6[0] = c->last.front; h_ms6[1] = c->last.bwt; h_ms6[2] = c->last.mtf;
+    h_ms6[3] = c->last.huff; h_ms6[4] = c->last.pack; h_ms6[5] = c->last.assemble;
+  }
+  if (h_rounds) *h_rounds = c->last.rounds;
+  return BZG_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ stage level
+namespace {
+
+// first HIP error of a sequence of enqueue calls
+struct Ck {
+  int e = 0;
+  void operator()(hipError_t x) {
+    if (!e && x != hipSuccess) e = (int)x;
+  }
+};
+thread_local Ck ck;
+
+struct StageLock {
+  Ctx* c = nullptr;
+  std::unique_lock<std::mutex> lk;
+  hipStream_t s = nullptr;
+};
+
+bzg_status stage_begin(StageLock& L, void* stream, int32_t len) {
+  ck.e = 0;
+  bzg_status st;
+  L.c = ctx_for(g_dev, &st);
+  if (!L.c) return st;
+  L.lk = std::unique_lock<std::mutex>(L.c->mu);
+  if (hipSetDevice(L.c->device) != hipSuccess) return fail(BZG_ERR_HIP, "hipSetDevice failed");
+  L.s = stream ? (hipStream_t)stream : L.c->stream;
+  int e = L.c->stage.setup_single(len < 1 ? 1 : len, L.s);
+  if (e) return hip_fail(e, "stage setup");
+  return BZG_OK;
+}
+
+template <typename T>
+int set_field(const Dev& d, size_t offset, T v, hipStream_t s) {
+  return (int)hipMemcpyAsync(reinterpret_cast<char*>(d.st) + offset, &v, sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+int sync(hipStream_t s) {
+  int e = ck.e;
+  ck.e = 0;
+  int r = (int)hipStreamSynchronize(s);
+  return e ? e : r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t bzg_rle1_bound(int32_t n) { return n < 0 ? 0 : cap_for(n); }
+
+bzg_status bzg_crc32(const uint8_t* d_in, int32_t n, uint32_t* h_crc, void* stream) {
+  if (!h_crc || n < 0 || (n > 0 && !d_in)) return fail(BZG_ERR_INVALID, "bad argument");
+  if (n > kStageMax) return fail(BZG_ERR_INVALID, "n exceeds the stage limit (16 MiB)");
+  if (n == 0) { *h_crc = 0; return BZG_OK; }
+  StageLock L;
+  bzg_status st = stage_begin(L, stream, n);
+  if (st) return st;
+  Engine& E = L.c->stage;
+  std::vector<StreamSpec> ss{StreamSpec{0, (uint64_t)n, 9, n}};
+  int e;
+  if ((e = E.plan(ss, true)) || (e = E.setup(d_in, L.s))) return hip_fail(e, "stage setup");
+  if ((e = E.run_front(L.s))) return hip_fail(e, "crc");
+  uint32_t crc;
+  ck(hipMemcpyAsync(&crc, &E.dev().st[0].crc, 4, hipMemcpyDeviceToHost, L.s));
+  if ((e = sync(L.s))) return hip_fail(e, "crc");
+  *h_crc = crc;
+  return BZG_OK;
+}
+
+bzg_status bzg_rle1(const uint8_t* d_in, int32_t n, uint8_t* d_out, int32_t* h_out_len, void* stream) {
+  if (!h_out_len || n < 0 || (n > 0 && (!d_in || !d_out))) return fail(BZG_ERR_INVALID, "bad argument");
+  if (n > kStageMax) return fail(BZG_ERR_INVALID, "n exceeds the stage limit (16 MiB)");
+  if (n == 0) { *h_out_len = 0; return BZG_OK; }  // rle1.cu:61-62
+  StageLock L;
+  bzg_status st = stage_begin(L, stream, n);
+  if (st) return st;
+  Engine& E = L.c->stage;
+  std::vector<StreamSpec> ss{StreamSpec{0, (uint64_t)n, 9, n}};
+  int e;
+  if ((e = E.plan(ss, true)) || (e = E.setup(d_in, L.s))) return hip_fail(e, "stage setup");
+  if ((e = E.run_front(L.s))) return hip_fail(e, "rle1");
+  int32_t m;
+  ck(hipMemcpyAsync(&m, &E.dev().st[0].m, 4, hipMemcpyDeviceToHost, L.s));
+  if ((e = sync(L.s))) return hip_fail(e, "rle1");
+  if (m > 0) ck(hipMemcpyAsync(d_out, E.dev().T, (size_t)m, hipMemcpyDeviceToDevice, L.s));
+  if ((e = sync(L.s))) return hip_fail(e, "rle1");
+  *h_out_len = m;
+  return BZG_OK;
+}
+
+bzg_status bzg_bwt(const uint8_t* d_in, int32_t m, int32_t* d_sa, int32_t* h_rounds, void* stream) {
+  if (m < 0 || (m > 0 && (!d_in || !d_sa))) return fail(BZG_ERR_INVALID, "bad argument");
+  if (m > kStageMax) return fail(BZG_ERR_INVALID, "m exceeds the stage limit (16 MiB)");
+  if (m == 0) { if (h_rounds) *h_rounds = 0; return BZG_OK; }  // fbwt.cu:48-49
+  StageLock L;
+  bzg_status st = stage_begin(L, stream, m);
+  if (st) return st;
+  Engine& E = L.c->stage;
+  const Dev& d = E.dev();
+  int e;
+  ck(hipMemcpyAsync(d.T, d_in, (size_t)m, hipMemcpyDeviceToDevice, L.s));
+  if ((e = set_field<int32_t>(d, offsetof(BlockState, m), m, L.s))) return hip_fail(e, "bwt");
+  int rounds = 0;
+  if ((e = E.run_bwt(L.s, &rounds))) return hip_fail(e, "bwt");
+  ck(hipMemcpyAsync(d_sa, d.SA, (size_t)m * 4, hipMemcpyDeviceToDevice, L.s));
+  if ((e = sync(L.s))) return hip_fail(e, "bwt");
+  if (h_rounds) *h_rounds = rounds;
+  return BZG_OK;
+}
+
+bzg_status bzg_symbols_table(const uint8_t* d_in, int32_t n, uint8_t* h_table, int32_t* h_count, void* stream) {
+  if (!h_table || !h_count || n < 0 || (n > 0 && !d_in)) return fail(BZG_ERR_INVALID, "bad argument");
+  if (n > kStageMax) return fail(BZG_ERR_INVALID, "n exceeds the stage limit (16 MiB)");
+  StageLock L;
+  bzg_status st = stage_begin(L, stream, n);
+  if (st) return st;
+  Engine& E = L.c->stage;
+  const Dev& d = E.dev();
+  int e;
+  if (n) ck(hipMemcpyAsync(d.T, d_in, (size_t)n, hipMemcpyDeviceToDevice, L.s));
+  if ((e = set_field<int32_t>(d, offsetof(BlockState, m), n, L.s))) return hip_fail(e, "symbols");
+  launch_presence(d, L.s);
+  uint32_t pres[8];
+  ck(hipMemcpyAsync(pres, d.st[0].present, 32, hipMemcpyDeviceToHost, L.s));
+  if ((e = sync(L.s))) return hip_fail(e, "symbols");
+  int cnt = 0;
+  for (int i = 0; i < 256; i++)
+    if ((pres[i >> 5] >> (i & 31)) & 1u) h_table[cnt++] = (uint8_t)i;
+  *h_count = cnt;
+  return BZG_OK;
+}
+
+bzg_status bzg_mtf(const uint8_t* d_rle1, const int32_t* d_sa, int32_t m, uint8_t* d_out, int32_t* h_orig_ptr,
+                   uint8_t* h_present, int32_t* h_n_in_use, void* stream) {
+  if (m < 0 || (m > 0 && (!d_rle1 || !d_sa || !d_out))) return fail(BZG_ERR_INVALID, "bad argument");
+  if (m > kStageMax) return fail(BZG_ERR_INVALID, "m exceeds the stage limit (16 MiB)");
+  if (m == 0) {  // fmtf.cu:150-151
+    if (h_n_in_use) *h_n_in_use = 0;
+    return BZG_OK;
+  }
+  StageLock L;
+  bzg_status st = stage_begin(L, stream, m);
+  if (st) return st;
+  Engine& E = L.c->stage;
+  const Dev& d = E.dev();
+  int e;
+  ck(hipMemcpyAsync(d.T, d_rle1, (size_t)m, hipMemcpyDeviceToDevice, L.s));
+  ck(hipMemcpyAsync(d.SA, d_sa, (size_t)m * 4, hipMemcpyDeviceToDevice, L.s));
+  if ((e = set_field<int32_t>(d, offsetof(BlockState, m), m, L.s))) return hip_fail(e, "mtf");
+  launch_presence(d, L.s);
+  launch_mtf(d, L.s);
+  if ((e = (int)hipGetLastError())) return hip_fail(e, "mtf");
+  ck(hipMemcpyAsync(d_out, d.MTF, (size_t)m, hipMemcpyDeviceToDevice, L.s));
+  BlockState bs;
+  ck(hipMemcpyAsync(&bs, d.st, sizeof bs, hipMemcpyDeviceToHost, L.s));
+  if ((e = sync(L.s))) return hip_fail(e, "mtf");
+  if (h_orig_ptr) *h_orig_ptr = bs.orig_ptr;
+  if (h_n_in_use) *h_n_in_use = bs.n_in_use;
+  if (h_present)
+    for (int i = 0; i < 256; i++) h_present[i] = (bs.present[i >> 5] >> (i & 31)) & 1u;
+  return BZG_OK;
+}
+
+bzg_status bzg_rle2(const uint8_t* d_mtf, int32_t m, int32_t table_size, uint16_t* d_out, uint32_t* d_out_len,
+                    void* stream) {
+  if (m < 0 || !d_out_len || (m > 0 && (!d_mtf || !d_out))) return fail(BZG_ERR_INVALID, "bad argument");
+  if (m > kStageMax || table_size < 0 || table_size > 256) return fail(BZG_ERR_INVALID, "size out of range");
+  StageLock L;
+  if (m == 0) {  // rle2.cu:106-110
+    bzg_status st;
+    Ctx* c = ctx_for(g_dev, &st);
+    if (!c) return st;
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    ck(hipMemsetAsync(d_out_len, 0, 4, s));
+    return sync(s) ? fail(BZG_ERR_HIP, "rle2") : BZG_OK;
+  }
+  bzg_status st = stage_begin(L, stream, m + 1);
+  if (st) return st;
+  Engine& E = L.c->stage;
+  const Dev& d = E.dev();
+  int e;
+  ck(hipMemcpyAsync(d.MTF, d_mtf, (size_t)m, hipMemcpyDeviceToDevice, L.s));
+  ck((hipError_t)set_field<int32_t>(d, offsetof(BlockState, m), m, L.s));
+  if ((e = set_field<int32_t>(d, offsetof(BlockState, n_in_use), table_size, L.s))) return hip_fail(e, "rle2");
+  launch_fnz(d, L.s);
+  launch_rle2(d, L.s);
+  if ((e = (int)hipGetLastError())) return hip_fail(e, "rle2");
+  int32_t s_len;
+  ck(hipMemcpyAsync(&s_len, &d.st[0].s, 4, hipMemcpyDeviceToHost, L.s));
+  if ((e = sync(L.s))) return hip_fail(e, "rle2");
+  ck(hipMemcpyAsync(d_out, d.SYM, (size_t)s_len * 2, hipMemcpyDeviceToDevice, L.s));
+  ck(hipMemcpyAsync(d_out_len, &d.st[0].s, 4, hipMemcpyDeviceToDevice, L.s));
+  if ((e = sync(L.s))) return hip_fail(e, "rle2");
+  return BZG_OK;
+}
+
+bzg_status bzg_histogram(const uint16_t* d_sym, int32_t s, int32_t* d_freq, void* stream) {
+  if (s < 0 || !d_freq || (s > 0 && !d_sym)) return fail(BZG_ERR_INVALID, "bad argument");
+  if (s > kStageMax) return fail(BZG_ERR_INVALID, "s exceeds the stage limit (16 MiB)");
+  if (s == 0) return BZG_OK;
+  StageLock L;
+  bzg_status st = stage_begin(L, stream, s);
+  if (st) return st;
+  Engine& E = L.c->stage;
+  const Dev& d = E.dev();
+  int e;
+  ck(hipMemcpyAsync(d.SYM, d_sym, (size_t)s * 2, hipMemcpyDeviceToDevice, L.s));
+  ck((hipError_t)set_field<int32_t>(d, offsetof(BlockState, s), s, L.s));
+  launch_sym_hist(d, L.s);
+  std::vector<int32_t> h(258), add(258);
+  ck(hipMemcpyAsync(add.data(), d.freq, 258 * 4, hipMemcpyDeviceToHost, L.s));
+  ck(hipMemcpyAsync(h.data(), d_freq, 258 * 4, hipMemcpyDeviceToHost, L.s));
+  if ((e = sync(L.s))) return hip_fail(e, "histogram");
+  for (int i = 0; i < 258; i++) h[i] += add[i];
+  ck(hipMemcpyAsync(d_freq, h.data(), 258 * 4, hipMemcpyHostToDevice, L.s));
+  if ((e = sync(L.s))) return hip_fail(e, "histogram");
+  return BZG_OK;
+}
+
+bzg_status bzg_huff_build(const uint16_t* d_sym, int32_t s, int32_t alphabet_size, uint8_t* h_len, int32_t* h_code,
+                          uint8_t* h_sel, int32_t* h_n_groups, int32_t* h_n_sel, void* stream) {
+  if (s <= 0 || !d_sym || !h_len || !h_code || !h_sel || !h_n_groups || !h_n_sel)
+    return fail(BZG_ERR_INVALID, "bad argument");
+  if (s > kStageMax || alphabet_size < 0 || alphabet_size > 256) return fail(BZG_ERR_INVALID, "size out of range");
+  StageLock L;
+  bzg_status st = stage_begin(L, stream, s);
+  if (st) return st;
+  Engine& E = L.c->stage;
+  const Dev& d = E.dev();
+  int e;
+  ck(hipMemcpyAsync(d.SYM, d_sym, (size_t)s * 2, hipMemcpyDeviceToDevice, L.s));
+  ck((hipError_t)set_field<int32_t>(d, offsetof(BlockState, s), s, L.s));
+  ck((hipError_t)set_field<int32_t>(d, offsetof(BlockState, n_in_use), alphabet_size, L.s));
+  launch_sym_hist(d, L.s);
+  launch_huffman(d, L.s);
+  if ((e = (int)hipGetLastError())) return hip_fail(e, "huff_build");
+  BlockState bs;
+  std::vector<uint8_t> ln(6 * 260);
+  std::vector<uint32_t> cd(6 * 260);
+  ck(hipMemcpyAsync(&bs, d.st, sizeof bs, hipMemcpyDeviceToHost, L.s));
+  ck(hipMemcpyAsync(ln.data(), d.len, ln.size(), hipMemcpyDeviceToHost, L.s));
+  ck(hipMemcpyAsync(cd.data(), d.code, cd.size() * 4, hipMemcpyDeviceToHost, L.s));
+  if ((e = sync(L.s))) return hip_fail(e, "huff_build");
+  ck(hipMemcpyAsync(h_sel, d.sel, (size_t)bs.n_sel, hipMemcpyDeviceToHost, L.s));
+  if ((e = sync(L.s))) return hip_fail(e, "huff_build");
+  for (int g = 0; g < 6; g++)
+    for (int v = 0; v < 258; v++) {
+      h_len[g * 258 + v] = ln[g * 260 + v];
+      h_code[g * 258 + v] = (int32_t)cd[g * 260 + v];
+    }
+  *h_n_groups = bs.n_groups;
+  *h_n_sel = bs.n_sel;
+  return BZG_OK;
+}
+
+bzg_status bzg_huff_encode(const uint16_t* d_sym, int32_t s, const uint8_t* h_len, const int32_t* h_code,
+                           const uint8_t* d_sel, int32_t n_sel, int32_t n_groups, uint32_t* d_words, int64_t* h_bits,
+                           void* stream) {
+  if (s <= 0 || !d_sym || !h_len || !h_code || !d_sel || !d_words || !h_bits) return fail(BZG_ERR_INVALID, "bad argument");
+  if (s > kStageMax || n_groups < 1 || n_groups > 6 || n_sel != (s + 49) / 50)
+    return fail(BZG_ERR_INVALID, "size out of range");
+  for (int g = 0; g < n_groups; g++)
+    for (int v = 0; v < 258; v++)
+      if (h_len[g * 258 + v] > 20) return fail(BZG_ERR_INVALID, "code length out of range");
+  StageLock L;
+  bzg_status st = stage_begin(L, stream, s);
+  if (st) return st;
+  Engine& E = L.c->stage;
+  const Dev& d = E.dev();
+  int e;
+  std::vector<uint8_t> ln(6 * 260, 0);
+  std::vector<uint32_t> cd(6 * 260, 0);
+  for (int g = 0; g < 6; g++)
+    for (int v = 0; v < 258; v++) {
+      ln[g * 260 + v] = h_len[g * 258 + v];
+      cd[g * 260 + v] = (uint32_t)h_code[g * 258 + v];
+    }
+  ck(hipMemcpyAsync(d.SYM, d_sym, (size_t)s * 2, hipMemcpyDeviceToDevice, L.s));
+  ck(hipMemcpyAsync(d.sel, d_sel, (size_t)n_sel, hipMemcpyDeviceToDevice, L.s));
+  ck(hipMemcpyAsync(d.len, ln.data(), ln.size(), hipMemcpyHostToDevice, L.s));
+  ck(hipMemcpyAsync(d.code, cd.data(), cd.size() * 4, hipMemcpyHostToDevice, L.s));
+  ck((hipError_t)set_field<int32_t>(d, offsetof(BlockState, s), s, L.s));
+  ck((hipError_t)set_field<int32_t>(d, offsetof(BlockState, n_sel), n_sel, L.s));
+  ck((hipError_t)set_field<int32_t>(d, offsetof(BlockState, n_groups), n_groups, L.s));
+  launch_pack(d, L.s);
+  if ((e = (int)hipGetLastError())) return hip_fail(e, "huff_encode");
+  int64_t bits;
+  ck(hipMemcpyAsync(&bits, &d.st[0].bits, 8, hipMemcpyDeviceToHost, L.s));
+  if ((e = sync(L.s))) return hip_fail(e, "huff_encode");
+  ck(hipMemcpyAsync(d_words, d.pay, (size_t)((bits + 31) / 32) * 4, hipMemcpyDeviceToDevice, L.s));
+  if ((e = sync(L.s))) return hip_fail(e, "huff_encode");
+  *h_bits = bits;
+  return BZG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+// engine.h - host side of the batch pipeline: plans a batch of blocks,
+// owns the device workspace and enqueues the stage kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "pipeline.h"
+
+namespace bzg {
+
+// One output stream: raw bytes [in_off, in_off+len) of the device input.
+struct StreamSpec {
+  uint64_t in_off;
+  uint64_t len;
+  int level;        // header digit 1..9
+  int block_bytes;  // raw bytes per block (900000 = the reference's cut)
+};
+
+// Per-block result copied back to the host (the reference's BlockData,
+// src/compression/compress.cu:13-24).
+struct BlockResult {
+  uint32_t crc;
+  int32_t m, s, orig_ptr, n_in_use, n_groups, n_sel, rounds;
+  uint8_t present[256];
+  int64_t bits, hdr_bits;
+  std::vector<uint32_t> hdr;    // header words (hdr_bits)
+  std::vector<uint32_t> words;  // payload words (bits)
+};
+
+// Timing of the last run, from HIP events on the run's stream (ms).
+struct StageTimes {
+  float front = 0, bwt = 0, mtf = 0, huff = 0, pack = 0, assemble = 0;
+  int rounds = 0;
+};
+
+class Engine {
+ public:
+  explicit Engine(int device = 0);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  // Host-side plan of a batch.  Returns 0 or a hipError_t value.
+  // allow_big lets one block exceed 900000 raw bytes (stage entry points).
+  int plan(const std::vector<StreamSpec>& streams, bool allow_big = false);
+  // Allocates (grow-only) and uploads the plan; d_in holds the raw bytes.
+  int setup(const uint8_t* d_in, hipStream_t s);
+  // Full pipeline on the planned batch.  With assemble = true the .bz2
+  // streams are written to the device output buffer.
+  int run(hipStream_t s, bool assemble, StageTimes* times = nullptr);
+  // Stage groups (used by run and by the per-stage entry points).
+  int run_front(hipStream_t s);
+  int run_bwt(hipStream_t s, int* rounds_out = nullptr);
+  int run_mtf_rle2(hipStream_t s);
+  int run_huffman(hipStream_t s);
+  int run_pack(hipStream_t s);
+  int run_assemble(hipStream_t s);
+
+  // Single-block helper for the per-stage entry points: a one-block batch of
+  // capacity for `len` elements (no raw input).
+  int setup_single(int32_t len, hipStream_t s);
+
+  // Copies per-stream layout (byte_off, bytes) to the host.
+  int stream_layout(std::vector<StreamState>& out, hipStream_t s);
+  // Copies one block's results to the host.
+  int block_result(int b, BlockResult& r, hipStream_t s);
+
+  const Dev& dev() const { return d_; }
+  int32_t nblk() const { return (int32_t)desc_.size(); }
+  int32_t nstream() const { return (int32_t)ss_.size(); }
+  uint8_t* out_buffer() const { return d_.out; }
+  int64_t out_capacity() const { return d_.out_cap; }
+  const std::vector<BlockDesc>& desc() const { return desc_; }
+  int device() const { return device_; }
+
+ private:
+  int grow(size_t bytes);
+  int device_;
+  Dev d_{};
+  void* ws_ = nullptr;
+  size_t ws_bytes_ = 0;
+  int32_t* h_ctr_ = nullptr;  // pinned
+  std::vector<BlockDesc> desc_;
+  std::vector<StreamState> ss_;
+  std::vector<int32_t> tn_blk_, tc_blk_;
+  int64_t total_cap_ = 0, sel_tot_ = 0, w_tot_ = 0, out_cap_ = 0;
+  int32_t max_cap_ = 0;
+  hipEvent_t ev_[8] = {};
+  int init_err_ = 0;
+};
+
+}  // namespace bzg

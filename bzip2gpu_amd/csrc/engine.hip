@@ -1,0 +1,326 @@
+// engine.hip - batch planning, workspace and stage sequencing (host code).
+#include "engine.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace bzg {
+
+void launch_tile_scan_c(const Dev& d, hipStream_t s);
+
+#define BZG_TRY(x)                       \
+  do {                                   \
+    hipError_t e_ = (x);                 \
+    if (e_ != hipSuccess) return (int)e_; \
+  } while (0)
+
+Engine::Engine(int device) : device_(device) {
+  int e = (int)hipSetDevice(device_);
+  if (!e) e = (int)hipHostMalloc((void**)&h_ctr_, CTR_N * sizeof(int32_t), hipHostMallocDefault);
+  for (auto& ev : ev_)
+    if (!e) e = (int)hipEventCreate(&ev);
+  init_err_ = e;
+}
+
+Engine::~Engine() {
+  (void)hipSetDevice(device_);
+  if (ws_) (void)hipFree(ws_);
+  if (h_ctr_) (void)hipHostFree(h_ctr_);
+  for (auto& ev : ev_)
+    if (ev) (void)hipEventDestroy(ev);
+}
+
+int Engine::plan(const std::vector<StreamSpec>& streams, bool allow_big) {
+  if (init_err_) return init_err_;
+  desc_.clear();
+  ss_.clear();
+  tn_blk_.clear();
+  tc_blk_.clear();
+  total_cap_ = sel_tot_ = w_tot_ = out_cap_ = 0;
+  max_cap_ = 0;
+  int32_t tn = 0, tc = 0;
+  for (size_t si = 0; si < streams.size(); si++) {
+    const StreamSpec& S = streams[si];
+    int level = std::min(9, std::max(1, S.level));
+    int bb = S.block_bytes > 0 ? S.block_bytes : 900000;
+    if (bb > (allow_big ? kStageMax : kMaxBlock)) return (int)hipErrorInvalidValue;
+    StreamState st{};
+    st.first_blk = (int32_t)desc_.size();
+    st.level = level;
+    int64_t sbytes = 16;
+    for (uint64_t off = 0; off < S.len; off += (uint64_t)bb) {
+      BlockDesc D{};
+      D.in_off = S.in_off + off;
+      D.n = (int32_t)std::min<uint64_t>((uint64_t)bb, S.len - off);
+      D.cap = cap_for(D.n);
+      D.off = (uint64_t)total_cap_;
+      D.sel_off = (uint64_t)sel_tot_;
+      D.w_off = (uint64_t)w_tot_;
+      D.tile_n0 = tn;
+      D.tile_c0 = tc;
+      D.stream = (int32_t)si;
+      D.level = level;
+      int32_t b = (int32_t)desc_.size();
+      for (int t = 0; t < tiles_for(D.n); t++) tn_blk_.push_back(b);
+      for (int t = 0; t < tiles_for(D.cap); t++) tc_blk_.push_back(b);
+      tn += tiles_for(D.n);
+      tc += tiles_for(D.cap);
+      total_cap_ += D.cap;
+      sel_tot_ += sel_cap_for(D.cap);
+      w_tot_ += pay_words_for(D.cap);
+      max_cap_ = std::max(max_cap_, D.cap);
+      sbytes += (pay_words_for(D.cap) + hdr_words_for(D.cap)) * 4;
+      desc_.push_back(D);
+    }
+    st.n_blk = (int32_t)desc_.size() - st.first_blk;
+    ss_.push_back(st);
+    out_cap_ += (sbytes + 3) & ~3ll;
+  }
+  return 0;
+}
+
+int Engine::grow(size_t bytes) {
+  if (bytes <= ws_bytes_) return 0;
+  if (ws_) {
+    BZG_TRY(hipDeviceSynchronize());
+    BZG_TRY(hipFree(ws_));
+    ws_ = nullptr;
+    ws_bytes_ = 0;
+  }
+  size_t want = bytes + bytes / 8;
+  BZG_TRY(hipMalloc(&ws_, want));
+  ws_bytes_ = want;
+  return 0;
+}
+
+namespace {
+struct Carver {
+  size_t off = 0;
+  char* base = nullptr;
+  template <typename T>
+  T* take(size_t count) {
+    off = (off + 255) & ~(size_t)255;
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += count * sizeof(T) + 64;
+    return p;
+  }
+};
+}  // namespace
+
+int Engine::setup(const uint8_t* d_in, hipStream_t s) {
+  BZG_TRY(hipSetDevice(device_));
+  const int64_t B = (int64_t)desc_.size();
+  const int64_t S = (int64_t)ss_.size();
+  const int64_t NTn = (int64_t)tn_blk_.size(), NTc = (int64_t)tc_blk_.size();
+  const int64_t CAP = total_cap_;
+  const int64_t seg_cap = CAP / 2 + B + 16;
+  const int64_t pack_cap = CAP / kHalf + B + 16;
+  const int64_t ltile_cap = CAP / 512 + B + 16;
+  const int64_t hdr_stride = B ? hdr_words_for(max_cap_) : 8;
+  auto carve = [&](Carver& c) {
+    Dev d{};
+    d.desc = c.take<BlockDesc>(B + 1);
+    d.st = c.take<BlockState>(B + 1);
+    d.ss = c.take<StreamState>(S + 1);
+    d.tn_blk = c.take<int32_t>(NTn + 1);
+    d.tc_blk = c.take<int32_t>(NTc + 1);
+    d.tn_crc = c.take<uint32_t>(NTn + 1);
+    d.tn_fb = c.take<int32_t>(NTn + 1);
+    d.tn_lb = c.take<int32_t>(NTn + 1);
+    d.tn_cnt = c.take<int32_t>(NTn + 1);
+    d.tn_off = c.take<int32_t>(NTn + 1);
+    d.tc_cnt = c.take<int32_t>(NTc + 1);
+    d.tc_off = c.take<int32_t>(NTc + 1);
+    d.tc_fnz = c.take<int32_t>(NTc + 1);
+    d.lo = c.take<int32_t>((NTc + 1) * 256);
+    d.T = c.take<uint8_t>(CAP + 64);
+    d.SA = c.take<uint32_t>(CAP + 16);
+    d.RK = c.take<uint32_t>(CAP + 16);
+    d.KEY = c.take<uint32_t>(CAP + 16);
+    d.SA2 = c.take<uint32_t>(CAP + 16);
+    d.KEY2 = c.take<uint32_t>(CAP + 16);
+    d.MTF = c.take<uint8_t>(CAP + 64);
+    d.SYM = c.take<uint16_t>(CAP + 64);
+    d.freq = c.take<int32_t>((B + 1) * 260);
+    d.rfreq = c.take<int32_t>((B + 1) * 6 * 260);
+    d.len = c.take<uint8_t>((B + 1) * 6 * 260);
+    d.code = c.take<uint32_t>((B + 1) * 6 * 260);
+    d.sel = c.take<uint8_t>(sel_tot_ + 64);
+    d.selmtf = c.take<uint8_t>(sel_tot_ + 64);
+    d.gbits = c.take<int32_t>(sel_tot_ + 64);
+    d.pay = c.take<uint32_t>(w_tot_ + 64);
+    d.hdr = c.take<uint32_t>((B + 1) * hdr_stride);
+    d.seg[0] = c.take<Seg>(seg_cap);
+    d.seg[1] = c.take<Seg>(seg_cap);
+    d.pa = c.take<int32_t>(seg_cap);
+    d.pb = c.take<int32_t>(seg_cap);
+    d.pc = c.take<int32_t>(seg_cap);
+    d.packs = c.take<int32_t>(pack_cap + 2);
+    d.ltile_seg = c.take<int32_t>(ltile_cap);
+    d.ltile_pos = c.take<int32_t>(ltile_cap);
+    d.llist = c.take<int32_t>(ltile_cap);
+    d.hist = c.take<uint32_t>(ltile_cap * 256);
+    d.lsum = c.take<int32_t>(ltile_cap * 2);
+    d.ctr = c.take<int32_t>(CTR_N);
+    d.scan_part = c.take<int32_t>(3 * kScanWG);
+    d.out = c.take<uint8_t>(out_cap_ + 64);
+    return d;
+  };
+  Carver probe;
+  carve(probe);
+  BZG_TRY((hipError_t)grow(probe.off + 4096));
+  Carver real;
+  real.base = static_cast<char*>(ws_);
+  Dev d = carve(real);
+  d.in = d_in;
+  d.nblk = (int32_t)B;
+  d.nstream = (int32_t)S;
+  d.ntile_n = (int32_t)NTn;
+  d.ntile_c = (int32_t)NTc;
+  d.total_cap = CAP;
+  d.hdr_stride = hdr_stride;
+  d.seg_cap = seg_cap;
+  d.ltile_cap = ltile_cap;
+  d.pack_cap = pack_cap;
+  d.out_cap = out_cap_;
+  d_ = d;
+  if (B) {
+    BZG_TRY(hipMemcpyAsync((void*)d.desc, desc_.data(), B * sizeof(BlockDesc), hipMemcpyHostToDevice, s));
+    BZG_TRY(hipMemcpyAsync((void*)d.tn_blk, tn_blk_.data(), NTn * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    BZG_TRY(hipMemcpyAsync((void*)d.tc_blk, tc_blk_.data(), NTc * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  }
+  if (S) BZG_TRY(hipMemcpyAsync(d.ss, ss_.data(), S * sizeof(StreamState), hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+int Engine::setup_single(int32_t len, hipStream_t s) {
+  std::vector<StreamSpec> one{StreamSpec{0, (uint64_t)len, 9, len}};
+  int rc = plan(one, true);
+  if (rc) return rc;
+  if (desc_.size() != 1) return (int)hipErrorInvalidValue;
+  rc = setup(nullptr, s);
+  if (rc) return rc;
+  launch_block_init(d_, s);
+  return (int)hipGetLastError();
+}
+
+int Engine::run_front(hipStream_t s) {
+  launch_block_init(d_, s);
+  launch_crc_rle1(d_, s);
+  return (int)hipGetLastError();
+}
+
+int Engine::run_bwt(hipStream_t s, int* rounds_out) {
+  launch_bwt_init(d_, s);
+  BZG_TRY(hipGetLastError());
+  int cur = 0, round = 0;
+  int32_t P = 0;
+  int keybits_late = 1;
+  while ((1 << keybits_late) < max_cap_ + 1) keybits_late++;
+  for (;;) {
+    launch_bwt_plan(d_, cur, 0, s);
+    BZG_TRY(hipMemcpyAsync(h_ctr_, d_.ctr, CTR_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    BZG_TRY(hipStreamSynchronize(s));
+    const int32_t nseg = h_ctr_[CTR_NSEG];
+    if (h_ctr_[CTR_ERR]) return (int)hipErrorIllegalState;
+    if (nseg == 0) break;
+    if (round > 40) return (int)hipErrorIllegalState;
+    const int32_t small = h_ctr_[CTR_SMALL];
+    const int32_t npacks = (small + kHalf - 1) / kHalf;
+    const int32_t ntiles = h_ctr_[CTR_LTILES];
+    const int32_t nlarge = h_ctr_[CTR_NLARGE];
+    const int keybits = round == 0 ? 32 : keybits_late;
+    launch_bwt_round(d_, cur, round, P, keybits, npacks, ntiles, nlarge, nseg, s);
+    BZG_TRY(hipGetLastError());
+    cur ^= 1;
+    P = round == 0 ? 4 : (P >= (1 << 29) ? P : 2 * P);
+    round++;
+  }
+  if (rounds_out) *rounds_out = round;
+  return 0;
+}
+
+int Engine::run_mtf_rle2(hipStream_t s) {
+  launch_mtf_rle2(d_, s);
+  return (int)hipGetLastError();
+}
+
+int Engine::run_huffman(hipStream_t s) {
+  launch_huffman(d_, s);
+  return (int)hipGetLastError();
+}
+
+int Engine::run_pack(hipStream_t s) {
+  launch_pack(d_, s);
+  return (int)hipGetLastError();
+}
+
+int Engine::run_assemble(hipStream_t s) {
+  launch_assemble(d_, s);
+  return (int)hipGetLastError();
+}
+
+int Engine::run(hipStream_t s, bool assemble, StageTimes* times) {
+  BZG_TRY(hipSetDevice(device_));
+  if (desc_.empty()) {
+    if (assemble && !ss_.empty()) return run_assemble(s);
+    return 0;
+  }
+  int rounds = 0;
+  if (times) BZG_TRY(hipEventRecord(ev_[0], s));
+  int rc = run_front(s);
+  if (rc) return rc;
+  if (times) BZG_TRY(hipEventRecord(ev_[1], s));
+  rc = run_bwt(s, &rounds);
+  if (rc) return rc;
+  if (times) BZG_TRY(hipEventRecord(ev_[2], s));
+  rc = run_mtf_rle2(s);
+  if (rc) return rc;
+  if (times) BZG_TRY(hipEventRecord(ev_[3], s));
+  rc = run_huffman(s);
+  if (rc) return rc;
+  if (times) BZG_TRY(hipEventRecord(ev_[4], s));
+  rc = run_pack(s);
+  if (rc) return rc;
+  if (times) BZG_TRY(hipEventRecord(ev_[5], s));
+  if (assemble) {
+    rc = run_assemble(s);
+    if (rc) return rc;
+  }
+  if (times) {
+    BZG_TRY(hipEventRecord(ev_[6], s));
+    BZG_TRY(hipEventSynchronize(ev_[6]));
+    float* f[6] = {&times->front, &times->bwt, &times->mtf, &times->huff, &times->pack, &times->assemble};
+    for (int i = 0; i < 6; i++) BZG_TRY(hipEventElapsedTime(f[i], ev_[i], ev_[i + 1]));
+    times->rounds = rounds;
+  }
+  return 0;
+}
+
+int Engine::stream_layout(std::vector<StreamState>& out, hipStream_t s) {
+  out.resize(ss_.size());
+  if (ss_.empty()) return 0;
+  BZG_TRY(hipMemcpyAsync(out.data(), d_.ss, ss_.size() * sizeof(StreamState), hipMemcpyDeviceToHost, s));
+  BZG_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int Engine::block_result(int b, BlockResult& r, hipStream_t s) {
+  BlockState st;
+  BZG_TRY(hipMemcpyAsync(&st, d_.st + b, sizeof st, hipMemcpyDeviceToHost, s));
+  BZG_TRY(hipStreamSynchronize(s));
+  r.crc = st.crc; r.m = st.m; r.s = st.s; r.orig_ptr = st.orig_ptr;
+  r.n_in_use = st.n_in_use; r.n_groups = st.n_groups; r.n_sel = st.n_sel;
+  r.rounds = st.rounds; r.bits = st.bits; r.hdr_bits = st.hdr_bits;
+  for (int i = 0; i < 256; i++) r.present[i] = (st.present[i >> 5] >> (i & 31)) & 1u;
+  r.hdr.resize((size_t)((st.hdr_bits + 31) / 32));
+  r.words.resize((size_t)((st.bits + 31) / 32));
+  if (!r.hdr.empty())
+    BZG_TRY(hipMemcpyAsync(r.hdr.data(), d_.hdr + (int64_t)b * d_.hdr_stride, r.hdr.size() * 4, hipMemcpyDeviceToHost, s));
+  if (!r.words.empty())
+    BZG_TRY(hipMemcpyAsync(r.words.data(), d_.pay + desc_[b].w_off, r.words.size() * 4, hipMemcpyDeviceToHost, s));
+  BZG_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // namespace bzg

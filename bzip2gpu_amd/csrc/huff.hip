@@ -1,0 +1,556 @@
+// huff.hip - Huffman table build, selectors, block headers, payload packing
+// and .bz2 stream assembly for a whole batch.
+//
+// Table build (reference: src/compression/huffman.cu:148-364, which follows
+// libbz2's sendMTFValues): n_groups from s, an initial split of the alphabet
+// into n_groups frequency ranges (cost 0 inside / 15 outside), then 4
+// iterations of: per 50-symbol group pick the first table of strictly
+// minimal cost and count the group's symbols for it, then rebuild every
+// table's code lengths with the libbz2 heap method (max length 17).
+// Codes are canonical by (length, symbol).
+//
+// Bit output: every writer owns a contiguous bit range of a zeroed word
+// array; words fully inside the range are plain stores, the (at most two)
+// words shared with a neighbour are atomicOr.  Words are MSB-first u32
+// (src/compression/huffman_encode.cu:30-71); the final stream stores them
+// byte-swapped so memory holds the .bz2 bytes in order.
+#include "pipeline.h"
+
+namespace bzg {
+
+// ---------------------------------------------------------------- bit out
+struct BitOut {
+  uint32_t* w;       // word array
+  int64_t pos;       // next bit position
+  int64_t lo, hi;    // owned bit range [lo, hi)
+  uint64_t acc;      // pending bits, right-aligned
+  int nacc;
+  bool swap;         // store byte-swapped (stream output)
+
+  __device__ void init(uint32_t* words, int64_t start, int64_t end, bool bswap) {
+    w = words; pos = start; lo = start; hi = end; acc = 0; nacc = 0; swap = bswap;
+  }
+  __device__ void store(int64_t wi, uint32_t v, bool full) {
+    if (swap) v = __builtin_bswap32(v);
+    if (full) w[wi] = v;
+    else if (v) atomicOr(&w[wi], v);
+  }
+  __device__ void put(uint32_t v, int n) {  // 0 <= n <= 32
+    if (n == 0) return;
+    acc = (acc << n) | (uint64_t)(n == 32 ? v : (v & ((1u << n) - 1)));
+    nacc += n;
+    int room = 32 - (int)(pos & 31);
+    while (nacc >= room) {
+      uint32_t bits = (uint32_t)(acc >> (nacc - room));
+      if (room < 32) bits &= (1u << room) - 1;
+      int64_t wi = pos >> 5;
+      store(wi, bits, wi * 32 >= lo && wi * 32 + 32 <= hi);
+      pos += room;
+      nacc -= room;
+      acc = nacc ? (acc & ((1ull << nacc) - 1)) : 0ull;
+      room = 32;
+    }
+  }
+  __device__ void finish() {  // trailing partial word (shared with the right)
+    if (nacc > 0) {
+      int sh = (int)(pos & 31);
+      store(pos >> 5, (uint32_t)acc << (32 - sh - nacc), false);
+      pos += nacc;
+      nacc = 0;
+      acc = 0;
+    }
+  }
+};
+
+// ------------------------------------------------------------ table build
+__global__ void __launch_bounds__(64) k_huff_init(Dev d) {
+  const int b = blockIdx.x;
+  BlockState* st = d.st + b;
+  const int s = st->s;
+  const int alpha = st->n_in_use + 2;
+  uint8_t* len = d.len + (int64_t)b * 6 * 260;
+  for (int q = threadIdx.x; q < 6 * 260; q += 64) len[q] = 15;
+  // zero this block's header words (header writers use atomicOr at edges)
+  uint32_t* hw = d.hdr + (int64_t)b * d.hdr_stride;
+  for (int64_t q = threadIdx.x; q < d.hdr_stride; q += 64) hw[q] = 0;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const int ng = s < 200 ? 2 : s < 600 ? 3 : s < 1200 ? 4 : s < 2400 ? 5 : 6;
+  st->n_groups = ng;
+  st->n_sel = (s + kGSize - 1) / kGSize;
+  const int32_t* freq = d.freq + (int64_t)b * 260;
+  // initial split, huffman.cu:148-185 (bound `ge < alphabet_size`, :160)
+  int gs = 0, rem = s;
+  for (int part = ng; part > 0; part--) {
+    int tf = rem / part, ge = gs - 1, a = 0;
+    while (a < tf && ge < alpha) { ge++; a += freq[ge]; }
+    if (ge > gs && part != ng && part != 1 && ((ng - part) % 2 == 1)) { a -= freq[ge]; ge--; }
+    for (int v = 0; v < alpha; v++) len[(part - 1) * 260 + v] = (v >= gs && v <= ge) ? 0 : 15;
+    gs = ge + 1;
+    rem -= a;
+  }
+}
+
+// One thread per 50-symbol group: costs, first strict minimum, counts.
+__global__ void __launch_bounds__(256) k_huff_select(Dev d) {
+  __shared__ uint8_t L[6][260];
+  __shared__ int32_t rf[6][260];
+  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
+  const int b = d.tc_blk[t];
+  const BlockDesc D = d.desc[b];
+  const int s = d.st[b].s;
+  const int k = t - D.tile_c0;
+  const int g0 = (k * kTile + kGSize - 1) / kGSize;
+  const int g1 = min(((k + 1) * kTile + kGSize - 1) / kGSize, (s + kGSize - 1) / kGSize);
+  if (g0 >= g1) return;
+  const int ng = d.st[b].n_groups;
+  for (int q = threadIdx.x; q < 6 * 260; q += 256) {
+    (&L[0][0])[q] = d.len[(int64_t)b * 6 * 260 + q];
+    (&rf[0][0])[q] = 0;
+  }
+  __syncthreads();
+  const int g = g0 + (int)threadIdx.x;
+  if (g < g1) {
+    const int gs = g * kGSize, ge = min(gs + kGSize, s);
+    const uint16_t* sym = d.SYM + D.off;
+    uint32_t cost[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = gs; i < ge; i++) {
+      uint16_t v = sym[i];
+#pragma unroll
+      for (int tt = 0; tt < 6; tt++) cost[tt] += L[tt][v];
+    }
+    uint32_t bc = 0xFFFFFFFFu;
+    int bt = 0;
+    for (int tt = 0; tt < ng; tt++)
+      if (cost[tt] < bc) { bc = cost[tt]; bt = tt; }
+    d.sel[D.sel_off + g] = (uint8_t)bt;
+    for (int i = gs; i < ge; i++) atomicAdd(&rf[bt][sym[i]], 1);
+  }
+  __syncthreads();
+  int32_t* grf = d.rfreq + (int64_t)b * 6 * 260;
+  for (int q = threadIdx.x; q < 6 * 260; q += 256) {
+    int v = (&rf[0][0])[q];
+    if (v) atomicAdd(&grf[q], v);
+  }
+}
+
+// libbz2 code-length method, one thread per table (huffman.cu:51-124).
+__global__ void __launch_bounds__(64) k_huff_lengths(Dev d) {
+  __shared__ int32_t heap_s[6][260];
+  __shared__ int32_t weight_s[6][520];
+  __shared__ int32_t parent_s[6][520];
+  const int b = blockIdx.x;
+  const int tt = threadIdx.x;
+  const int ng = d.st[b].n_groups;
+  if (tt >= ng) return;
+  const int alpha = d.st[b].n_in_use + 2;
+  int32_t* heap = heap_s[tt];
+  int32_t* weight = weight_s[tt];
+  int32_t* parent = parent_s[tt];
+  int32_t* fr = d.rfreq + ((int64_t)b * 6 + tt) * 260;
+  uint8_t* len = d.len + ((int64_t)b * 6 + tt) * 260;
+  for (int i = 0; i < alpha; i++) {
+    int f = fr[i];
+    weight[i + 1] = (f == 0 ? 1 : f) << 8;
+    fr[i] = 0;  // ready for the next iteration
+  }
+  while (true) {
+    int nNodes = alpha, nHeap = 0;
+    heap[0] = 0; weight[0] = 0; parent[0] = -2;
+    for (int i = 1; i <= alpha; i++) {
+      parent[i] = -1;
+      nHeap++;
+      heap[nHeap] = i;
+      int zz = nHeap, tmp = heap[zz];
+      while (weight[tmp] < weight[heap[zz >> 1]]) { heap[zz] = heap[zz >> 1]; zz >>= 1; }
+      heap[zz] = tmp;
+    }
+    while (nHeap > 1) {
+      int pick[2];
+      for (int q = 0; q < 2; q++) {
+        pick[q] = heap[1];
+        heap[1] = heap[nHeap];
+        nHeap--;
+        int zz = 1, tmp = heap[zz];
+        while (true) {
+          int yy = zz << 1;
+          if (yy > nHeap) break;
+          if (yy < nHeap && weight[heap[yy + 1]] < weight[heap[yy]]) yy++;
+          if (weight[tmp] < weight[heap[yy]]) break;
+          heap[zz] = heap[yy];
+          zz = yy;
+        }
+        heap[zz] = tmp;
+      }
+      nNodes++;
+      parent[pick[0]] = parent[pick[1]] = nNodes;
+      int w1 = weight[pick[0]], w2 = weight[pick[1]];
+      int d1 = w1 & 0xff, d2 = w2 & 0xff;
+      weight[nNodes] = ((w1 & 0xffffff00) + (w2 & 0xffffff00)) | (1 + (d1 > d2 ? d1 : d2));
+      parent[nNodes] = -1;
+      nHeap++;
+      heap[nHeap] = nNodes;
+      int zz = nHeap, tmp = heap[zz];
+      while (weight[tmp] < weight[heap[zz >> 1]]) { heap[zz] = heap[zz >> 1]; zz >>= 1; }
+      heap[zz] = tmp;
+    }
+    bool tooLong = false;
+    for (int i = 1; i <= alpha; i++) {
+      int j = 0, kk = i;
+      while (parent[kk] >= 0) { kk = parent[kk]; j++; }
+      len[i - 1] = (uint8_t)j;
+      if (j > 17) tooLong = true;
+    }
+    if (!tooLong) break;
+    for (int i = 1; i <= alpha; i++) {
+      int j = weight[i] >> 8;
+      j = 1 + (j / 2);
+      weight[i] = j << 8;
+    }
+  }
+}
+
+// Codes (huffman.cu:135-146, :352-362), selector MTF (compress.cu:119-137)
+// and the block header bits (compress.cu:290-356) into the block's header
+// words.  Lanes 0..ng-1 assign codes; the selector MTF runs as a wave scan
+// over 64 chunks (list state = last occurrence of each table id).
+__global__ void __launch_bounds__(64) k_huff_codes(Dev d) {
+  __shared__ int32_t tbits[8];
+  const int b = blockIdx.x;
+  const int lane = lane_id();
+  BlockState* st = d.st + b;
+  const BlockDesc D = d.desc[b];
+  const int ng = st->n_groups, nsel = st->n_sel;
+  const int alpha = st->n_in_use + 2;
+  if (lane < ng) {
+    const uint8_t* len = d.len + ((int64_t)b * 6 + lane) * 260;
+    uint32_t* code = d.code + ((int64_t)b * 6 + lane) * 260;
+    int mn = 32, mx = 0;
+    for (int v = 0; v < alpha; v++) { mn = min(mn, (int)len[v]); mx = max(mx, (int)len[v]); }
+    uint32_t vec = 0;
+    for (int l = mn; l <= mx; l++) {
+      for (int v = 0; v < alpha; v++)
+        if (len[v] == l) code[v] = vec++;
+      vec <<= 1;
+    }
+    // coding-table bits: 5 + sum(1 + 2*|delta|)
+    int cur = len[0], nb = 5;
+    for (int v = 0; v < alpha; v++) {
+      int tg = len[v];
+      nb += 1 + 2 * (tg > cur ? tg - cur : cur - tg);
+      cur = tg;
+    }
+    tbits[lane] = nb;
+  }
+  // selector MTF: lane-chunked, entering state by max-scan of last occurrence
+  const uint8_t* sel = d.sel + D.sel_off;
+  uint8_t* smtf = d.selmtf + D.sel_off;
+  const int per = (nsel + 63) / 64;
+  const int c0 = min(nsel, lane * per), c1 = min(nsel, c0 + per);
+  int lastocc[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) lastocc[q] = -1;
+  for (int i = c0; i < c1; i++) lastocc[sel[i]] = i;
+  int ent[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    int v = wave_incl_max(lastocc[q]);
+    int ex = __shfl_up(v, 1, 64);
+    ent[q] = lane == 0 ? -1 : ex;
+    // initial list 0,1,..,ng-1: encode as -(1+q)-1000000000 so never-seen ids
+    // rank after seen ones, in ascending order
+    if (ent[q] < 0) ent[q] = -1000000000 - q;
+  }
+  uint8_t lst[6];
+  {
+    // order table ids by ent descending
+    int ids[6] = {0, 1, 2, 3, 4, 5};
+    for (int a = 0; a < ng; a++)
+      for (int c = a + 1; c < ng; c++)
+        if (ent[ids[c]] > ent[ids[a]]) { int t2 = ids[a]; ids[a] = ids[c]; ids[c] = t2; }
+    for (int a = 0; a < 6; a++) lst[a] = (uint8_t)ids[a];
+  }
+  int my_bits = 0;
+  for (int i = c0; i < c1; i++) {
+    uint8_t v = sel[i];
+    int j = 0;
+    while (lst[j] != v) j++;
+    for (int q = j; q > 0; q--) lst[q] = lst[q - 1];
+    lst[0] = v;
+    smtf[i] = (uint8_t)j;
+    my_bits += j + 1;
+  }
+  __syncthreads();
+  // header layout
+  int nused = 0;
+  uint32_t l1 = 0;
+  for (int r = 0; r < 16; r++) {
+    uint32_t wv = st->present[r >> 1];
+    uint32_t half = (r & 1) ? (wv >> 16) : (wv & 0xFFFF);
+    if (half) { l1 |= 1u << (15 - r); nused++; }
+  }
+  const int64_t fixed = 48 + 32 + 1 + 24 + 16 + 16 * nused + 3 + 15;
+  int sb_ex = wave_incl_sum(my_bits) - my_bits;
+  int sel_total = wave_sum(my_bits);
+  int tb_before = 0, tb_total = 0;
+  for (int q = 0; q < ng; q++) {
+    if (q < lane) tb_before += tbits[q];
+    tb_total += tbits[q];
+  }
+  uint32_t* hw = d.hdr + (int64_t)b * d.hdr_stride;
+  const int64_t hdr_bits = fixed + sel_total + tb_total;
+  // fixed part (lane 0)
+  if (lane == 0) {
+    BitOut o;
+    o.init(hw, 0, fixed, false);
+    o.put(0x314159u, 24); o.put(0x265359u, 24);
+    o.put(st->crc, 32);
+    o.put(0, 1);
+    o.put((uint32_t)st->orig_ptr, 24);
+    o.put(l1, 16);
+    for (int r = 0; r < 16; r++) {
+      if (!((l1 >> (15 - r)) & 1u)) continue;
+      uint32_t wv = st->present[r >> 1];
+      uint32_t half = (r & 1) ? (wv >> 16) : (wv & 0xFFFF);
+      uint32_t l2 = 0;
+      for (int q = 0; q < 16; q++) if ((half >> q) & 1u) l2 |= 1u << (15 - q);
+      o.put(l2, 16);
+    }
+    o.put((uint32_t)ng, 3);
+    o.put((uint32_t)nsel, 15);
+    o.finish();
+    st->hdr_bits = hdr_bits;
+  }
+  // selectors: lane's chunk owns [fixed + sb_ex, + my_bits)
+  {
+    BitOut o;
+    int64_t a = fixed + sb_ex;
+    o.init(hw, a, a + my_bits, false);
+    for (int i = c0; i < c1; i++) {
+      int j = smtf[i];
+      // j ones then a zero
+      o.put(((1u << j) - 1) << 1, j + 1);
+    }
+    o.finish();
+  }
+  // coding tables: lane q < ng owns its table's range
+  if (lane < ng) {
+    const uint8_t* len = d.len + ((int64_t)b * 6 + lane) * 260;
+    BitOut o;
+    int64_t a = fixed + sel_total + tb_before;
+    o.init(hw, a, a + tbits[lane], false);
+    int cur = len[0];
+    o.put((uint32_t)cur, 5);
+    for (int v = 0; v < alpha; v++) {
+      int tg = len[v];
+      while (cur < tg) { o.put(2, 2); cur++; }
+      while (cur > tg) { o.put(3, 2); cur--; }
+      o.put(0, 1);
+    }
+    o.finish();
+  }
+}
+
+// ------------------------------------------------------------ payload
+// bits per 50-symbol group
+__global__ void __launch_bounds__(256) k_group_bits(Dev d) {
+  __shared__ uint8_t L[6][260];
+  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
+  const int b = d.tc_blk[t];
+  const BlockDesc D = d.desc[b];
+  const int s = d.st[b].s;
+  const int k = t - D.tile_c0;
+  const int g0 = (k * kTile + kGSize - 1) / kGSize;
+  const int g1 = min(((k + 1) * kTile + kGSize - 1) / kGSize, (s + kGSize - 1) / kGSize);
+  if (g0 >= g1) return;
+  for (int q = threadIdx.x; q < 6 * 260; q += 256) (&L[0][0])[q] = d.len[(int64_t)b * 6 * 260 + q];
+  __syncthreads();
+  const int g = g0 + (int)threadIdx.x;
+  if (g < g1) {
+    const int gs = g * kGSize, ge = min(gs + kGSize, s);
+    const int tt = d.sel[D.sel_off + g];
+    const uint16_t* sym = d.SYM + D.off;
+    int nb = 0;
+    for (int i = gs; i < ge; i++) nb += L[tt][sym[i]];
+    d.gbits[D.sel_off + g] = nb;
+  }
+}
+
+// per block: exclusive scan of group bits -> offsets, W, zero payload words
+__global__ void __launch_bounds__(256) k_group_scan(Dev d) {
+  __shared__ int32_t tmp[8];
+  const int b = blockIdx.x;
+  const BlockDesc D = d.desc[b];
+  const int nsel = d.st[b].n_sel;
+  int run = 0;
+  for (int base = 0; base < nsel; base += 256) {
+    int g = base + threadIdx.x;
+    int v = g < nsel ? d.gbits[D.sel_off + g] : 0;
+    int tot;
+    int ex = wg_excl_sum<256, int>(v, tmp, &tot);
+    if (g < nsel) d.gbits[D.sel_off + g] = run + ex;
+    run += tot;
+  }
+  if (threadIdx.x == 0) d.st[b].bits = run;
+  const int64_t nw = ((int64_t)run + 31) / 32 + 1;
+  for (int64_t q = threadIdx.x; q < nw; q += 256) d.pay[D.w_off + q] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_pack(Dev d) {
+  __shared__ uint8_t L[6][260];
+  __shared__ uint32_t C[6][260];
+  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
+  const int b = d.tc_blk[t];
+  const BlockDesc D = d.desc[b];
+  const int s = d.st[b].s;
+  const int k = t - D.tile_c0;
+  const int g0 = (k * kTile + kGSize - 1) / kGSize;
+  const int nsel = (s + kGSize - 1) / kGSize;
+  const int g1 = min(((k + 1) * kTile + kGSize - 1) / kGSize, nsel);
+  if (g0 >= g1) return;
+  for (int q = threadIdx.x; q < 6 * 260; q += 256) {
+    (&L[0][0])[q] = d.len[(int64_t)b * 6 * 260 + q];
+    (&C[0][0])[q] = d.code[(int64_t)b * 6 * 260 + q];
+  }
+  __syncthreads();
+  const int g = g0 + (int)threadIdx.x;
+  if (g < g1) {
+    const int gs = g * kGSize, ge = min(gs + kGSize, s);
+    const int tt = d.sel[D.sel_off + g];
+    const uint16_t* sym = d.SYM + D.off;
+    const int64_t a = d.gbits[D.sel_off + g];
+    const int64_t e = g + 1 < nsel ? d.gbits[D.sel_off + g + 1] : d.st[b].bits;
+    BitOut o;
+    o.init(d.pay + D.w_off, a, e, false);
+    for (int i = gs; i < ge; i++) {
+      uint16_t v = sym[i];
+      o.put(C[tt][v], L[tt][v]);
+    }
+    o.finish();
+  }
+}
+
+// ------------------------------------------------------------ assembly
+// One workgroup: per stream block bit offsets, combined CRC, sizes, byte
+// offsets; zeroes the words the placement kernels share and writes the
+// stream header word.
+__global__ void __launch_bounds__(256) k_layout(Dev d) {
+  __shared__ int64_t tmp64[8];
+  uint32_t* out = reinterpret_cast<uint32_t*>(d.out);
+  int64_t run = 0;
+  for (int base = 0; base < d.nstream; base += 256) {
+    const int si = base + threadIdx.x;
+    int64_t words = 0;
+    StreamState S{};
+    if (si < d.nstream) {
+      S = d.ss[si];
+      int64_t off = 32;
+      uint32_t crc = 0;
+      for (int q = 0; q < S.n_blk; q++) {
+        BlockState* st = d.st + S.first_blk + q;
+        st->bit_off = off;
+        off += st->hdr_bits + st->bits;
+        crc = ((crc << 1) | (crc >> 31)) ^ st->crc;
+      }
+      off += 48 + 32;
+      S.bits = off;
+      S.crc = crc;
+      S.bytes = (off + 7) / 8;
+      words = (off + 31) / 32;
+    }
+    int64_t inc = wave_incl_sum(words);
+    if (lane_id() == 63) tmp64[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    int64_t pre = 0;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); w++) pre += tmp64[w];
+    int64_t tot = tmp64[0] + tmp64[1] + tmp64[2] + tmp64[3];
+    __syncthreads();
+    if (si < d.nstream) {
+      S.byte_off = (run + pre + inc - words) * 4;
+      d.ss[si] = S;
+      uint32_t* w0 = out + S.byte_off / 4;
+      w0[0] = __builtin_bswap32(0x425A6830u + (uint32_t)S.level);
+      for (int q = 0; q < S.n_blk; q++) {
+        const BlockState* st = d.st + S.first_blk + q;
+        int64_t a = st->bit_off, e = a + st->hdr_bits + st->bits;
+        w0[a >> 5] = 0;
+        w0[(e - 1) >> 5] = 0;
+        // the header/payload seam inside a block is shared too
+        int64_t mid = a + st->hdr_bits;
+        w0[mid >> 5] = 0;
+      }
+      int64_t ta = S.bits - 80;
+      w0[ta >> 5] = 0;
+      w0[(S.bits - 1) >> 5] = 0;
+    }
+    run += tot;
+  }
+}
+
+// Copies `nbits` bits from aligned source words into the stream at bit
+// `dst` (stream-relative), shared edge words by atomicOr.
+__device__ inline void place_bits(uint32_t* w0, const uint32_t* src, int64_t dst, int64_t nbits,
+                                  int tid, int nth) {
+  if (nbits <= 0) return;
+  const int sh = (int)(dst & 31);
+  const int64_t wa = dst >> 5, we = (dst + nbits - 1) >> 5;
+  const int64_t nsrcw = (nbits + 31) / 32;
+  for (int64_t wi = wa + tid; wi <= we; wi += nth) {
+    // destination word wi gets source bits [(wi-wa)*32 - sh, +32)
+    int64_t j = wi - wa;
+    uint32_t hi = (j - 1 >= 0 && j - 1 < nsrcw) ? src[j - 1] : 0u;
+    uint32_t lo = j < nsrcw ? src[j] : 0u;
+    uint32_t v = sh == 0 ? lo : ((hi << (32 - sh)) | (lo >> sh));
+    // mask off bits outside [dst, dst+nbits)
+    int64_t wbit0 = wi * 32;
+    uint32_t mask = 0xFFFFFFFFu;
+    if (wbit0 < dst) mask &= 0xFFFFFFFFu >> (dst - wbit0);
+    int64_t endb = dst + nbits;
+    if (wbit0 + 32 > endb) mask &= ~(0xFFFFFFFFu >> (endb - wbit0));
+    v &= mask;
+    if (mask == 0xFFFFFFFFu) w0[wi] = __builtin_bswap32(v);
+    else if (v) atomicOr(&w0[wi], __builtin_bswap32(v));
+  }
+}
+
+__global__ void __launch_bounds__(256) k_place(Dev d) {
+  const int b = blockIdx.x;
+  const BlockDesc D = d.desc[b];
+  const BlockState* st = d.st + b;
+  const StreamState S = d.ss[D.stream];
+  uint32_t* w0 = reinterpret_cast<uint32_t*>(d.out) + S.byte_off / 4;
+  place_bits(w0, d.hdr + (int64_t)b * d.hdr_stride, st->bit_off, st->hdr_bits, threadIdx.x, 256);
+  place_bits(w0, d.pay + D.w_off, st->bit_off + st->hdr_bits, st->bits, threadIdx.x, 256);
+}
+
+__global__ void __launch_bounds__(64) k_trailer(Dev d) {
+  const int si = blockIdx.x * 64 + threadIdx.x;
+  if (si >= d.nstream) return;
+  const StreamState S = d.ss[si];
+  uint32_t* w0 = reinterpret_cast<uint32_t*>(d.out) + S.byte_off / 4;
+  uint32_t src[3] = {0x17724538u, (0x5090u << 16) | (S.crc >> 16), (S.crc & 0xFFFFu) << 16};
+  place_bits(w0, src, S.bits - 80, 80, 0, 1);
+}
+
+void launch_huffman(const Dev& d, hipStream_t s) {
+  k_huff_init<<<d.nblk, 64, 0, s>>>(d);
+  for (int it = 0; it < 4; it++) {
+    k_huff_select<<<d.ntile_c, 256, 0, s>>>(d);
+    k_huff_lengths<<<d.nblk, 64, 0, s>>>(d);
+  }
+  k_huff_codes<<<d.nblk, 64, 0, s>>>(d);
+}
+
+void launch_pack(const Dev& d, hipStream_t s) {
+  k_group_bits<<<d.ntile_c, 256, 0, s>>>(d);
+  k_group_scan<<<d.nblk, 256, 0, s>>>(d);
+  k_pack<<<d.ntile_c, 256, 0, s>>>(d);
+}
+
+void launch_assemble(const Dev& d, hipStream_t s) {
+  k_layout<<<1, 256, 0, s>>>(d);
+  k_place<<<d.nblk, 256, 0, s>>>(d);
+  k_trailer<<<(d.nstream + 63) / 64, 64, 0, s>>>(d);
+}
+
+}  // namespace bzg

@@ -1,0 +1,301 @@
+// mtf.hip - forward MTF of the BWT last column, origPtr, and RLE2.
+//
+// MTF (reference: src/compression/fmtf.cu:40-97, :147-230).  The MTF list at
+// any point is the set of present bytes ordered by most recent occurrence,
+// never-seen bytes after them in ascending order (the initial list).  So the
+// list state is a vector lastocc[256], states compose by element-wise max,
+// and the state entering each 4096-symbol segment is an exclusive max-scan
+// of per-segment last-occurrence vectors (k_mtf_scan).  Within a segment one
+// wave handles 64 symbols per step: for lane i with symbol c,
+//   c seen earlier in the step at lane p:  out = #distinct symbols in (p, i)
+//   otherwise:  out = pos(c) + #{first occurrences j < i with pos(c_j) > pos(c)}
+// where pos() is the list position entering the step.
+//
+// RLE2 (reference: src/compression/rle2.cu:11-77): nonzero v -> v+1, a zero
+// run of length k -> the bits of k+1 below its leading one, LSB first
+// (RUNA = 0, RUNB = 1), EOB = n_in_use + 1 appended.  Same count/scan/write
+// structure as RLE1, with run ends from per-tile first-nonzero summaries.
+#include "pipeline.h"
+
+namespace bzg {
+
+void launch_tile_scan_c(const Dev& d, hipStream_t s);
+
+// L[i] = T[(SA[i]-1) mod m] (fmtf.cu:55), orig_ptr, per-tile last occurrence
+__global__ void __launch_bounds__(256) k_mtf_gather(Dev d) {
+  __shared__ int32_t lo[256];
+  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
+  const int b = d.tc_blk[t];
+  const BlockDesc D = d.desc[b];
+  const int m = d.st[b].m;
+  const int k = t - D.tile_c0;
+  const int start = k * kTile;
+  if (start >= m) return;
+  lo[threadIdx.x] = -1;
+  __syncthreads();
+  const int n = min(kTile, m - start);
+  for (int e = threadIdx.x; e < n; e += 256) {
+    int i = start + e;
+    uint32_t v = d.SA[D.off + i];
+    uint8_t c = d.T[D.off + (v == 0 ? (uint32_t)(m - 1) : v - 1)];
+    d.MTF[D.off + i] = c;
+    if (v == 0) d.st[b].orig_ptr = i;
+    atomicMax(&lo[c], i);
+  }
+  __syncthreads();
+  d.lo[(int64_t)t * 256 + threadIdx.x] = lo[threadIdx.x];
+}
+
+// Per block: entering list state of every segment (exclusive max-scan).
+__global__ void __launch_bounds__(256) k_mtf_scan(Dev d) {
+  __shared__ int32_t tmp[8];
+  const int b = blockIdx.x;
+  const BlockDesc D = d.desc[b];
+  const int m = d.st[b].m;
+  const int s = threadIdx.x;
+  const bool present = (d.st[b].present[s >> 5] >> (s & 31)) & 1u;
+  int tot;
+  int rank = wg_excl_sum<256, int>(present ? 1 : 0, tmp, &tot);
+  if (s == 0) d.st[b].n_in_use = tot;
+  int E = present ? -1 - rank : (int)0x80000000;
+  const int nt = tiles_for(m);
+  for (int k = 0; k < nt; k++) {
+    int64_t idx = (int64_t)(D.tile_c0 + k) * 256 + s;
+    int v = d.lo[idx];
+    d.lo[idx] = E;
+    E = v > E ? v : E;
+  }
+}
+
+// One wave per segment: the MTF proper.
+__global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
+  __shared__ int32_t E[256];
+  __shared__ uint8_t pos2sym[256];
+  __shared__ uint8_t newlist[256];
+  __shared__ int32_t sym2pos[256];
+  __shared__ uint8_t flag[256];
+  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
+  const int b = d.tc_blk[t];
+  const BlockDesc D = d.desc[b];
+  const int m = d.st[b].m;
+  const int k = t - D.tile_c0;
+  const int start = k * kTile;
+  const int lane = lane_id();
+  if (start >= m) return;
+  const int nuse = d.st[b].n_in_use;
+  for (int q = lane; q < 256; q += 64) {
+    E[q] = d.lo[(int64_t)t * 256 + q];
+    flag[q] = 0;
+  }
+  __syncthreads();
+  // list position of every present symbol: # present symbols more recent
+  for (int q = lane; q < 256; q += 64) {
+    int e = E[q];
+    int p = 0;
+    if (e != (int)0x80000000) {
+      for (int r = 0; r < 256; r++) p += (E[r] > e) ? 1 : 0;
+      sym2pos[q] = p;
+      pos2sym[p] = (uint8_t)q;
+    } else {
+      sym2pos[q] = 255;
+    }
+  }
+  __syncthreads();
+  const int n = min(kTile, m - start);
+  uint8_t* mtf = d.MTF + D.off + start;
+  int fnz = 0x7fffffff;
+  for (int base = 0; base < n; base += 64) {
+    const int i = base + lane;
+    const bool valid = i < n;
+    const uint32_t c = valid ? mtf[i] : 0u;
+    uint64_t vmask = ballot(valid);
+    uint64_t peers = vmask;
+#pragma unroll
+    for (int bb = 0; bb < 8; bb++) {
+      bool bit = (c >> bb) & 1u;
+      uint64_t mm = ballot(bit);
+      peers &= bit ? mm : ~mm;
+    }
+    const uint64_t below = lanemask_lt();
+    const uint64_t pb = peers & below;
+    const bool has_prev = valid && pb != 0;
+    const int prev = has_prev ? 63 - __clzll(pb) : -1;
+    const uint64_t above = valid ? (peers & ~below & ~(1ull << lane)) : 0ull;
+    const int nxt = above ? __builtin_ctzll(above) : 64;
+    const int mypos = valid ? sym2pos[c] : 0;
+    const bool first = valid && !has_prev;
+    const int fpos = first ? mypos : -1;
+    int cnt = 0;
+    for (int j = 0; j < 64; j++) {
+      int nj = __builtin_amdgcn_readlane(nxt, j);
+      int fj = __builtin_amdgcn_readlane(fpos, j);
+      if (has_prev) cnt += (j > prev && j < lane && nj >= lane) ? 1 : 0;
+      else cnt += (j < lane && fj > mypos) ? 1 : 0;
+    }
+    const int out = has_prev ? cnt : mypos + cnt;
+    if (valid) mtf[i] = (uint8_t)out;
+    uint64_t nz = ballot(valid && out != 0);
+    if (nz && fnz == 0x7fffffff) fnz = start + base + __builtin_ctzll(nz);
+    // new list: this step's symbols by last occurrence (latest first), then
+    // the untouched symbols in their old order
+    const uint64_t firstm = ballot(first);
+    const int K = __popcll(firstm);
+    const bool last = valid && nxt == 64;
+    const uint64_t lastm = ballot(last);
+    if (first) flag[mypos] = 1;
+    __syncthreads();
+    int keep[4];
+    int ksum = 0;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      int q = lane * 4 + r;
+      keep[r] = (q < nuse && !flag[q]) ? 1 : 0;
+      ksum += keep[r];
+    }
+    int kex = wave_incl_sum(ksum) - ksum;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      int q = lane * 4 + r;
+      if (keep[r]) newlist[K + kex] = pos2sym[q];
+      kex += keep[r];
+    }
+    if (last) newlist[__popcll(lastm >> lane) - 1] = (uint8_t)c;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      int q = lane * 4 + r;
+      if (q < nuse) {
+        uint8_t sy = newlist[q];
+        pos2sym[q] = sy;
+        sym2pos[sy] = q;
+      }
+      flag[q] = 0;
+    }
+    __syncthreads();
+  }
+  if (lane == 0) d.tc_fnz[t] = fnz;
+}
+
+__device__ inline int ilog2(uint32_t v) { return 31 - __clz(v); }
+
+// RLE2 count / write over capacity tiles.
+template <bool WRITE>
+__device__ inline void rle2_tile(const Dev& d) {
+  __shared__ int32_t tmp[8];
+  __shared__ int32_t carry;
+  __shared__ int32_t xe[256];
+  __shared__ int32_t hist[260];
+  const int tid = threadIdx.x;
+  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
+  const int b = d.tc_blk[t];
+  const BlockDesc D = d.desc[b];
+  const int m = d.st[b].m;
+  const int k = t - D.tile_c0;
+  const int start = k * kTile;
+  if (start >= m) {
+    if (!WRITE && tid == 0) d.tc_cnt[t] = 0;
+    return;
+  }
+  const int nt = tiles_for(m);
+  if (WRITE) for (int q = tid; q < 260; q += 256) hist[q] = 0;
+  {
+    int f = 0x7fffffff;
+    for (int j = k + 1 + tid; j < nt; j += 256) {
+      int v = d.tc_fnz[D.tile_c0 + j];
+      if (v != 0x7fffffff) { f = v; break; }
+    }
+    f = wave_min(f);
+    if (lane_id() == 0) tmp[tid >> 6] = f;
+    __syncthreads();
+    if (tid == 0) {
+      int ff = min(min(tmp[0], tmp[1]), min(tmp[2], tmp[3]));
+      carry = ff == 0x7fffffff ? m : ff;
+    }
+    __syncthreads();
+  }
+  const int n = min(kTile, m - start);
+  const uint8_t* src = d.MTF + D.off + start;
+  const int q0 = tid * 16;
+  uint8_t x[16];
+  int firstNZ = 0x7fffffff;
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    x[q] = (q0 + q < n) ? src[q0 + q] : 1;
+  }
+  for (int q = 15; q >= 0; q--)
+    if (q0 + q < n && x[q] != 0) firstNZ = start + q0 + q;
+  int incl = wg_rev_incl_min<256, int>(firstNZ, tmp, carry);
+  xe[tid] = incl;
+  __syncthreads();
+  int es = tid < 255 ? xe[tid + 1] : carry;
+  uint8_t prevb = q0 > 0 ? src[q0 - 1] : (start > 0 ? src[-1] : 1);
+  int e_of[16];
+  {
+    int nb = es;
+    for (int q = 15; q >= 0; q--) {
+      e_of[q] = nb;
+      if (q0 + q < n && x[q] != 0) nb = start + q0 + q;
+    }
+  }
+  int cs[16];
+  int cnt = 0;
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    int c = 0;
+    if (q0 + q < n) {
+      uint8_t pv = q == 0 ? prevb : x[q - 1];
+      if (x[q] != 0) c = 1;
+      else if (pv != 0 || start + q0 + q == 0) c = ilog2((uint32_t)(e_of[q] - (start + q0 + q)) + 1);
+    }
+    cs[q] = c;
+    cnt += c;
+  }
+  int total;
+  int excl = wg_excl_sum<256, int>(cnt, tmp, &total);
+  if (!WRITE) {
+    if (tid == 0) d.tc_cnt[t] = total;
+    return;
+  }
+  uint16_t* dst = d.SYM + D.off + d.tc_off[t] + excl;
+  int o = 0;
+  for (int q = 0; q < 16; q++) {
+    if (!cs[q]) continue;
+    if (x[q] != 0) {
+      uint16_t v = (uint16_t)(x[q] + 1);
+      dst[o++] = v;
+      atomicAdd(&hist[v], 1);
+    } else {
+      uint32_t kk = (uint32_t)(e_of[q] - (start + q0 + q)) + 1;
+      for (int r = 0; r < cs[q]; r++) {
+        uint16_t v = (uint16_t)((kk >> r) & 1u);
+        dst[o++] = v;
+        atomicAdd(&hist[v], 1);
+      }
+    }
+  }
+  __syncthreads();
+  for (int q = tid; q < 260; q += 256)
+    if (hist[q]) atomicAdd(&d.freq[(int64_t)b * 260 + q], hist[q]);
+}
+
+__global__ void __launch_bounds__(256) k_rle2_count(Dev d) { rle2_tile<false>(d); }
+__global__ void __launch_bounds__(256) k_rle2_write(Dev d) { rle2_tile<true>(d); }
+
+void launch_mtf(const Dev& d, hipStream_t s) {
+  k_mtf_gather<<<d.ntile_c, 256, 0, s>>>(d);
+  k_mtf_scan<<<d.nblk, 256, 0, s>>>(d);
+  k_mtf_apply<<<d.ntile_c, 64, 0, s>>>(d);
+}
+
+void launch_rle2(const Dev& d, hipStream_t s) {
+  k_rle2_count<<<d.ntile_c, 256, 0, s>>>(d);
+  launch_tile_scan_c(d, s);
+  k_rle2_write<<<d.ntile_c, 256, 0, s>>>(d);
+}
+
+void launch_mtf_rle2(const Dev& d, hipStream_t s) {
+  launch_mtf(d, s);
+  launch_rle2(d, s);
+}
+
+}  // namespace bzg

@@ -1,0 +1,130 @@
+// pipeline.h - device views of one batch and the stage launchers.
+//
+// A batch is a list of blocks (each <= 900000 raw bytes) from one or more
+// output streams.  Every stage kernel works on the whole batch at once: a
+// batch of one block is the plain per-block path, a batch of many blocks is
+// the batched path.  All buffers live in one device workspace; per-block
+// arrays are indexed by BlockDesc::off (capacity-sized slots) so no stage
+// needs a host round trip to place its output.
+#pragma once
+
+#include "common.h"
+
+namespace bzg {
+
+struct Seg {           // an unresolved group of the suffix sort
+  int32_t blk;         // block index in the batch
+  int32_t start;       // first SA index (block-local)
+  int32_t len;         // number of rotations in the group (>= 2)
+  int32_t pad;
+};
+
+constexpr int kPack = 2048;          // elements per small-group sort workgroup
+constexpr int kHalf = kPack / 2;     // groups longer than this use the tile path
+constexpr int kScanWG = 512;         // fixed grid of the seg-list scans
+constexpr int kHdrFixedBits = 48 + 32 + 1 + 24 + 16 + 16 * 16 + 3 + 15;
+
+__host__ __device__ inline int32_t sel_cap_for(int32_t cap) { return cap / kGSize + 2; }
+__host__ __device__ inline int64_t pay_words_for(int32_t cap) { return (int64_t)cap * 17 / 32 + 8; }
+__host__ __device__ inline int64_t hdr_words_for(int32_t cap) {
+  return ((int64_t)sel_cap_for(cap) * 7 + kMaxGroups * (5 + kMaxAlpha * 33) + kHdrFixedBits) / 32 + 8;
+}
+
+struct Dev {
+  // batch description
+  const BlockDesc* desc;
+  BlockState* st;
+  StreamState* ss;
+  const uint8_t* in;
+  int32_t nblk, nstream;
+  int32_t ntile_n, ntile_c;
+  int64_t total_cap;
+  // tile -> block maps
+  const int32_t* tn_blk;
+  const int32_t* tc_blk;
+  // raw-tile scratch
+  uint32_t* tn_crc;
+  int32_t* tn_fb;
+  int32_t* tn_lb;
+  int32_t* tn_cnt;
+  int32_t* tn_off;
+  // capacity-tile scratch
+  int32_t* tc_cnt;
+  int32_t* tc_off;
+  int32_t* tc_fnz;
+  int32_t* lo;         // [ntile_c][256] MTF last-occurrence summaries
+  // capacity-sized block arrays
+  uint8_t* T;          // RLE1 output
+  uint32_t* SA;
+  uint32_t* RK;
+  uint32_t* KEY;
+  uint32_t* SA2;
+  uint32_t* KEY2;
+  uint8_t* MTF;
+  uint16_t* SYM;
+  // Huffman
+  int32_t* freq;       // [nblk][260]
+  int32_t* rfreq;      // [nblk][6][260]
+  uint8_t* len;        // [nblk][6][260]
+  uint32_t* code;      // [nblk][6][260]
+  uint8_t* sel;        // sel_off-indexed
+  uint8_t* selmtf;
+  int32_t* gbits;      // per 50-group bit count, then offsets (sel_off-indexed)
+  uint32_t* pay;       // payload words (w_off-indexed)
+  uint32_t* hdr;       // header words (h_off = per-block stride hdr_stride)
+  int64_t hdr_stride;  // words per block in hdr
+  // suffix sort work lists
+  Seg* seg[2];
+  int64_t seg_cap;
+  int32_t* pa;         // per-seg exclusive prefix of small-group lengths
+  int32_t* pb;         // per-seg exclusive prefix of large-group tile counts
+  int32_t* pc;         // per-seg exclusive prefix of large-group count
+  int32_t* packs;      // pack -> first seg index
+  int32_t* ltile_seg;  // large tile -> seg index
+  int32_t* ltile_pos;  // large tile -> first element within its seg
+  int32_t* llist;      // compact list of large seg indices
+  uint32_t* hist;      // [ltile][256] digit histograms / scatter offsets
+  int32_t* lsum;       // [ltile][2] first/last group boundary in tile
+  int64_t ltile_cap;
+  int64_t pack_cap;
+  int32_t* ctr;        // counters: [0],[1] list sizes; [2..] plan totals
+  int32_t* scan_part;  // [3][kScanWG] scan partials
+  // output
+  uint8_t* out;
+  int64_t out_cap;
+};
+
+// counter slots
+enum : int {
+  CTR_LIST0 = 0,
+  CTR_LIST1 = 1,
+  CTR_SMALL = 2,   // total elements in small groups
+  CTR_LTILES = 3,  // total tiles of large groups
+  CTR_NLARGE = 4,  // number of large groups
+  CTR_NSEG = 5,    // number of groups in the current list
+  CTR_ERR = 6,
+  CTR_N = 16
+};
+
+// ---- launchers (each enqueues kernels on `s`, no host synchronisation)
+void launch_block_init(const Dev& d, hipStream_t s);
+void launch_crc_rle1(const Dev& d, hipStream_t s);   // CRC + RLE1 -> T, m, present
+void launch_bwt_init(const Dev& d, hipStream_t s);   // SA = identity, first group list
+// one suffix-sort round; returns via host the number of groups processed
+void launch_bwt_plan(const Dev& d, int cur, int nseg, hipStream_t s);
+void launch_bwt_round(const Dev& d, int cur, int round, int32_t P, int key_bits,
+                      int32_t npacks, int32_t ntiles, int32_t nlarge, int32_t nseg,
+                      hipStream_t s);
+void launch_mtf_rle2(const Dev& d, hipStream_t s);   // MTF, orig_ptr, RLE2 -> SYM, s, freq
+void launch_mtf(const Dev& d, hipStream_t s);        // L gather, MTF, orig_ptr, n_in_use
+void launch_rle2(const Dev& d, hipStream_t s);       // RLE2 (needs tc_fnz, n_in_use)
+void launch_tile_scan_c(const Dev& d, hipStream_t s);
+// helpers for the per-stage entry points
+void launch_presence(const Dev& d, hipStream_t s);   // st.present from T[0..m)
+void launch_fnz(const Dev& d, hipStream_t s);        // tc_fnz from MTF[0..m)
+void launch_sym_hist(const Dev& d, hipStream_t s);   // freq from SYM[0..s)
+void launch_huffman(const Dev& d, hipStream_t s);    // tables, selectors, codes, headers
+void launch_pack(const Dev& d, hipStream_t s);       // payload words per block
+void launch_assemble(const Dev& d, hipStream_t s);   // stream layout + placement into out
+
+}  // namespace bzg

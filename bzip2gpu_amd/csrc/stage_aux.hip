@@ -1,0 +1,63 @@
+// stage_aux.hip - small kernels that let a single stage run on its own
+// (the per-stage entry points start from a stage's input, not from raw
+// bytes, so the summaries an earlier stage would have produced are rebuilt).
+#include "pipeline.h"
+
+namespace bzg {
+
+__global__ void __launch_bounds__(256) k_presence(Dev d) {
+  __shared__ uint32_t pres[8];
+  const int t = blockIdx.x;
+  const int b = d.tc_blk[t];
+  const BlockDesc D = d.desc[b];
+  const int m = d.st[b].m;
+  const int start = (t - D.tile_c0) * kTile;
+  if (threadIdx.x < 8) pres[threadIdx.x] = 0;
+  __syncthreads();
+  for (int e = threadIdx.x; e < kTile && start + e < m; e += 256) {
+    uint8_t c = d.T[D.off + start + e];
+    atomicOr(&pres[c >> 5], 1u << (c & 31));
+  }
+  __syncthreads();
+  if (threadIdx.x < 8 && pres[threadIdx.x]) atomicOr(&d.st[b].present[threadIdx.x], pres[threadIdx.x]);
+}
+
+__global__ void __launch_bounds__(256) k_fnz(Dev d) {
+  __shared__ int32_t tmp[4];
+  const int t = blockIdx.x;
+  const int b = d.tc_blk[t];
+  const BlockDesc D = d.desc[b];
+  const int m = d.st[b].m;
+  const int start = (t - D.tile_c0) * kTile;
+  int f = 0x7fffffff;
+  for (int e = threadIdx.x; e < kTile && start + e < m; e += 256)
+    if (d.MTF[D.off + start + e] != 0) f = min(f, start + e);
+  f = wave_min(f);
+  if (lane_id() == 0) tmp[threadIdx.x >> 6] = f;
+  __syncthreads();
+  if (threadIdx.x == 0) d.tc_fnz[t] = min(min(tmp[0], tmp[1]), min(tmp[2], tmp[3]));
+}
+
+__global__ void __launch_bounds__(256) k_sym_hist(Dev d) {
+  __shared__ int32_t h[260];
+  const int t = blockIdx.x;
+  const int b = d.tc_blk[t];
+  const BlockDesc D = d.desc[b];
+  const int s = d.st[b].s;
+  const int start = (t - D.tile_c0) * kTile;
+  for (int q = threadIdx.x; q < 260; q += 256) h[q] = 0;
+  __syncthreads();
+  for (int e = threadIdx.x; e < kTile && start + e < s; e += 256) {
+    uint16_t v = d.SYM[D.off + start + e];
+    if (v < 260) atomicAdd(&h[v], 1);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < 260; q += 256)
+    if (h[q]) atomicAdd(&d.freq[(int64_t)b * 260 + q], h[q]);
+}
+
+void launch_presence(const Dev& d, hipStream_t s) { k_presence<<<d.ntile_c, 256, 0, s>>>(d); }
+void launch_fnz(const Dev& d, hipStream_t s) { k_fnz<<<d.ntile_c, 256, 0, s>>>(d); }
+void launch_sym_hist(const Dev& d, hipStream_t s) { k_sym_hist<<<d.ntile_c, 256, 0, s>>>(d); }
+
+}  // namespace bzg

@@ -1,0 +1,150 @@
+/*
+ * bzg.h - C ABI of the MI355X bzip2 block-compression path.
+ *
+ * Plain C types only (pointers, sizes, status codes); streams are passed as
+ * opaque `void*` (a hipStream_t, or NULL for the null stream).  Every entry
+ * point returns a bzg_status; bzg_last_error() gives the message of the last
+ * failure on the calling thread.  The HIP kernels behind these functions are
+ * the only implementation: there is no CPU fallback.
+ *
+ * Each entry point names the reference interface it replaces
+ * (SebastianBoteroLeonik/bzip2gpu).  Device pointers ("d_") must point to
+ * device memory of the current device; host pointers ("h_") to host memory.
+ */
+#ifndef BZG_H
+#define BZG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t bzg_status;
+#define BZG_OK 0
+#define BZG_ERR_INVALID 1  /* bad argument (size, level, null pointer) */
+#define BZG_ERR_HIP 2      /* a HIP runtime call failed */
+#define BZG_ERR_CAPACITY 3 /* output buffer too small */
+#define BZG_ERR_INTERNAL 4 /* a device-side invariant failed */
+
+/* Raw bytes per block of the reference's stream driver: the input is cut
+ * into fixed 900000-byte chunks at every level (src/main.cpp:26,36,
+ * src/utils/io.cpp:17-18). */
+#define BZG_REF_BLOCK_BYTES 900000
+
+/* Version string of this library. */
+const char *bzg_version(void);
+/* Message of the last failure on this thread ("" if none). */
+const char *bzg_last_error(void);
+/* Selects and initialises a device (optional; device 0 by default). */
+bzg_status bzg_init(int device);
+
+/* ---------------------------------------------------------------- whole path
+ * Replace bzip2_gpu_compress (include/compression.h:86-87,
+ * src/compression/compress.cu:184-375): one .bz2 stream with header digit
+ * `level` (clamped to 1..9) from `len` raw bytes cut into `block_bytes`
+ * blocks (BZG_REF_BLOCK_BYTES reproduces the reference; 100000*level gives
+ * stock-sized blocks).  Empty input yields the 14-byte empty stream.
+ */
+/* Worst-case size of the stream produced for `len` raw bytes cut into
+ * `block_bytes` blocks. */
+size_t bzg_compress_bound(size_t len, int32_t block_bytes);
+
+/* Batched path: all blocks of the input in one device pipeline; stream
+ * assembly on the device.  Host buffers in, host buffer out. */
+bzg_status bzg_compress_buffer(const uint8_t *h_in, size_t len, int level,
+                               int32_t block_bytes, uint8_t *h_out,
+                               size_t out_cap, size_t *out_len);
+
+/* Plain path: one block per pipeline launch on one stream, blocks
+ * serialised on the host in order (the reference's compress_block +
+ * BitWriter structure, compress.cu:26-182).  Same output bytes. */
+bzg_status bzg_compress_plain(const uint8_t *h_in, size_t len, int level,
+                              int32_t block_bytes, uint8_t *h_out,
+                              size_t out_cap, size_t *out_len);
+
+/* Batched path over many independent inputs already resident on the
+ * device: input i is d_in[in_off[i] .. in_off[i]+lens[i]) and becomes its
+ * own stream with level levels[i].  block_bytes <= 0 selects 100000*level
+ * per input.  On return *d_out points to a device buffer owned by the
+ * library (valid until the next call on this device) holding stream i at
+ * h_byte_off[i], h_bytes[i] bytes long. */
+bzg_status bzg_compress_batch_device(const uint8_t *d_in, int32_t n_inputs,
+                                     const uint64_t *in_off,
+                                     const uint64_t *lens,
+                                     const int32_t *levels,
+                                     int32_t block_bytes, uint8_t **d_out,
+                                     int64_t *h_byte_off, int64_t *h_bytes,
+                                     void *stream);
+
+/* Per-stage times of the last batched call on this thread's device (ms,
+ * HIP events), and the number of suffix-sort rounds. */
+bzg_status bzg_last_stage_times(float *h_ms6, int32_t *h_rounds);
+
+/* ------------------------------------------------------------- stage level
+ * One block at a time, device buffers in and out, like the reference's
+ * stage functions (include/compression.h:17-94).  Outputs are written to
+ * caller-provided device buffers; the callee allocates nothing the caller
+ * must free.
+ */
+
+/* bzip2_crc32 (src/compression/crc32.h:7, crc32.cpp:22-29). */
+bzg_status bzg_crc32(const uint8_t *d_in, int32_t n, uint32_t *h_crc,
+                     void *stream);
+
+/* Stage entry points accept blocks of up to 16 MiB (1 << 24 elements).
+ * Capacity d_out needs for bzg_rle1 on n raw bytes (>= 5*ceil(n/4)). */
+int32_t bzg_rle1_bound(int32_t n);
+/* rle1_compress (include/compression.h:17-18, rle1.cu:60-113). */
+bzg_status bzg_rle1(const uint8_t *d_in, int32_t n, uint8_t *d_out,
+                    int32_t *h_out_len, void *stream);
+
+/* fbwt (include/compression.h:75, fbwt.cu:47-105): d_sa[m] = sorted
+ * cyclic rotations, identical rotations in ascending index order.
+ * *h_rounds = prefix-doubling rounds executed (may be NULL). */
+bzg_status bzg_bwt(const uint8_t *d_in, int32_t m, int32_t *d_sa,
+                   int32_t *h_rounds, void *stream);
+
+/* make_symbols_table (include/compression.h:44-45, fmtf.cu:114-141):
+ * h_table[256] receives the present bytes ascending, *h_count their number. */
+bzg_status bzg_symbols_table(const uint8_t *d_in, int32_t n, uint8_t *h_table,
+                             int32_t *h_count, void *stream);
+
+/* fmtf (include/compression.h:54-56, fmtf.cu:147-230). h_present[256]
+ * may be NULL.  Returns n_in_use in *h_n_in_use. */
+bzg_status bzg_mtf(const uint8_t *d_rle1, const int32_t *d_sa, int32_t m,
+                   uint8_t *d_out, int32_t *h_orig_ptr, uint8_t *h_present,
+                   int32_t *h_n_in_use, void *stream);
+
+/* rle2_compress (include/compression.h:31-32, rle2.cu:98-158): d_out needs
+ * m+1 entries; *d_out_len (device) receives the symbol count incl. EOB. */
+bzg_status bzg_rle2(const uint8_t *d_mtf, int32_t m, int32_t table_size,
+                    uint16_t *d_out, uint32_t *d_out_len, void *stream);
+
+/* count_frequencies (huffman.cu:126-133, called directly by
+ * tests/compression/huffman.cu:7,29): d_freq[258] += histogram. */
+bzg_status bzg_histogram(const uint16_t *d_sym, int32_t s, int32_t *d_freq,
+                         void *stream);
+
+/* huffman_build_trees (include/compression.h:67-72, huffman.cu:256-364):
+ * alphabet_size = n_in_use (2 is added inside, as in the reference).
+ * h_len[6*258], h_code[6*258] row-major; h_sel holds ceil(s/50) bytes. */
+bzg_status bzg_huff_build(const uint16_t *d_sym, int32_t s,
+                          int32_t alphabet_size, uint8_t *h_len,
+                          int32_t *h_code, uint8_t *h_sel,
+                          int32_t *h_n_groups, int32_t *h_n_sel,
+                          void *stream);
+
+/* huffman_encode (include/compression.h:89-94, huffman_encode.cu:72-143):
+ * MSB-first packing into d_words (ceil(17*s/32)+2 words); *h_bits = W. */
+bzg_status bzg_huff_encode(const uint16_t *d_sym, int32_t s,
+                           const uint8_t *h_len, const int32_t *h_code,
+                           const uint8_t *d_sel, int32_t n_sel,
+                           int32_t n_groups, uint32_t *d_words,
+                           int64_t *h_bits, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,47 @@
+// compression.h - drop-in for the reference's stage API
+// (SebastianBoteroLeonik/bzip2gpu include/compression.h:17-94), with
+// hipStream_t in place of cudaStream_t.  Same names, argument meaning and
+// ownership: functions that take `T *&out` allocate it stream-ordered with
+// hipMallocAsync and the caller releases it with hipFreeAsync/hipFree;
+// host arrays returned through `T *&` are new[]'d and the caller delete[]s
+// them.  Any failure prints a message to stderr and exits with
+// EXIT_FAILURE, as the reference's CUDA_ERROR_CHECK does (include/utils.h:3-12).
+#ifndef BZG_COMPRESSION_H
+#define BZG_COMPRESSION_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "io.h"
+
+int rle1_compress(const uint8_t *in, int in_len, uint8_t *&out, hipStream_t stream = 0);
+
+void rle2_compress(const uint8_t *in, int in_len, uint16_t *out, uint32_t *out_len,
+                   int table_size, hipStream_t stream = 0);
+
+int make_symbols_table(const uint8_t *d_in, int d_in_len, uint8_t *&symbols_table,
+                       hipStream_t stream = 0);
+
+int fmtf(const uint8_t *in_original, const int *in_suffix_array, int in_len, uint8_t *&out,
+         int &orig_ptr, bool *present_symbols = nullptr, hipStream_t stream = 0);
+
+constexpr int max_n_groups = 6;
+constexpr int max_alphabet_size = 258;
+
+int huffman_build_trees(uint16_t *device_data_in, int data_in_len, int alphabet_size,
+                        uint8_t len[max_n_groups][max_alphabet_size],
+                        int32_t code[max_n_groups][max_alphabet_size], uint8_t *&selectors,
+                        int &n_groups, hipStream_t stream);
+
+void fbwt(const uint8_t *in, int in_len, int *&out, hipStream_t stream = 0);
+
+void bzip2_gpu_compress(BZFileInputStream &in_stream, int n, std::vector<uint8_t> &out);
+
+int huffman_encode(uint16_t *dev_data_in, int data_in_len, int alphabet_size,
+                   uint32_t *&dev_encoded_data, uint8_t len[max_n_groups][max_alphabet_size],
+                   int32_t code[max_n_groups][max_alphabet_size], uint8_t *dev_selectors,
+                   int32_t num_selectors, int n_groups, hipStream_t stream);
+
+#endif  // BZG_COMPRESSION_H
