@@ -2,9 +2,32 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace bzg {
+
+namespace {
+// BZG_TRACE=1: synchronise after every stage and print a timestamped line
+// to stderr (diagnostics only; never set in timed runs).
+bool trace_on() {
+  static const bool on = [] {
+    const char* v = getenv("BZG_TRACE");
+    return v && v[0] == '1';
+  }();
+  return on;
+}
+void trace(hipStream_t s, const char* what, long a = -1, long b = -1, long c = -1) {
+  if (!trace_on()) return;
+  static auto t0 = std::chrono::steady_clock::now();
+  hipError_t e = hipStreamSynchronize(s);
+  double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  fprintf(stderr, "[bzg %10.3f ms] %s %ld %ld %ld (%s)\n", ms, what, a, b, c, hipGetErrorString(e));
+  fflush(stderr);
+}
+}  // namespace
 
 void launch_tile_scan_c(const Dev& d, hipStream_t s);
 
@@ -230,8 +253,10 @@ int Engine::run_bwt(hipStream_t s, int* rounds_out) {
     const int32_t ntiles = h_ctr_[CTR_LTILES];
     const int32_t nlarge = h_ctr_[CTR_NLARGE];
     const int keybits = round == 0 ? 32 : keybits_late;
+    trace(s, "bwt round start", round, nseg, ntiles);
     launch_bwt_round(d_, cur, round, P, keybits, npacks, ntiles, nlarge, nseg, s);
     BZG_TRY(hipGetLastError());
+    trace(s, "bwt round done", round, npacks, nlarge);
     cur ^= 1;
     P = round == 0 ? 4 : (P >= (1 << 29) ? P : 2 * P);
     round++;
@@ -263,29 +288,40 @@ int Engine::run_assemble(hipStream_t s) {
 int Engine::run(hipStream_t s, bool assemble, StageTimes* times) {
   BZG_TRY(hipSetDevice(device_));
   if (desc_.empty()) {
-    if (assemble && !ss_.empty()) return run_assemble(s);
+    if (assemble && !ss_.empty()) {
+      int rc = run_assemble(s);
+      trace(s, "assemble (no blocks) done");
+      return rc;
+    }
     return 0;
   }
   int rounds = 0;
+  trace(s, "run start", (long)desc_.size(), (long)ss_.size());
   if (times) BZG_TRY(hipEventRecord(ev_[0], s));
   int rc = run_front(s);
   if (rc) return rc;
+  trace(s, "front done");
   if (times) BZG_TRY(hipEventRecord(ev_[1], s));
   rc = run_bwt(s, &rounds);
   if (rc) return rc;
+  trace(s, "bwt done", rounds);
   if (times) BZG_TRY(hipEventRecord(ev_[2], s));
   rc = run_mtf_rle2(s);
   if (rc) return rc;
+  trace(s, "mtf+rle2 done");
   if (times) BZG_TRY(hipEventRecord(ev_[3], s));
   rc = run_huffman(s);
   if (rc) return rc;
+  trace(s, "huffman done");
   if (times) BZG_TRY(hipEventRecord(ev_[4], s));
   rc = run_pack(s);
   if (rc) return rc;
+  trace(s, "pack done");
   if (times) BZG_TRY(hipEventRecord(ev_[5], s));
   if (assemble) {
     rc = run_assemble(s);
     if (rc) return rc;
+    trace(s, "assemble done");
   }
   if (times) {
     BZG_TRY(hipEventRecord(ev_[6], s));

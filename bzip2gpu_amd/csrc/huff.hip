@@ -548,8 +548,9 @@ void launch_pack(const Dev& d, hipStream_t s) {
 }
 
 void launch_assemble(const Dev& d, hipStream_t s) {
+  if (d.nstream == 0) return;
   k_layout<<<1, 256, 0, s>>>(d);
-  k_place<<<d.nblk, 256, 0, s>>>(d);
+  if (d.nblk > 0) k_place<<<d.nblk, 256, 0, s>>>(d);
   k_trailer<<<(d.nstream + 63) / 64, 64, 0, s>>>(d);
 }
 

@@ -61,19 +61,23 @@ __global__ void __launch_bounds__(256) k_mtf_scan(Dev d) {
   const int nt = tiles_for(m);
   for (int k = 0; k < nt; k++) {
     int64_t idx = (int64_t)(D.tile_c0 + k) * 256 + s;
-    int v = d.lo[idx];
+    int v = d.lo[idx];  // -1: not in this segment
     d.lo[idx] = E;
-    E = v > E ? v : E;
+    if (present && v >= 0) E = v;  // later segments hold later positions
   }
 }
 
-// One wave per segment: the MTF proper.
+// One wave per segment: the MTF proper.  The list is kept as sym2pos[]
+// (position of every present byte).  After each 64-symbol step the K
+// distinct bytes of the step take positions 0..K-1 by last occurrence, and
+// every other present byte moves from position p to
+// K + p - #(step bytes whose old position is below p).
 __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
   __shared__ int32_t E[256];
-  __shared__ uint8_t pos2sym[256];
-  __shared__ uint8_t newlist[256];
   __shared__ int32_t sym2pos[256];
-  __shared__ uint8_t flag[256];
+  __shared__ int32_t newpos[256];
+  __shared__ uint32_t cmask[8];   // old positions of this step's bytes
+  __shared__ uint32_t smask[8];   // this step's bytes
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int b = d.tc_blk[t];
   const BlockDesc D = d.desc[b];
@@ -82,23 +86,18 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
   const int start = k * kTile;
   const int lane = lane_id();
   if (start >= m) return;
-  const int nuse = d.st[b].n_in_use;
-  for (int q = lane; q < 256; q += 64) {
-    E[q] = d.lo[(int64_t)t * 256 + q];
-    flag[q] = 0;
-  }
+  for (int q = lane; q < 256; q += 64) E[q] = d.lo[(int64_t)t * 256 + q];
+  if (lane < 8) { cmask[lane] = 0; smask[lane] = 0; }
   __syncthreads();
-  // list position of every present symbol: # present symbols more recent
+  // list position of every present byte: # present bytes more recent
   for (int q = lane; q < 256; q += 64) {
     int e = E[q];
-    int p = 0;
+    int p = -1;  // absent from the block
     if (e != (int)0x80000000) {
+      p = 0;
       for (int r = 0; r < 256; r++) p += (E[r] > e) ? 1 : 0;
-      sym2pos[q] = p;
-      pos2sym[p] = (uint8_t)q;
-    } else {
-      sym2pos[q] = 255;
     }
+    sym2pos[q] = p;
   }
   __syncthreads();
   const int n = min(kTile, m - start);
@@ -108,8 +107,7 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
     const int i = base + lane;
     const bool valid = i < n;
     const uint32_t c = valid ? mtf[i] : 0u;
-    uint64_t vmask = ballot(valid);
-    uint64_t peers = vmask;
+    uint64_t peers = ballot(valid);
 #pragma unroll
     for (int bb = 0; bb < 8; bb++) {
       bool bit = (c >> bb) & 1u;
@@ -136,41 +134,41 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
     if (valid) mtf[i] = (uint8_t)out;
     uint64_t nz = ballot(valid && out != 0);
     if (nz && fnz == 0x7fffffff) fnz = start + base + __builtin_ctzll(nz);
-    // new list: this step's symbols by last occurrence (latest first), then
-    // the untouched symbols in their old order
-    const uint64_t firstm = ballot(first);
-    const int K = __popcll(firstm);
+    // list update
+    const int K = __popcll(ballot(first));
     const bool last = valid && nxt == 64;
     const uint64_t lastm = ballot(last);
-    if (first) flag[mypos] = 1;
-    __syncthreads();
-    int keep[4];
-    int ksum = 0;
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      int q = lane * 4 + r;
-      keep[r] = (q < nuse && !flag[q]) ? 1 : 0;
-      ksum += keep[r];
+    if (first) {
+      atomicOr(&cmask[mypos >> 5], 1u << (mypos & 31));
+      atomicOr(&smask[c >> 5], 1u << (c & 31));
     }
-    int kex = wave_incl_sum(ksum) - ksum;
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      int q = lane * 4 + r;
-      if (keep[r]) newlist[K + kex] = pos2sym[q];
-      kex += keep[r];
-    }
-    if (last) newlist[__popcll(lastm >> lane) - 1] = (uint8_t)c;
+    if (last) newpos[c] = __popcll(lastm >> lane) - 1;
     __syncthreads();
+    uint32_t cm[8];
+#pragma unroll
+    for (int w = 0; w < 8; w++) cm[w] = cmask[w];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      int q = lane * 4 + r;
-      if (q < nuse) {
-        uint8_t sy = newlist[q];
-        pos2sym[q] = sy;
-        sym2pos[sy] = q;
+      const int sy = lane * 4 + r;
+      const int op = sym2pos[sy];
+      if (op < 0) continue;
+      int np;
+      if ((smask[sy >> 5] >> (sy & 31)) & 1u) {
+        np = newpos[sy];
+      } else {
+        int lower = 0;
+#pragma unroll
+        for (int w = 0; w < 8; w++) {
+          int lo_bit = w * 32;
+          if (lo_bit + 32 <= op) lower += __popc(cm[w]);
+          else if (lo_bit < op) lower += __popc(cm[w] & ((1u << (op - lo_bit)) - 1u));
+        }
+        np = K + op - lower;
       }
-      flag[q] = 0;
+      sym2pos[sy] = np;
     }
+    __syncthreads();
+    if (lane < 8) { cmask[lane] = 0; smask[lane] = 0; }
     __syncthreads();
   }
   if (lane == 0) d.tc_fnz[t] = fnz;

@@ -17,6 +17,15 @@ def _case(msg):
     sys.stdout.flush()
 
 
+def _same(a: bytes, b: bytes, what: str) -> None:
+    """Byte equality without pytest's (very slow) diff of large bytes."""
+    if a == b:
+        return
+    n = min(len(a), len(b))
+    first = next((i for i in range(n) if a[i] != b[i]), n)
+    raise AssertionError(f"{what}: lengths {len(a)} vs {len(b)}, first difference at byte {first}")
+
+
 def _inputs():
     import bzip2gpu_amd.datagen as dg
     rng = np.random.default_rng(11)
@@ -42,7 +51,7 @@ def test_compress_matches_oracle(dev, path):
         _case(f"compress {path} {name} len={len(data)}")
         got = fn(data, 9)
         want = oracle.compress(data, 9)
-        assert got == want, name
+        _same(got, want, f"{path} {name}")
 
 
 def test_levels_and_block_sizes(dev):
@@ -53,7 +62,7 @@ def test_levels_and_block_sizes(dev):
         for bb in [900000, 100000 * level]:
             _case(f"level={level} block_bytes={bb}")
             got = bz.compress(data, level, block_bytes=bb)
-            assert got == oracle.compress(data, level, block_bytes=bb)
+            _same(got, oracle.compress(data, level, block_bytes=bb), f"level {level} bb {bb}")
             dec, sizes = bz2dec.decompress(got)
             assert dec == data
             if max(sizes) <= 100000 * level:
@@ -65,7 +74,7 @@ def test_stock_equal_blocks(dev):
     rng = np.random.default_rng(2)
     data = rng.integers(0, 256, 899000, dtype=np.uint8).tobytes()
     _case("stock-equal random block")
-    assert bz.compress(data, 9) == bz2.compress(data, 9)
+    _same(bz.compress(data, 9), bz2.compress(data, 9), "stock-equal")
 
 
 def test_batch_device_many_inputs(dev):
@@ -94,4 +103,4 @@ def test_batch_device_many_inputs(dev):
     host = bytes(host)
     for i, d in enumerate(datas):
         s = host[bo[i]:bo[i] + by[i]]
-        assert s == oracle.compress(d, int(levels[i]), block_bytes=100000 * int(levels[i])), i
+        _same(s, oracle.compress(d, int(levels[i]), block_bytes=100000 * int(levels[i])), f"input {i}")

@@ -123,6 +123,22 @@ def test_mtf_reference(dev, alphabet):
     np.testing.assert_array_equal(out.cpu().numpy(), ref)
 
 
+def test_mtf_late_first_occurrence(dev):
+    """A present byte whose first occurrence is past the first 4096-symbol
+    segment, with most byte values absent from the block."""
+    import torch
+    import bzip2gpu_amd as bz
+    rng = np.random.default_rng(21)
+    L = rng.integers(10, 14, 30000).astype(np.uint8)
+    L[20000] = 200
+    L[25000] = 3
+    n = L.size
+    sa = ((np.arange(n) + 1) % n).astype(np.int32)
+    _case("mtf late first occurrence")
+    out, op, nuse, _ = bz.mtf(_t(dev, L), torch.from_numpy(sa).to(dev))
+    np.testing.assert_array_equal(out.cpu().numpy(), oracle.mtf_plain(L))
+
+
 def test_mtf_after_bwt(dev):
     import bzip2gpu_amd as bz
     import bzip2gpu_amd.datagen as dg
