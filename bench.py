@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark: bzip2 compression input GB/s on one MI355X.
+
+Default workload (mix256): one 256 MiB stream at level 9, cut into
+900000-byte blocks (299 blocks), content = 1 MiB segments drawn 50/25/25
+from order-2 Markov text (Zipf 1.2), uniform bytes and runs with geometric
+lengths of mean 64, seeded.  A step compresses the whole input with the
+batched pipeline (every block in one device pipeline, stream assembled on
+the device), input already resident in HBM.  The plain per-block path (one
+block per pipeline run, host serialisation) is timed once on the same
+input beside it, and both outputs are checked byte-for-byte against each
+other and against the CPU oracle.
+
+Prints one JSON line (see the keys below).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MiB = 1 << 20
+
+
+def _args():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--workload", default="mix256", choices=["mix256", "text1", "periodic64"])
+    p.add_argument("--mib", type=int, default=0, help="override input size (MiB)")
+    p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-check", action="store_true")
+    p.add_argument("--probe", default="k_keys", help="kernel timed live with HIP events for the roofline")
+    p.add_argument("--calib-copy", action="store_true",
+                   help="also run one plain device copy of the input (PMC traffic calibration)")
+    p.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
+                   help="JSON with HBM bytes per launch of the probed kernel, measured by rocprofv3 PMC runs")
+    return p.parse_args()
+
+
+# Algorithmic HBM bytes of one unit of work of each probed kernel, and the
+# name of the unit (per-unit figures restated in DESIGN.md).
+ALG_BYTES = {
+    # suffix-sort key gather: read SA 4 + gather the rank at SA+P 4 + write key 4
+    "k_keys": (12, "bwt_rows"),
+    # LDS sort of a pack: read key 4 + SA 4, write SA 4 + scatter rank 4
+    "k_sort_small": (16, "bwt_rows_small"),
+    # MTF: read last-column byte, write MTF byte
+    "k_mtf_apply": (2, "rle1_bytes"),
+}
+
+
+def _workload(name, mib):
+    import bzip2gpu_amd.datagen as dg
+    if name == "mix256":
+        n = (mib or 256) * MiB
+        return dg.mix(n, dg.SEED0 + 1), {"workload": "mix", "bytes": n, "level": 9, "block_bytes": 900000,
+                                         "content": "1MiB segments: 50% order-2 text Zipf1.2 / 25% uniform / 25% runs mean 64"}
+    if name == "text1":
+        n = (mib or 1) * MiB
+        return dg.markov(n, dg.SEED0, order=1, s=1.1), {"workload": "text1", "bytes": n, "level": 9,
+                                                        "block_bytes": 900000, "content": "order-1 Markov text Zipf1.1"}
+    n = (mib or 64) * MiB
+    return dg.workload_periodic(n), {"workload": "periodic", "bytes": n, "level": 9, "block_bytes": 900000,
+                                     "content": "period 1/2/7/1000/65536 by block, 0.1% mutations"}
+
+
+def _hip_copy_d2h(ptr, nbytes):
+    import numpy as np
+    hip = ctypes.CDLL("libamdhip64.so")
+    out = np.empty(nbytes, dtype=np.uint8)
+    rc = hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ptr), ctypes.c_size_t(nbytes), 2)
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy failed: {rc}")
+    return out.tobytes()
+
+
+def _cpu_baseline(data, level, block_bytes):
+    """The oracle (a restatement of the reference path) on the host cores,
+    block-parallel over min(nproc, 16) threads (the reference's worker cap),
+    on the whole workload (a bounded 10-40 CPU-second sample)."""
+    import oracle
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    t = time.perf_counter()
+    out = oracle.compress(data, level, block_bytes=block_bytes, threads=threads)
+    dt = time.perf_counter() - t
+    return {"value": round(data.size / dt / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"the whole {data.size // MiB} MiB workload, oracle block-parallel on {threads} threads, "
+                      f"{dt:.2f} s wall"}, out
+
+
+def main():
+    a = _args()
+    if a.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {a.gpus}); multi-GPU runs are not part of this build",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import bzip2gpu_amd as bz
+
+    data, cfg = _workload(a.workload, a.mib)
+    level, bb = cfg["level"], cfg["block_bytes"]
+    n = data.size
+    dev = torch.device("cuda:0")
+    d_in = torch.from_numpy(data).to(dev)
+    torch.cuda.synchronize()
+
+    def batched():
+        return bz.compress_batch_device(d_in, [0], [n], [level], bb)
+
+    for _ in range(max(0, a.warmup)):
+        batched()
+    torch.cuda.synchronize()
+    if a.calib_copy:
+        cp = d_in.clone()
+        torch.cuda.synchronize()
+        del cp
+    bz.probe_kernel(a.probe)
+    bz.probe_read()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ts = time.perf_counter()
+        ptr, bo, by = batched()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    probe_ms, probe_n = bz.probe_read()
+    bz.probe_kernel(None)
+    stages, rounds, rows = bz.last_stage_times()
+    ms = wall / a.steps * 1e3
+    value = n / (wall / a.steps) / 1e9
+    out_batched = _hip_copy_d2h(ptr + int(bo[0]), int(by[0]))
+
+    res = {
+        "metric": "input GB/s at 1 MI355X, batched vs plain per-block (bit-exact); p50/p99 latency",
+        "value": round(value, 4),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": dict(cfg, path="batched", blocks=(n + bb - 1) // bb),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+        "stage_ms": {k: round(v, 3) for k, v in stages.items()},
+        "bwt_rounds": rounds,
+        "compressed_bytes": int(by[0]),
+    }
+    per_unit, unit = ALG_BYTES.get(a.probe, (0, ""))
+    units = {"bwt_rows": rows, "rle1_bytes": 0}.get(unit, 0)
+    if probe_n and units and per_unit:
+        alg = per_unit * units  # per step
+        per_launch = alg * a.steps / probe_n
+        avg_s = probe_ms / probe_n / 1e3
+        achieved = per_launch / avg_s / 1e9
+        traffic = None
+        if a.traffic and os.path.exists(a.traffic):
+            with open(a.traffic) as f:
+                traffic = json.load(f).get(a.probe)
+        res["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
+                           "frac": round(achieved / 8000.0, 4), "traffic": traffic,
+                           "kernel": a.probe, "launches": probe_n, "avg_launch_us": round(probe_ms / probe_n * 1e3, 2),
+                           "alg_bytes_per_launch": int(per_launch),
+                           "alg_bytes_rule": f"{per_unit} B x {unit}"}
+    if not a.no_plain:
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out_plain = bz.compress_plain_device(d_in, level, bb)
+        dtp = time.perf_counter() - t
+        res["plain"] = {"value": round(n / dtp / 1e9, 4), "unit": "GB/s", "seconds": round(dtp, 3),
+                        "identical_to_batched": out_plain == out_batched}
+        res["batched_over_plain"] = round(value / (n / dtp / 1e9), 2)
+    if not a.no_cpu:
+        cpu, cpu_out = _cpu_baseline(data, level, bb)
+        res["cpu_baseline"] = cpu
+        res["identical_to_oracle"] = cpu_out == out_batched
+    elif not a.no_check:
+        import oracle
+        threads = max(1, min(os.cpu_count() or 1, 16))
+        res["identical_to_oracle"] = oracle.compress(data, level, block_bytes=bb, threads=threads) == out_batched
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

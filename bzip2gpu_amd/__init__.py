@@ -14,6 +14,7 @@ from ._lib import (  # noqa: F401
     compress_batch_device,
     compress_bound,
     compress_plain,
+    compress_plain_device,
     crc32,
     histogram,
     huff_build,
@@ -22,6 +23,8 @@ from ._lib import (  # noqa: F401
     library_path,
     last_stage_times,
     mtf,
+    probe_kernel,
+    probe_read,
     rle1,
     rle2,
     symbols_table,

@@ -49,8 +49,11 @@ def lib():
         "bzg_compress_bound": ([sz, i32], sz),
         "bzg_compress_buffer": ([P, sz, ctypes.c_int, i32, P, sz, SZP], i32),
         "bzg_compress_plain": ([P, sz, ctypes.c_int, i32, P, sz, SZP], i32),
+        "bzg_compress_plain_device": ([P, sz, ctypes.c_int, i32, P, sz, SZP, P], i32),
         "bzg_compress_batch_device": ([P, i32, P, P, P, i32, ctypes.POINTER(ctypes.c_void_p), P, P, P], i32),
-        "bzg_last_stage_times": ([P, I32P], i32),
+        "bzg_last_stage_times": ([P, I32P, I64P], i32),
+        "bzg_probe_kernel": ([ctypes.c_char_p], i32),
+        "bzg_probe_read": ([ctypes.POINTER(ctypes.c_float), I32P], i32),
         "bzg_crc32": ([P, i32, U32P, P], i32),
         "bzg_rle1_bound": ([i32], i32),
         "bzg_rle1": ([P, i32, P, I32P, P], i32),
@@ -111,6 +114,19 @@ def compress_plain(data, level: int = 9, block_bytes: int = REF_BLOCK_BYTES) -> 
     return _compress(lib().bzg_compress_plain, data, level, block_bytes)
 
 
+def compress_plain_device(d_in, level: int = 9, block_bytes: int = REF_BLOCK_BYTES, stream=None) -> bytes:
+    """Plain path on an input already in device memory (torch uint8 tensor)."""
+    n = d_in.numel()
+    cap = compress_bound(n, block_bytes)
+    out = np.empty(cap, dtype=np.uint8)
+    got = ctypes.c_size_t(0)
+    st = lib().bzg_compress_plain_device(ctypes.c_void_p(d_in.data_ptr()) if n else None, n, level, block_bytes,
+                                         out.ctypes.data_as(ctypes.c_void_p), cap, ctypes.byref(got),
+                                         ctypes.c_void_p(stream) if stream else None)
+    _check(st, "bzg_compress_plain_device")
+    return out[: got.value].tobytes()
+
+
 def compress_batch_device(d_in, offsets, lengths, levels, block_bytes: int = 0, stream=None):
     """Many independent inputs already on the device (torch uint8 tensor).
 
@@ -134,11 +150,26 @@ def compress_batch_device(d_in, offsets, lengths, levels, block_bytes: int = 0, 
 
 
 def last_stage_times():
+    """({stage: ms}, bwt rounds, suffix-sort rows processed) of the last batched call."""
     ms = (ctypes.c_float * 6)()
     r = ctypes.c_int32(0)
-    _check(lib().bzg_last_stage_times(ms, ctypes.byref(r)), "bzg_last_stage_times")
+    rows = ctypes.c_int64(0)
+    _check(lib().bzg_last_stage_times(ms, ctypes.byref(r), ctypes.byref(rows)), "bzg_last_stage_times")
     keys = ["front", "bwt", "mtf_rle2", "huffman", "pack", "assemble"]
-    return dict(zip(keys, list(ms))), int(r.value)
+    return dict(zip(keys, list(ms))), int(r.value), int(rows.value)
+
+
+def probe_kernel(name) -> None:
+    """Time every later launch of kernel `name` with HIP events (None: off)."""
+    _check(lib().bzg_probe_kernel(name.encode() if name else None), "bzg_probe_kernel")
+
+
+def probe_read():
+    """(total ms, launches) of the probed kernel since the last read."""
+    ms = ctypes.c_float(0)
+    n = ctypes.c_int32(0)
+    _check(lib().bzg_probe_read(ctypes.byref(ms), ctypes.byref(n)), "bzg_probe_read")
+    return float(ms.value), int(n.value)
 
 
 # ------------------------------------------------------------- stage level

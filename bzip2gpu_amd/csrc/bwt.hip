@@ -75,26 +75,29 @@ __global__ void __launch_bounds__(256) k_bwt_init(Dev d) {
 }
 
 // ------------------------------------------------------------------ plan
-struct PlanVals { int a, b, c; };
+struct PlanVals { int a, b, c, e; };
 __device__ inline PlanVals plan_vals(const Seg& g) {
   bool small = g.len <= kHalf;
-  return PlanVals{small ? g.len : 0, small ? 0 : tiles_for(g.len), small ? 0 : 1};
+  return PlanVals{small ? g.len : 0, small ? 0 : tiles_for(g.len), small ? 0 : 1, small ? 0 : g.len};
 }
 
 __global__ void __launch_bounds__(256) k_plan_scan1(Dev d, int cur) {
-  __shared__ int32_t tmp[3][8];
+  __shared__ int32_t tmp[4][8];
   const int n = d.ctr[cur];
   const int per = (n + kScanWG - 1) / kScanWG;
   const int i0 = blockIdx.x * per, i1 = min(n, i0 + per);
-  int sa = 0, sb = 0, sc = 0;
+  int sa = 0, sb = 0, sc = 0, se = 0;
   for (int i = i0 + threadIdx.x; i < i1; i += 256) {
     PlanVals v = plan_vals(d.seg[cur][i]);
-    sa += v.a; sb += v.b; sc += v.c;
+    sa += v.a; sb += v.b; sc += v.c; se += v.e;
   }
-  sa = wave_sum(sa); sb = wave_sum(sb); sc = wave_sum(sc);
-  if (lane_id() == 0) { tmp[0][threadIdx.x >> 6] = sa; tmp[1][threadIdx.x >> 6] = sb; tmp[2][threadIdx.x >> 6] = sc; }
+  sa = wave_sum(sa); sb = wave_sum(sb); sc = wave_sum(sc); se = wave_sum(se);
+  if (lane_id() == 0) {
+    tmp[0][threadIdx.x >> 6] = sa; tmp[1][threadIdx.x >> 6] = sb;
+    tmp[2][threadIdx.x >> 6] = sc; tmp[3][threadIdx.x >> 6] = se;
+  }
   __syncthreads();
-  if (threadIdx.x < 3) {
+  if (threadIdx.x < 4) {
     int s = 0;
     for (int w = 0; w < 4; w++) s += tmp[threadIdx.x][w];
     d.scan_part[threadIdx.x * kScanWG + blockIdx.x] = s;
@@ -103,12 +106,12 @@ __global__ void __launch_bounds__(256) k_plan_scan1(Dev d, int cur) {
 
 __global__ void __launch_bounds__(kScanWG) k_plan_scan2(Dev d, int cur) {
   __shared__ int32_t tmp[kScanWG / 64 + 1];
-  for (int q = 0; q < 3; q++) {
+  for (int q = 0; q < 4; q++) {
     int v = d.scan_part[q * kScanWG + threadIdx.x];
     int tot;
     int ex = wg_excl_sum<kScanWG, int>(v, tmp, &tot);
     d.scan_part[q * kScanWG + threadIdx.x] = ex;
-    if (threadIdx.x == 0) d.ctr[CTR_SMALL + q] = tot;
+    if (threadIdx.x == 0) d.ctr[q < 3 ? CTR_SMALL + q : CTR_LELEMS] = tot;
   }
   if (threadIdx.x == 0) d.ctr[CTR_NSEG] = d.ctr[cur];
 }
@@ -615,18 +618,30 @@ void launch_bwt_round(const Dev& d, int cur, int round, int32_t P, int key_bits,
                       hipStream_t s) {
   k_mk_packs<<<(npacks + 1 + 255) / 256, 256, 0, s>>>(d, npacks, nseg);
   k_mk_large<<<(nseg + 255) / 256, 256, 0, s>>>(d, cur, nseg);
-  if (npacks + ntiles > 0) k_keys<<<npacks + ntiles, 256, 0, s>>>(d, cur, round, P, npacks);
-  if (npacks > 0) k_sort_small<<<npacks, 256, 0, s>>>(d, cur, key_bits);
+  if (npacks + ntiles > 0) {
+    probe_begin("k_keys", s);
+    k_keys<<<npacks + ntiles, 256, 0, s>>>(d, cur, round, P, npacks);
+    probe_end("k_keys", s);
+  }
+  if (npacks > 0) {
+    probe_begin("k_sort_small", s);
+    k_sort_small<<<npacks, 256, 0, s>>>(d, cur, key_bits);
+    probe_end("k_sort_small", s);
+  }
   if (ntiles > 0) {
     const int npass = (key_bits + 7) / 8;
     for (int p = 0; p < npass; p++) {
       k_lhist<<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
       k_lscan<<<nlarge, 256, 0, s>>>(d, cur);
+      probe_begin("k_lscatter", s);
       k_lscatter<<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
+      probe_end("k_lscatter", s);
     }
     const int fin = npass & 1;
     k_lsum<<<ntiles, 256, 0, s>>>(d, cur, fin);
+    probe_begin("k_lheads", s);
     k_lheads<<<ntiles, 256, 0, s>>>(d, cur, fin);
+    probe_end("k_lheads", s);
   }
 }
 

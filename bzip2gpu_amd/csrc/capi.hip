@@ -171,33 +171,33 @@ bzg_status bzg_compress_buffer(const uint8_t* h_in, size_t len, int level, int32
   return BZG_OK;
 }
 
-bzg_status bzg_compress_plain(const uint8_t* h_in, size_t len, int level, int32_t block_bytes,
-                              uint8_t* h_out, size_t out_cap, size_t* out_len) {
-  if ((!h_in && len) || !h_out || !out_len) return fail(BZG_ERR_INVALID, "null pointer");
-  if (block_bytes <= 0 || block_bytes > BZG_REF_BLOCK_BYTES) return fail(BZG_ERR_INVALID, "block_bytes out of range");
-  bzg_status st;
-  Ctx* c = ctx_for(g_dev, &st);
-  if (!c) return st;
-  std::lock_guard<std::mutex> lk(c->mu);
-  if (hipSetDevice(c->device) != hipSuccess) return fail(BZG_ERR_HIP, "hipSetDevice failed");
+namespace {
+
+// Shared body of the plain path: d_in is either the caller's device buffer
+// or the context's staging buffer refilled per block from h_in.
+bzg_status plain_run(Ctx* c, const uint8_t* h_in, const uint8_t* d_in, size_t len, int level, int32_t block_bytes,
+                     uint8_t* h_out, size_t out_cap, size_t* out_len, hipStream_t s) {
   level = clamp_level(level);
   std::vector<uint8_t> out;
   out.reserve(len / 2 + 64);
   HostBits bw(out);
   bw.put(0x42, 8); bw.put(0x5A, 8); bw.put(0x68, 8); bw.put((uint32_t)(0x30 + level), 8);
   uint32_t combined = 0;
-  int e = ensure_input(c, (size_t)block_bytes + 16);
-  if (e) return hip_fail(e, "input buffer");
+  int e;
+  if (h_in && (e = ensure_input(c, (size_t)block_bytes + 16))) return hip_fail(e, "input buffer");
   BlockResult r;
   for (size_t off = 0; off < len; off += (size_t)block_bytes) {
     size_t n = len - off < (size_t)block_bytes ? len - off : (size_t)block_bytes;
-    hipError_t he = hipMemcpyAsync(c->d_in, h_in + off, n, hipMemcpyHostToDevice, c->stream);
-    if (he != hipSuccess) return hip_fail((int)he, "H2D");
+    const uint8_t* src = d_in ? d_in + off : c->d_in;
+    if (h_in) {
+      hipError_t he = hipMemcpyAsync(c->d_in, h_in + off, n, hipMemcpyHostToDevice, s);
+      if (he != hipSuccess) return hip_fail((int)he, "H2D");
+    }
     std::vector<StreamSpec> ss{StreamSpec{0, n, level, block_bytes}};
     if ((e = c->eng.plan(ss))) return hip_fail(e, "plan");
-    if ((e = c->eng.setup(c->d_in, c->stream))) return hip_fail(e, "setup");
-    if ((e = c->eng.run(c->stream, false, nullptr))) return hip_fail(e, "run");
-    if ((e = c->eng.block_result(0, r, c->stream))) return hip_fail(e, "block result");
+    if ((e = c->eng.setup(src, s))) return hip_fail(e, "setup");
+    if ((e = c->eng.run(s, false, nullptr))) return hip_fail(e, "run");
+    if ((e = c->eng.block_result(0, r, s))) return hip_fail(e, "block result");
     combined = ((combined << 1) | (combined >> 31)) ^ r.crc;
     bw.put_words(r.hdr, r.hdr_bits);
     bw.put_words(r.words, r.bits);
@@ -209,6 +209,34 @@ bzg_status bzg_compress_plain(const uint8_t* h_in, size_t len, int level, int32_
   if (out.size() > out_cap) return fail(BZG_ERR_CAPACITY, "output buffer too small");
   memcpy(h_out, out.data(), out.size());
   return BZG_OK;
+}
+
+}  // namespace
+
+bzg_status bzg_compress_plain(const uint8_t* h_in, size_t len, int level, int32_t block_bytes,
+                              uint8_t* h_out, size_t out_cap, size_t* out_len) {
+  if ((!h_in && len) || !h_out || !out_len) return fail(BZG_ERR_INVALID, "null pointer");
+  if (block_bytes <= 0 || block_bytes > BZG_REF_BLOCK_BYTES) return fail(BZG_ERR_INVALID, "block_bytes out of range");
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(BZG_ERR_HIP, "hipSetDevice failed");
+  return plain_run(c, h_in ? h_in : reinterpret_cast<const uint8_t*>(""), nullptr, len, level, block_bytes, h_out,
+                   out_cap, out_len, c->stream);
+}
+
+bzg_status bzg_compress_plain_device(const uint8_t* d_in, size_t len, int level, int32_t block_bytes,
+                                     uint8_t* h_out, size_t out_cap, size_t* out_len, void* stream) {
+  if ((!d_in && len) || !h_out || !out_len) return fail(BZG_ERR_INVALID, "null pointer");
+  if (block_bytes <= 0 || block_bytes > BZG_REF_BLOCK_BYTES) return fail(BZG_ERR_INVALID, "block_bytes out of range");
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(BZG_ERR_HIP, "hipSetDevice failed");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  return plain_run(c, nullptr, d_in, len, level, block_bytes, h_out, out_cap, out_len, s);
 }
 
 bzg_status bzg_compress_batch_device(const uint8_t* d_in, int32_t n_inputs, const uint64_t* in_off,
@@ -243,7 +271,21 @@ bzg_status bzg_compress_batch_device(const uint8_t* d_in, int32_t n_inputs, cons
   return BZG_OK;
 }
 
-bzg_status bzg_last_stage_times(float* h_ms6, int32_t* h_rounds) {
+bzg_status bzg_probe_kernel(const char* kernel_name) {
+  probe_set(kernel_name);
+  return BZG_OK;
+}
+
+bzg_status bzg_probe_read(float* h_total_ms, int32_t* h_launches) {
+  if (!h_total_ms || !h_launches) return fail(BZG_ERR_INVALID, "null pointer");
+  int n = 0;
+  int e = probe_read(h_total_ms, &n);
+  if (e) return hip_fail(e, "probe");
+  *h_launches = n;
+  return BZG_OK;
+}
+
+bzg_status bzg_last_stage_times(float* h_ms6, int32_t* h_rounds, int64_t* h_bwt_rows) {
   bzg_status st;
   Ctx* c = ctx_for(g_dev, &st);
   if (!c) return st;
@@ -252,6 +294,7 @@ bzg_status bzg_last_stage_times(float* h_ms6, int32_t* h_rounds) {
     h_ms6[3] = c->last.huff; h_ms6[4] = c->last.pack; h_ms6[5] = c->last.assemble;
   }
   if (h_rounds) *h_rounds = c->last.rounds;
+  if (h_bwt_rows) *h_bwt_rows = c->last.bwt_elems;
   return BZG_OK;
 }
 

@@ -34,7 +34,12 @@ struct BlockResult {
 struct StageTimes {
   float front = 0, bwt = 0, mtf = 0, huff = 0, pack = 0, assemble = 0;
   int rounds = 0;
+  int64_t bwt_elems = 0;  // rows keyed and sorted, summed over rounds
 };
+
+// per-kernel HIP-event probe (one target kernel name at a time)
+void probe_set(const char* kernel);
+int probe_read(float* total_ms, int* launches);
 
 class Engine {
  public:
@@ -88,6 +93,7 @@ class Engine {
   std::vector<int32_t> tn_blk_, tc_blk_;
   int64_t total_cap_ = 0, sel_tot_ = 0, w_tot_ = 0, out_cap_ = 0;
   int32_t max_cap_ = 0;
+  int64_t active_elems_ = 0;
   hipEvent_t ev_[8] = {};
   int init_err_ = 0;
 };

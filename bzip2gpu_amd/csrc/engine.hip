@@ -6,6 +6,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
+#include <utility>
 
 namespace bzg {
 
@@ -28,6 +30,63 @@ void trace(hipStream_t s, const char* what, long a = -1, long b = -1, long c = -
   fflush(stderr);
 }
 }  // namespace
+
+// ------------------------------------------------------------------ probe
+namespace {
+struct ProbeState {
+  std::string target;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  size_t used = 0;
+  bool open = false;
+};
+ProbeState& probe() {
+  static ProbeState p;
+  return p;
+}
+}  // namespace
+
+void probe_begin(const char* kernel, hipStream_t s) {
+  ProbeState& P = probe();
+  if (P.target.empty() || P.target != kernel) return;
+  if (P.used == P.ev.size()) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return;
+    if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
+    P.ev.emplace_back(a, b);
+  }
+  if (hipEventRecord(P.ev[P.used].first, s) == hipSuccess) P.open = true;
+}
+
+void probe_end(const char* kernel, hipStream_t s) {
+  ProbeState& P = probe();
+  if (!P.open || P.target != kernel) return;
+  if (hipEventRecord(P.ev[P.used].second, s) == hipSuccess) P.used++;
+  P.open = false;
+}
+
+void probe_set(const char* kernel) {
+  ProbeState& P = probe();
+  P.target = kernel ? kernel : "";
+  P.used = 0;
+  P.open = false;
+}
+
+int probe_read(float* total_ms, int* launches) {
+  ProbeState& P = probe();
+  float tot = 0;
+  for (size_t i = 0; i < P.used; i++) {
+    hipError_t e = hipEventSynchronize(P.ev[i].second);
+    if (e != hipSuccess) return (int)e;
+    float ms = 0;
+    e = hipEventElapsedTime(&ms, P.ev[i].first, P.ev[i].second);
+    if (e != hipSuccess) return (int)e;
+    tot += ms;
+  }
+  *total_ms = tot;
+  *launches = (int)P.used;
+  P.used = 0;
+  return 0;
+}
 
 void launch_tile_scan_c(const Dev& d, hipStream_t s);
 
@@ -129,6 +188,7 @@ struct Carver {
   }
 };
 }  // namespace
+
 
 int Engine::setup(const uint8_t* d_in, hipStream_t s) {
   BZG_TRY(hipSetDevice(device_));
@@ -234,6 +294,7 @@ int Engine::run_front(hipStream_t s) {
 }
 
 int Engine::run_bwt(hipStream_t s, int* rounds_out) {
+  active_elems_ = 0;
   launch_bwt_init(d_, s);
   BZG_TRY(hipGetLastError());
   int cur = 0, round = 0;
@@ -253,6 +314,7 @@ int Engine::run_bwt(hipStream_t s, int* rounds_out) {
     const int32_t ntiles = h_ctr_[CTR_LTILES];
     const int32_t nlarge = h_ctr_[CTR_NLARGE];
     const int keybits = round == 0 ? 32 : keybits_late;
+    active_elems_ += (int64_t)small + (int64_t)h_ctr_[CTR_LELEMS];
     trace(s, "bwt round start", round, nseg, ntiles);
     launch_bwt_round(d_, cur, round, P, keybits, npacks, ntiles, nlarge, nseg, s);
     BZG_TRY(hipGetLastError());
@@ -329,6 +391,7 @@ int Engine::run(hipStream_t s, bool assemble, StageTimes* times) {
     float* f[6] = {&times->front, &times->bwt, &times->mtf, &times->huff, &times->pack, &times->assemble};
     for (int i = 0; i < 6; i++) BZG_TRY(hipEventElapsedTime(f[i], ev_[i], ev_[i + 1]));
     times->rounds = rounds;
+    times->bwt_elems = active_elems_;
   }
   return 0;
 }

@@ -535,8 +535,12 @@ __global__ void __launch_bounds__(64) k_trailer(Dev d) {
 void launch_huffman(const Dev& d, hipStream_t s) {
   k_huff_init<<<d.nblk, 64, 0, s>>>(d);
   for (int it = 0; it < 4; it++) {
+    probe_begin("k_huff_select", s);
     k_huff_select<<<d.ntile_c, 256, 0, s>>>(d);
+    probe_end("k_huff_select", s);
+    probe_begin("k_huff_lengths", s);
     k_huff_lengths<<<d.nblk, 64, 0, s>>>(d);
+    probe_end("k_huff_lengths", s);
   }
   k_huff_codes<<<d.nblk, 64, 0, s>>>(d);
 }
@@ -544,7 +548,9 @@ void launch_huffman(const Dev& d, hipStream_t s) {
 void launch_pack(const Dev& d, hipStream_t s) {
   k_group_bits<<<d.ntile_c, 256, 0, s>>>(d);
   k_group_scan<<<d.nblk, 256, 0, s>>>(d);
+  probe_begin("k_pack", s);
   k_pack<<<d.ntile_c, 256, 0, s>>>(d);
+  probe_end("k_pack", s);
 }
 
 void launch_assemble(const Dev& d, hipStream_t s) {

@@ -282,7 +282,9 @@ __global__ void __launch_bounds__(256) k_rle2_write(Dev d) { rle2_tile<true>(d);
 void launch_mtf(const Dev& d, hipStream_t s) {
   k_mtf_gather<<<d.ntile_c, 256, 0, s>>>(d);
   k_mtf_scan<<<d.nblk, 256, 0, s>>>(d);
+  probe_begin("k_mtf_apply", s);
   k_mtf_apply<<<d.ntile_c, 64, 0, s>>>(d);
+  probe_end("k_mtf_apply", s);
 }
 
 void launch_rle2(const Dev& d, hipStream_t s) {

@@ -94,6 +94,11 @@ struct Dev {
   int64_t out_cap;
 };
 
+// ---- per-kernel timing probe (HIP events around the launches of one named
+// kernel on the launching stream; off unless a target name is set)
+void probe_begin(const char* kernel, hipStream_t s);
+void probe_end(const char* kernel, hipStream_t s);
+
 // counter slots
 enum : int {
   CTR_LIST0 = 0,
@@ -103,6 +108,7 @@ enum : int {
   CTR_NLARGE = 4,  // number of large groups
   CTR_NSEG = 5,    // number of groups in the current list
   CTR_ERR = 6,
+  CTR_LELEMS = 7,  // total elements in large groups
   CTR_N = 16
 };
 

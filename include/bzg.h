@@ -64,6 +64,14 @@ bzg_status bzg_compress_plain(const uint8_t *h_in, size_t len, int level,
                               int32_t block_bytes, uint8_t *h_out,
                               size_t out_cap, size_t *out_len);
 
+/* Plain path with the input already resident on the device (d_in, len
+ * bytes); each block is compressed by its own pipeline run on `stream` and
+ * its result copied back for host serialisation, as above. */
+bzg_status bzg_compress_plain_device(const uint8_t *d_in, size_t len, int level,
+                                     int32_t block_bytes, uint8_t *h_out,
+                                     size_t out_cap, size_t *out_len,
+                                     void *stream);
+
 /* Batched path over many independent inputs already resident on the
  * device: input i is d_in[in_off[i] .. in_off[i]+lens[i]) and becomes its
  * own stream with level levels[i].  block_bytes <= 0 selects 100000*level
@@ -79,8 +87,18 @@ bzg_status bzg_compress_batch_device(const uint8_t *d_in, int32_t n_inputs,
                                      void *stream);
 
 /* Per-stage times of the last batched call on this thread's device (ms,
- * HIP events), and the number of suffix-sort rounds. */
-bzg_status bzg_last_stage_times(float *h_ms6, int32_t *h_rounds);
+ * HIP events), the number of suffix-sort rounds, and the suffix-sort rows
+ * keyed and sorted, summed over rounds. */
+bzg_status bzg_last_stage_times(float *h_ms6, int32_t *h_rounds,
+                                int64_t *h_bwt_rows);
+
+/* Kernel timing probe (no counterpart in the reference; for benchmarks):
+ * bzg_probe_kernel(name) makes every later launch of the named kernel be
+ * bracketed by HIP events on its stream ("" or NULL turns it off);
+ * bzg_probe_read returns the summed duration and launch count since the
+ * last read. */
+bzg_status bzg_probe_kernel(const char *kernel_name);
+bzg_status bzg_probe_read(float *h_total_ms, int32_t *h_launches);
 
 /* ------------------------------------------------------------- stage level
  * One block at a time, device buffers in and out, like the reference's
