@@ -246,13 +246,13 @@ __device__ inline uint32_t make_key(const Dev& d, uint64_t off, int m, int j, in
 // ------------------------------------------------------------------- keys
 __global__ void __launch_bounds__(256) k_keys(Dev d, int cur, int round, int P, int npacks) {
   __shared__ PackLds L;
-  const int w = blockIdx.x;
+  const int w = (int)xcd_remap(blockIdx.x, gridDim.x);
   if (w < npacks) {
     load_pack(d, cur, w, L);
     if (L.count == 0) return;
     const int total = L.total;
     for (int j = threadIdx.x; j < L.count; j += 256)
-      atomicMax(&d.st[L.blk[j]].rounds, round + 1);
+      if (j == 0 || L.blk[j] != L.blk[j - 1]) atomicMax(&d.st[L.blk[j]].rounds, round + 1);
     for (int e = threadIdx.x; e < total; e += 256) {
       int s = find_seg(L, e);
       int b = L.blk[s];
@@ -275,6 +275,27 @@ __global__ void __launch_bounds__(256) k_keys(Dev d, int cur, int round, int P, 
       d.KEY[off + j] = make_key(d, off, m, j, round, P);
     }
   }
+}
+
+// Appends groups to the next list: each thread holds `cnt` groups in g[].
+template <int N>
+__device__ inline void emit_groups_n(const Dev& d, int nxt, int cnt, const Seg* g, int32_t* tmp, int32_t* s_base) {
+  int tot;
+  int ex = wg_excl_sum<256, int>(cnt, tmp, &tot);
+  if (threadIdx.x == 0 && tot > 0) *s_base = atomicAdd(&d.ctr[nxt], tot);
+  __syncthreads();
+  if (tot > 0) {
+    const int base = *s_base + ex;
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+      if (q < cnt) {
+        int idx = base + q;
+        if (idx < d.seg_cap) d.seg[nxt][idx] = g[q];
+        else d.ctr[CTR_ERR] = 1;
+      }
+    }
+  }
+  __syncthreads();
 }
 
 // Appends groups to the next list: each thread has 0/1 group to emit.
@@ -301,7 +322,7 @@ __global__ void __launch_bounds__(256) k_sort_small(Dev d, int cur, int keybits)
   __shared__ int32_t s_base;
   __shared__ int32_t tmp[8];
   const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
-  const int k = blockIdx.x;
+  const int k = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int nxt = cur ^ 1;
   load_pack(d, cur, k, L);
   if (L.count == 0) return;
@@ -394,27 +415,29 @@ __global__ void __launch_bounds__(256) k_sort_small(Dev d, int cur, int keybits)
       d.RK[off + v] = (uint32_t)hj;
     }
   }
+  Seg out[4];
+  int nout = 0;
+  (void)has;
+  (void)ng;
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     int e = e0 + q;
-    has = false;
     if (e < total) {
       bool end = (e == total - 1) || (Ks[e + 1] != Ks[e]);
       int glen = e - head_of[q] + 1;
-      if (end && glen > 1) {
+      if (end && glen > 1) {  // >= 2 rows each, so <= 4 end in 8 rows
         int s = (int)(Ks[e] >> keybits);
-        has = true;
-        ng = Seg{L.blk[s], L.start[s] + (head_of[q] - L.lofs[s]), glen, 0};
+        out[nout++ & 3] = Seg{L.blk[s], L.start[s] + (head_of[q] - L.lofs[s]), glen, 0};
       }
     }
-    emit_groups(d, nxt, has, ng, tmp, &s_base);
   }
+  emit_groups_n<4>(d, nxt, nout, out, tmp, &s_base);
 }
 
 // ----------------------------------------------------------- large groups
 __global__ void __launch_bounds__(256) k_lhist(Dev d, int cur, int shift, int src) {
   __shared__ uint32_t h[256];
-  const int t = blockIdx.x;
+  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const Seg g = d.seg[cur][d.ltile_seg[t]];
   const int pos = d.ltile_pos[t];
   const uint64_t off = d.desc[g.blk].off;
@@ -451,7 +474,7 @@ __global__ void __launch_bounds__(256) k_lscan(Dev d, int cur) {
 __global__ void __launch_bounds__(256) k_lscatter(Dev d, int cur, int shift, int src) {
   __shared__ uint32_t cnt[4][256];
   const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
-  const int t = blockIdx.x;
+  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const Seg g = d.seg[cur][d.ltile_seg[t]];
   const int pos = d.ltile_pos[t];
   const uint64_t off = d.desc[g.blk].off;
@@ -496,7 +519,7 @@ __global__ void __launch_bounds__(256) k_lscatter(Dev d, int cur, int shift, int
 // first / last group boundary (seg-relative) of each large tile
 __global__ void __launch_bounds__(256) k_lsum(Dev d, int cur, int fin) {
   __shared__ int32_t tmp[8];
-  const int t = blockIdx.x;
+  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const Seg g = d.seg[cur][d.ltile_seg[t]];
   const int pos = d.ltile_pos[t];
   const uint64_t base = d.desc[g.blk].off + g.start;
@@ -524,7 +547,7 @@ __global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
   __shared__ int32_t s_carry, s_base;
   __shared__ int32_t xs[256];
   const int tid = threadIdx.x;
-  const int t = blockIdx.x;
+  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int si = d.ltile_seg[t];
   const Seg g = d.seg[cur][si];
   const int pos = d.ltile_pos[t];
@@ -592,21 +615,23 @@ __global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
       d.RK[off + v] = (uint32_t)(g.start + head_of[q]);
     }
   }
+  Seg out[8];
+  int nout = 0;
+  (void)has;
+  (void)ng;
 #pragma unroll
   for (int q = 0; q < 16; q++) {
     int e = e0 + q;
-    has = false;
     if (e < n) {
       int p = pos + e;
       bool end = (p == g.len - 1) || (kk[q + 1] != kk[q]);
       int glen = p - head_of[q] + 1;
-      if (end && glen > 1) {
-        has = true;
-        ng = Seg{g.blk, g.start + head_of[q], glen, 0};
-      }
+      // a group ending here has >= 2 rows, and only the first one can
+      // start before this thread's rows, so <= 8 end in 16 rows
+      if (end && glen > 1) out[nout++ & 7] = Seg{g.blk, g.start + head_of[q], glen, 0};
     }
-    emit_groups(d, nxt, has, ng, tmp, &s_base);
   }
+  emit_groups_n<8>(d, nxt, nout, out, tmp, &s_base);
 }
 
 void launch_bwt_init(const Dev& d, hipStream_t s) {

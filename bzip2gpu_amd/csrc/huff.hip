@@ -109,10 +109,15 @@ __global__ void __launch_bounds__(256) k_huff_select(Dev d) {
     (&rf[0][0])[q] = 0;
   }
   __syncthreads();
+  // stage this tile's groups' symbols (coalesced) in LDS
+  __shared__ uint16_t S[(kTile / kGSize + 2) * kGSize];
+  const int sbase = g0 * kGSize, send = min(g1 * kGSize, s);
+  for (int i = sbase + (int)threadIdx.x; i < send; i += 256) S[i - sbase] = d.SYM[D.off + i];
+  __syncthreads();
   const int g = g0 + (int)threadIdx.x;
   if (g < g1) {
     const int gs = g * kGSize, ge = min(gs + kGSize, s);
-    const uint16_t* sym = d.SYM + D.off;
+    const uint16_t* sym = S - sbase;
     uint32_t cost[6] = {0, 0, 0, 0, 0, 0};
     for (int i = gs; i < ge; i++) {
       uint16_t v = sym[i];
@@ -365,11 +370,15 @@ __global__ void __launch_bounds__(256) k_group_bits(Dev d) {
   if (g0 >= g1) return;
   for (int q = threadIdx.x; q < 6 * 260; q += 256) (&L[0][0])[q] = d.len[(int64_t)b * 6 * 260 + q];
   __syncthreads();
+  __shared__ uint16_t S[(kTile / kGSize + 2) * kGSize];
+  const int sbase = g0 * kGSize, send = min(g1 * kGSize, s);
+  for (int i = sbase + (int)threadIdx.x; i < send; i += 256) S[i - sbase] = d.SYM[D.off + i];
+  __syncthreads();
   const int g = g0 + (int)threadIdx.x;
   if (g < g1) {
     const int gs = g * kGSize, ge = min(gs + kGSize, s);
     const int tt = d.sel[D.sel_off + g];
-    const uint16_t* sym = d.SYM + D.off;
+    const uint16_t* sym = S - sbase;
     int nb = 0;
     for (int i = gs; i < ge; i++) nb += L[tt][sym[i]];
     d.gbits[D.sel_off + g] = nb;
@@ -413,11 +422,15 @@ __global__ void __launch_bounds__(256) k_pack(Dev d) {
     (&C[0][0])[q] = d.code[(int64_t)b * 6 * 260 + q];
   }
   __syncthreads();
+  __shared__ uint16_t S[(kTile / kGSize + 2) * kGSize];
+  const int sbase = g0 * kGSize, send = min(g1 * kGSize, s);
+  for (int i = sbase + (int)threadIdx.x; i < send; i += 256) S[i - sbase] = d.SYM[D.off + i];
+  __syncthreads();
   const int g = g0 + (int)threadIdx.x;
   if (g < g1) {
     const int gs = g * kGSize, ge = min(gs + kGSize, s);
     const int tt = d.sel[D.sel_off + g];
-    const uint16_t* sym = d.SYM + D.off;
+    const uint16_t* sym = S - sbase;
     const int64_t a = d.gbits[D.sel_off + g];
     const int64_t e = g + 1 < nsel ? d.gbits[D.sel_off + g + 1] : d.st[b].bits;
     BitOut o;
