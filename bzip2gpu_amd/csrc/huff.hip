@@ -139,78 +139,99 @@ __global__ void __launch_bounds__(256) k_huff_select(Dev d) {
   }
 }
 
-// libbz2 code-length method, one thread per table (huffman.cu:51-124).
+// libbz2 code-length method, one thread per table (huffman.cu:51-124):
+// binary min-heap on weight = freq<<8 | depth with the same sift rules, so
+// ties resolve exactly as in the reference.  Heap entries carry their
+// weight next to the node id (one LDS load per comparison), and depths are
+// assigned top-down over the node list (parents have larger ids than their
+// children) instead of walking each leaf's parent chain.
 __global__ void __launch_bounds__(64) k_huff_lengths(Dev d) {
-  __shared__ int32_t heap_s[6][260];
-  __shared__ int32_t weight_s[6][520];
+  __shared__ int2 heap_s[6][260];
   __shared__ int32_t parent_s[6][520];
+  __shared__ int32_t leafw_s[6][260];
   const int b = blockIdx.x;
   const int tt = threadIdx.x;
   const int ng = d.st[b].n_groups;
   if (tt >= ng) return;
   const int alpha = d.st[b].n_in_use + 2;
-  int32_t* heap = heap_s[tt];
-  int32_t* weight = weight_s[tt];
-  int32_t* parent = parent_s[tt];
+  int2* heap = heap_s[tt];
+  int32_t* parent = parent_s[tt];  // reused as depth in the length pass
+  int32_t* leafw = leafw_s[tt];
   int32_t* fr = d.rfreq + ((int64_t)b * 6 + tt) * 260;
   uint8_t* len = d.len + ((int64_t)b * 6 + tt) * 260;
   for (int i = 0; i < alpha; i++) {
     int f = fr[i];
-    weight[i + 1] = (f == 0 ? 1 : f) << 8;
+    leafw[i + 1] = (f == 0 ? 1 : f) << 8;
     fr[i] = 0;  // ready for the next iteration
   }
   while (true) {
     int nNodes = alpha, nHeap = 0;
-    heap[0] = 0; weight[0] = 0; parent[0] = -2;
+    heap[0] = make_int2(0, 0);
     for (int i = 1; i <= alpha; i++) {
       parent[i] = -1;
       nHeap++;
-      heap[nHeap] = i;
-      int zz = nHeap, tmp = heap[zz];
-      while (weight[tmp] < weight[heap[zz >> 1]]) { heap[zz] = heap[zz >> 1]; zz >>= 1; }
+      int2 tmp = make_int2(leafw[i], i);
+      int zz = nHeap;
+      while (true) {
+        int2 up = heap[zz >> 1];
+        if (!(tmp.x < up.x)) break;
+        heap[zz] = up;
+        zz >>= 1;
+      }
       heap[zz] = tmp;
     }
     while (nHeap > 1) {
-      int pick[2];
+      int2 pick[2];
       for (int q = 0; q < 2; q++) {
         pick[q] = heap[1];
-        heap[1] = heap[nHeap];
+        int2 tmp = heap[nHeap];
         nHeap--;
-        int zz = 1, tmp = heap[zz];
+        int zz = 1;
         while (true) {
           int yy = zz << 1;
           if (yy > nHeap) break;
-          if (yy < nHeap && weight[heap[yy + 1]] < weight[heap[yy]]) yy++;
-          if (weight[tmp] < weight[heap[yy]]) break;
-          heap[zz] = heap[yy];
+          int2 c = heap[yy];
+          if (yy < nHeap) {
+            int2 c2 = heap[yy + 1];
+            if (c2.x < c.x) { yy++; c = c2; }
+          }
+          if (tmp.x < c.x) break;
+          heap[zz] = c;
           zz = yy;
         }
         heap[zz] = tmp;
       }
       nNodes++;
-      parent[pick[0]] = parent[pick[1]] = nNodes;
-      int w1 = weight[pick[0]], w2 = weight[pick[1]];
+      parent[pick[0].y] = nNodes;
+      parent[pick[1].y] = nNodes;
+      int w1 = pick[0].x, w2 = pick[1].x;
       int d1 = w1 & 0xff, d2 = w2 & 0xff;
-      weight[nNodes] = ((w1 & 0xffffff00) + (w2 & 0xffffff00)) | (1 + (d1 > d2 ? d1 : d2));
+      int2 tmp = make_int2(((w1 & 0xffffff00) + (w2 & 0xffffff00)) | (1 + (d1 > d2 ? d1 : d2)), nNodes);
       parent[nNodes] = -1;
       nHeap++;
-      heap[nHeap] = nNodes;
-      int zz = nHeap, tmp = heap[zz];
-      while (weight[tmp] < weight[heap[zz >> 1]]) { heap[zz] = heap[zz >> 1]; zz >>= 1; }
+      int zz = nHeap;
+      while (true) {
+        int2 up = heap[zz >> 1];
+        if (!(tmp.x < up.x)) break;
+        heap[zz] = up;
+        zz >>= 1;
+      }
       heap[zz] = tmp;
     }
+    // depth of every node, root (nNodes) first
     bool tooLong = false;
+    parent[nNodes] = 0;
+    for (int v = nNodes - 1; v >= 1; v--) parent[v] = parent[parent[v]] + 1;
     for (int i = 1; i <= alpha; i++) {
-      int j = 0, kk = i;
-      while (parent[kk] >= 0) { kk = parent[kk]; j++; }
+      int j = parent[i];
       len[i - 1] = (uint8_t)j;
       if (j > 17) tooLong = true;
     }
     if (!tooLong) break;
     for (int i = 1; i <= alpha; i++) {
-      int j = weight[i] >> 8;
+      int j = leafw[i] >> 8;
       j = 1 + (j / 2);
-      weight[i] = j << 8;
+      leafw[i] = j << 8;
     }
   }
 }

@@ -67,6 +67,24 @@ __global__ void __launch_bounds__(256) k_mtf_scan(Dev d) {
   }
 }
 
+// Transpose of a 64x64 bit matrix held one row per lane (bit c of lane r is
+// element (r, c)): six stages that swap the off-diagonal s x s blocks.
+__device__ inline uint64_t bit_transpose64(uint64_t x) {
+  const int lane = lane_id();
+  const uint64_t hi[6] = {0xFFFFFFFF00000000ull, 0xFFFF0000FFFF0000ull, 0xFF00FF00FF00FF00ull,
+                          0xF0F0F0F0F0F0F0F0ull, 0xCCCCCCCCCCCCCCCCull, 0xAAAAAAAAAAAAAAAAull};
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int sft = 32 >> k;
+    uint32_t lo32 = __shfl_xor((uint32_t)x, sft, 64);
+    uint32_t hi32 = __shfl_xor((uint32_t)(x >> 32), sft, 64);
+    uint64_t p = ((uint64_t)hi32 << 32) | lo32;
+    if (lane & sft) x = (x & hi[k]) | ((p >> sft) & ~hi[k]);
+    else x = (x & ~hi[k]) | ((p << sft) & hi[k]);
+  }
+  return x;
+}
+
 // One wave per segment: the MTF proper.  The list is kept as sym2pos[]
 // (position of every present byte).  After each 64-symbol step the K
 // distinct bytes of the step take positions 0..K-1 by last occurrence, and
@@ -122,20 +140,40 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
     const int nxt = above ? __builtin_ctzll(above) : 64;
     const int mypos = valid ? sym2pos[c] : 0;
     const bool first = valid && !has_prev;
-    const int fpos = first ? mypos : -1;
+    // Repeated byte: distinct bytes strictly between its previous lane and
+    // this lane = lanes j in (prev, lane) whose next occurrence is >= lane.
+    // Row j of a 64x64 bit matrix marks the lanes i with j < i <= nxt_j;
+    // its transpose gives, for lane i, the set of such j.
+    uint64_t row = 0;
+    if (valid) {
+      uint64_t upto = nxt >= 63 ? ~0ull : ((2ull << nxt) - 1ull);
+      row = upto & ~((2ull << lane) - 1ull);
+      if (lane == 63) row = 0;
+    }
+    uint64_t col = bit_transpose64(row);
     int cnt = 0;
-    for (int j = 0; j < 64; j++) {
-      int nj = __builtin_amdgcn_readlane(nxt, j);
-      int fj = __builtin_amdgcn_readlane(fpos, j);
-      if (has_prev) cnt += (j > prev && j < lane && nj >= lane) ? 1 : 0;
-      else cnt += (j < lane && fj > mypos) ? 1 : 0;
+    if (has_prev) {
+      uint64_t between = lanemask_lt() & ~((2ull << prev) - 1ull);
+      cnt = __popcll(col & between);
+    }
+    // First occurrence: its old position, plus the first occurrences in
+    // earlier lanes whose old position was behind it.
+    uint64_t fm = ballot(first);
+    const int K = __popcll(fm);
+    {
+      uint64_t f = fm;
+      while (f) {
+        int j = __builtin_ctzll(f);
+        f &= f - 1;
+        int fj = __builtin_amdgcn_readlane(mypos, j);
+        if (first && j < lane && fj > mypos) cnt++;
+      }
     }
     const int out = has_prev ? cnt : mypos + cnt;
     if (valid) mtf[i] = (uint8_t)out;
     uint64_t nz = ballot(valid && out != 0);
     if (nz && fnz == 0x7fffffff) fnz = start + base + __builtin_ctzll(nz);
     // list update
-    const int K = __popcll(ballot(first));
     const bool last = valid && nxt == 64;
     const uint64_t lastm = ballot(last);
     if (first) {
