@@ -31,7 +31,8 @@ def _args():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="mix256", choices=["mix256", "text1", "periodic64"])
+    p.add_argument("--workload", default="mix256", choices=["mix256", "text1", "periodic64", "batch4096"])
+    p.add_argument("--inputs", type=int, default=4096, help="batch4096: number of inputs")
     p.add_argument("--mib", type=int, default=0, help="override input size (MiB)")
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
@@ -71,6 +72,84 @@ def _workload(name, mib):
                                      "content": "period 1/2/7/1000/65536 by block, 0.1% mutations"}
 
 
+def _batch_inputs(n_inputs, pool_bytes, seed=20261205):
+    """batch4096: input i is a slice of the seeded mix pool; size
+    floor(exp(U(ln 1024, ln 8 MiB))), level U{1..9}, blocks of 100000*level."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(1024), np.log(8 * MiB), n_inputs))).astype(np.int64)
+    levels = rng.integers(1, 10, n_inputs).astype(np.int32)
+    offs = (rng.random(n_inputs) * (pool_bytes - sizes)).astype(np.int64)
+    return offs, sizes, levels
+
+
+def run_batch(a):
+    import numpy as np
+    import torch
+    import bzip2gpu_amd as bz
+    import bzip2gpu_amd.datagen as dg
+    pool = dg.mix(256 * MiB, dg.SEED0 + 2)
+    offs, sizes, levels = _batch_inputs(a.inputs, pool.size)
+    total = int(sizes.sum())
+    d_pool = torch.from_numpy(pool).cuda()
+    torch.cuda.synchronize()
+
+    def batched():
+        return bz.compress_batch_device(d_pool, offs.astype(np.uint64), sizes.astype(np.uint64), levels, 0)
+
+    for _ in range(max(0, a.warmup)):
+        batched()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ts = time.perf_counter()
+        ptr, bo, by = batched()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+    wall = time.perf_counter() - t0
+    stages, rounds, rows = bz.last_stage_times()
+    value = total / (wall / a.steps) / 1e9
+    end = int(bo[-1] + by[-1])
+    allout = _hip_copy_d2h(ptr, end)
+    outs = [allout[int(bo[i]):int(bo[i] + by[i])] for i in range(a.inputs)]
+    res = {
+        "metric": "input GB/s at 1 MI355X, batched vs plain per-block (bit-exact); p50/p99 latency",
+        "value": round(value, 4), "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": "batch", "inputs": a.inputs, "bytes": total,
+                   "sizes": "log-uniform 1 KiB..8 MiB", "levels": "U{1..9}", "block_bytes": "100000*level",
+                   "content": "slices of a seeded 256 MiB mix pool", "path": "batched"},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+        "stage_ms": {k: round(v, 3) for k, v in stages.items()}, "bwt_rounds": rounds,
+        "compressed_bytes": int(sum(by)),
+    }
+    if not a.no_plain:
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        same = True
+        for i in range(a.inputs):
+            o = bz.compress_plain_device(d_pool[int(offs[i]):int(offs[i] + sizes[i])], int(levels[i]),
+                                         100000 * int(levels[i]))
+            same = same and (o == outs[i])
+        dtp = time.perf_counter() - t
+        res["plain"] = {"value": round(total / dtp / 1e9, 4), "unit": "GB/s", "seconds": round(dtp, 3),
+                        "identical_to_batched": bool(same)}
+        res["batched_over_plain"] = round(value / (total / dtp / 1e9), 2)
+    if not a.no_check:
+        import oracle
+        idx = np.random.default_rng(5).choice(a.inputs, size=min(64, a.inputs), replace=False)
+        ok = True
+        for i in idx:
+            want = oracle.compress(pool[int(offs[i]):int(offs[i] + sizes[i])], int(levels[i]),
+                                   block_bytes=100000 * int(levels[i]), threads=8)
+            ok = ok and (want == outs[int(i)])
+        res["identical_to_oracle"] = bool(ok)
+        res["oracle_sample_inputs"] = int(idx.size)
+    print(json.dumps(res), flush=True)
+
+
 def _hip_copy_d2h(ptr, nbytes):
     import numpy as np
     hip = ctypes.CDLL("libamdhip64.so")
@@ -101,6 +180,8 @@ def main():
         print(f"bench.py runs on exactly one GPU (asked for {a.gpus}); multi-GPU runs are not part of this build",
               file=sys.stderr)
         sys.exit(2)
+    if a.workload == "batch4096":
+        return run_batch(a)
     import numpy as np
     import torch
     import bzip2gpu_amd as bz
@@ -160,6 +241,11 @@ def main():
         "bwt_rounds": rounds,
         "compressed_bytes": int(by[0]),
     }
+    rs = bz.last_round_stats()
+    # suffix-sort rounds: rows sorted, GPU ms, and GB/s at 20 algorithmic
+    # bytes per row per round (read SA 4 + gather 2 ranks 8 + write SA 4 + rank 4)
+    res["bwt_round_stats"] = [{"rows": r, "ms": round(t, 3), "gbps_alg": round(20 * r / (t * 1e-3) / 1e9, 1) if t else None}
+                              for r, t in rs]
     per_unit, unit = ALG_BYTES.get(a.probe, (0, ""))
     units = {"bwt_rows": rows, "rle1_bytes": 0}.get(unit, 0)
     if probe_n and units and per_unit:

@@ -21,12 +21,14 @@ from ._lib import (  # noqa: F401
     huff_encode,
     lib,
     library_path,
+    last_round_stats,
     last_stage_times,
     mtf,
     probe_kernel,
     probe_read,
     rle1,
     rle2,
+    set_max_batch_bytes,
     symbols_table,
     version,
 )

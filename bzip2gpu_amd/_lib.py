@@ -51,6 +51,8 @@ def lib():
         "bzg_compress_plain": ([P, sz, ctypes.c_int, i32, P, sz, SZP], i32),
         "bzg_compress_plain_device": ([P, sz, ctypes.c_int, i32, P, sz, SZP, P], i32),
         "bzg_compress_batch_device": ([P, i32, P, P, P, i32, ctypes.POINTER(ctypes.c_void_p), P, P, P], i32),
+        "bzg_last_round_stats": ([P, P, i32], i32),
+        "bzg_set_max_batch_bytes": ([ctypes.c_uint64], i32),
         "bzg_last_stage_times": ([P, I32P, I64P], i32),
         "bzg_probe_kernel": ([ctypes.c_char_p], i32),
         "bzg_probe_read": ([ctypes.POINTER(ctypes.c_float), I32P], i32),
@@ -147,6 +149,18 @@ def compress_batch_device(d_in, offsets, lengths, levels, block_bytes: int = 0, 
         ctypes.c_void_p(stream) if stream else None)
     _check(st, "bzg_compress_batch_device")
     return dout.value, bo, by
+
+
+def last_round_stats(max_rounds: int = 64):
+    """[(rows, gpu ms)] of every suffix-sort round of the last pipeline run."""
+    rows = np.zeros(max_rounds, dtype=np.int64)
+    ms = np.zeros(max_rounds, dtype=np.float32)
+    n = lib().bzg_last_round_stats(rows.ctypes.data_as(ctypes.c_void_p), ms.ctypes.data_as(ctypes.c_void_p), max_rounds)
+    return [(int(rows[i]), float(ms[i])) for i in range(max(0, min(n, max_rounds)))]
+
+
+def set_max_batch_bytes(n: int) -> None:
+    _check(lib().bzg_set_max_batch_bytes(int(n)), "bzg_set_max_batch_bytes")
 
 
 def last_stage_times():

@@ -28,6 +28,9 @@ struct Ctx {
   size_t d_in_cap = 0;
   hipStream_t stream = nullptr;
   StageTimes last{};
+  uint64_t max_batch_bytes = 1ull << 30;  // raw bytes per device sub-batch
+  uint8_t* d_res = nullptr;                // multi-sub-batch results
+  size_t d_res_cap = 0;
 };
 
 std::mutex g_mu;
@@ -83,6 +86,23 @@ int ensure_input(Ctx* c, size_t bytes) {
   hipError_t e = hipMalloc(&c->d_in, want);
   if (e != hipSuccess) return (int)e;
   c->d_in_cap = want;
+  return 0;
+}
+
+int ensure_result(Ctx* c, size_t bytes) {
+  if (bytes <= c->d_res_cap) return 0;
+  uint8_t* nb = nullptr;
+  size_t want = bytes + bytes / 4 + 4096;
+  hipError_t e = hipMalloc(&nb, want);
+  if (e != hipSuccess) return (int)e;
+  if (c->d_res) {
+    // keep what earlier sub-batches of this call already placed
+    e = hipMemcpy(nb, c->d_res, c->d_res_cap, hipMemcpyDeviceToDevice);
+    if (e == hipSuccess) e = hipFree(c->d_res);
+    if (e != hipSuccess) return (int)e;
+  }
+  c->d_res = nb;
+  c->d_res_cap = want;
   return 0;
 }
 
@@ -251,23 +271,85 @@ bzg_status bzg_compress_batch_device(const uint8_t* d_in, int32_t n_inputs, cons
   std::lock_guard<std::mutex> lk(c->mu);
   if (hipSetDevice(c->device) != hipSuccess) return fail(BZG_ERR_HIP, "hipSetDevice failed");
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  // Inputs are taken in order in sub-batches of at most max_batch_bytes raw
+  // bytes (bounded workspace); every sub-batch's streams are copied into
+  // the library-owned result buffer, 4-byte aligned, in input order.
+  const uint64_t max_batch = c->max_batch_bytes;
   std::vector<StreamSpec> ss;
-  ss.reserve(n_inputs);
-  for (int i = 0; i < n_inputs; i++) {
-    int lv = clamp_level(levels[i]);
-    ss.push_back(StreamSpec{in_off[i], lens[i], lv, block_bytes > 0 ? block_bytes : 100000 * lv});
-  }
-  int e;
-  if ((e = c->eng.plan(ss))) return hip_fail(e, "plan");
-  if ((e = c->eng.setup(d_in, s))) return hip_fail(e, "setup");
-  if ((e = c->eng.run(s, true, &c->last))) return hip_fail(e, "run");
   std::vector<StreamState> lay;
-  if ((e = c->eng.stream_layout(lay, s))) return hip_fail(e, "layout");
-  for (int i = 0; i < n_inputs; i++) {
-    h_byte_off[i] = lay[i].byte_off;
-    h_bytes[i] = lay[i].bytes;
+  int64_t out_pos = 0;
+  int e;
+  StageTimes sum{};
+  int i0 = 0;
+  while (i0 < n_inputs) {
+    ss.clear();
+    uint64_t tot = 0;
+    int i1 = i0;
+    while (i1 < n_inputs && (i1 == i0 || tot + lens[i1] <= max_batch)) {
+      int lv = clamp_level(levels[i1]);
+      ss.push_back(StreamSpec{in_off[i1], lens[i1], lv, block_bytes > 0 ? block_bytes : 100000 * lv});
+      tot += lens[i1];
+      i1++;
+    }
+    if ((e = c->eng.plan(ss))) return hip_fail(e, "plan");
+    if ((e = c->eng.setup(d_in, s))) return hip_fail(e, "setup");
+    StageTimes t{};
+    if ((e = c->eng.run(s, true, &t))) return hip_fail(e, "run");
+    sum.front += t.front; sum.bwt += t.bwt; sum.mtf += t.mtf; sum.huff += t.huff;
+    sum.pack += t.pack; sum.assemble += t.assemble; sum.rounds = std::max(sum.rounds, t.rounds);
+    sum.bwt_elems += t.bwt_elems;
+    if ((e = c->eng.stream_layout(lay, s))) return hip_fail(e, "layout");
+    const bool single = (i0 == 0 && i1 == n_inputs);
+    if (single) {
+      for (int i = i0; i < i1; i++) {
+        h_byte_off[i] = lay[i - i0].byte_off;
+        h_bytes[i] = lay[i - i0].bytes;
+      }
+      *d_out = c->eng.out_buffer();
+    } else {
+      int64_t need = 0;
+      for (auto& L : lay) need += (L.bytes + 3) & ~3ll;
+      if ((e = ensure_result(c, (size_t)(out_pos + need)))) return hip_fail(e, "result buffer");
+      for (int i = i0; i < i1; i++) {
+        const StreamState& L = lay[i - i0];
+        hipError_t he = hipMemcpyAsync(c->d_res + out_pos, c->eng.out_buffer() + L.byte_off, (size_t)L.bytes,
+                                       hipMemcpyDeviceToDevice, s);
+        if (he != hipSuccess) return hip_fail((int)he, "result copy");
+        h_byte_off[i] = out_pos;
+        h_bytes[i] = L.bytes;
+        out_pos += (L.bytes + 3) & ~3ll;
+      }
+      *d_out = c->d_res;
+    }
+    i0 = i1;
   }
-  *d_out = c->eng.out_buffer();
+  if (n_inputs == 0) *d_out = c->eng.out_buffer();
+  hipError_t he = hipStreamSynchronize(s);
+  if (he != hipSuccess) return hip_fail((int)he, "batch");
+  c->last = sum;
+  return BZG_OK;
+}
+
+int32_t bzg_last_round_stats(int64_t* h_rows, float* h_ms, int32_t max_rounds) {
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return -1;
+  const auto& rr = c->eng.round_rows();
+  const auto& rm = c->eng.round_ms();
+  int32_t n = (int32_t)std::min(rr.size(), rm.size());
+  for (int32_t i = 0; i < n && i < max_rounds; i++) {
+    if (h_rows) h_rows[i] = rr[i];
+    if (h_ms) h_ms[i] = rm[i];
+  }
+  return n;
+}
+
+bzg_status bzg_set_max_batch_bytes(uint64_t bytes) {
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->max_batch_bytes = bytes < (1u << 20) ? (1u << 20) : bytes;
   return BZG_OK;
 }
 

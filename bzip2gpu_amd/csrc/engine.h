@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "pipeline.h"
@@ -79,6 +80,9 @@ class Engine {
   uint8_t* out_buffer() const { return d_.out; }
   int64_t out_capacity() const { return d_.out_cap; }
   const std::vector<BlockDesc>& desc() const { return desc_; }
+  // suffix-sort rows and GPU milliseconds of every round of the last run
+  const std::vector<int64_t>& round_rows() const { return round_rows_; }
+  const std::vector<float>& round_ms() const { return round_ms_; }
   int device() const { return device_; }
 
  private:
@@ -94,6 +98,9 @@ class Engine {
   int64_t total_cap_ = 0, sel_tot_ = 0, w_tot_ = 0, out_cap_ = 0;
   int32_t max_cap_ = 0;
   int64_t active_elems_ = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> rev_;
+  std::vector<int64_t> round_rows_;
+  std::vector<float> round_ms_;
   hipEvent_t ev_[8] = {};
   int init_err_ = 0;
 };

@@ -110,6 +110,10 @@ Engine::~Engine() {
   if (h_ctr_) (void)hipHostFree(h_ctr_);
   for (auto& ev : ev_)
     if (ev) (void)hipEventDestroy(ev);
+  for (auto& pr : rev_) {
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
 }
 
 int Engine::plan(const std::vector<StreamSpec>& streams, bool allow_big) {
@@ -295,6 +299,7 @@ int Engine::run_front(hipStream_t s) {
 
 int Engine::run_bwt(hipStream_t s, int* rounds_out) {
   active_elems_ = 0;
+  round_rows_.clear();
   launch_bwt_init(d_, s);
   BZG_TRY(hipGetLastError());
   int cur = 0, round = 0;
@@ -316,14 +321,25 @@ int Engine::run_bwt(hipStream_t s, int* rounds_out) {
     const int keybits = round == 0 ? 32 : keybits_late;
     active_elems_ += (int64_t)small + (int64_t)h_ctr_[CTR_LELEMS];
     trace(s, "bwt round start", round, nseg, ntiles);
+    if (rev_.size() < (size_t)round + 1) {
+      hipEvent_t a = nullptr, b = nullptr;
+      BZG_TRY(hipEventCreate(&a));
+      BZG_TRY(hipEventCreate(&b));
+      rev_.emplace_back(a, b);
+    }
+    BZG_TRY(hipEventRecord(rev_[round].first, s));
     launch_bwt_round(d_, cur, round, P, keybits, npacks, ntiles, nlarge, nseg, s);
     BZG_TRY(hipGetLastError());
+    BZG_TRY(hipEventRecord(rev_[round].second, s));
+    round_rows_.push_back((int64_t)small + (int64_t)h_ctr_[CTR_LELEMS]);
     trace(s, "bwt round done", round, npacks, nlarge);
     cur ^= 1;
     P = round == 0 ? 4 : (P >= (1 << 29) ? P : 2 * P);
     round++;
   }
   if (rounds_out) *rounds_out = round;
+  round_ms_.assign((size_t)round, 0.f);
+  for (int r = 0; r < round; r++) BZG_TRY(hipEventElapsedTime(&round_ms_[r], rev_[r].first, rev_[r].second));
   return 0;
 }
 

@@ -86,11 +86,21 @@ bzg_status bzg_compress_batch_device(const uint8_t *d_in, int32_t n_inputs,
                                      int64_t *h_byte_off, int64_t *h_bytes,
                                      void *stream);
 
+/* Largest raw byte count one device pipeline run takes in the batched
+ * entry points (default 1 GiB); bigger input sets run as consecutive
+ * sub-batches with bounded device memory (about 70 bytes per input byte). */
+bzg_status bzg_set_max_batch_bytes(uint64_t bytes);
+
 /* Per-stage times of the last batched call on this thread's device (ms,
  * HIP events), the number of suffix-sort rounds, and the suffix-sort rows
  * keyed and sorted, summed over rounds. */
 bzg_status bzg_last_stage_times(float *h_ms6, int32_t *h_rounds,
                                 int64_t *h_bwt_rows);
+
+/* Rows sorted and GPU milliseconds of every prefix-doubling round of the
+ * last pipeline run (its last sub-batch); returns the number of rounds and
+ * fills at most max_rounds entries (either array may be NULL). */
+int32_t bzg_last_round_stats(int64_t *h_rows, float *h_ms, int32_t max_rounds);
 
 /* Kernel timing probe (no counterpart in the reference; for benchmarks):
  * bzg_probe_kernel(name) makes every later launch of the named kernel be
