@@ -8,6 +8,7 @@ a GPU is missing, calls raise.
 from ._lib import (  # noqa: F401
     BzgError,
     REF_BLOCK_BYTES,
+    Server,
     build,
     bwt,
     compress,

@@ -52,6 +52,12 @@ def lib():
         "bzg_compress_plain_device": ([P, sz, ctypes.c_int, i32, P, sz, SZP, P], i32),
         "bzg_compress_batch_device": ([P, i32, P, P, P, i32, ctypes.POINTER(ctypes.c_void_p), P, P, P], i32),
         "bzg_last_round_stats": ([P, P, i32], i32),
+        "bzg_server_create": ([i32, i32, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)], i32),
+        "bzg_server_destroy": ([P], None),
+        "bzg_server_submit": ([P, P, ctypes.c_uint64, i32, I64P], i32),
+        "bzg_server_wait": ([P, i64, I64P, I64P, I64P], i32),
+        "bzg_server_keep": ([P, P, ctypes.c_uint64], i32),
+        "bzg_server_replay": ([P, P, i32, P, P, P, P, P, P, P, P], i32),
         "bzg_set_max_batch_bytes": ([ctypes.c_uint64], i32),
         "bzg_last_stage_times": ([P, I32P, I64P], i32),
         "bzg_probe_kernel": ([ctypes.c_char_p], i32),
@@ -291,3 +297,64 @@ def huff_encode(d_sym, ln, cd, sel, n_groups: int):
                                  n_groups, _ptr(words), ctypes.byref(bits), _stream()), "bzg_huff_encode")
     nw = (bits.value + 31) // 32
     return words[:nw], int(bits.value)
+
+
+class Server:
+    """Continuous-batching request loop (C++), see bzg_server_* in bzg.h.
+
+    mode "batched": queued requests that fit are compressed together, one
+    pipeline run per worker at a time; mode "plain": each request alone on
+    the per-block path.
+    """
+
+    def __init__(self, mode: str = "batched", workers: int = 4, max_batch_bytes: int = 256 << 20,
+                 hbm_budget_bytes: int = 200 * 10**9):
+        h = ctypes.c_void_p(0)
+        _check(lib().bzg_server_create(0 if mode == "batched" else 1, workers, max_batch_bytes, hbm_budget_bytes,
+                                       ctypes.byref(h)), "bzg_server_create")
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().bzg_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def keep_outputs(self, d_arena):
+        """Keep every output in a torch uint8 device tensor (None: sizes only)."""
+        _check(lib().bzg_server_keep(self._h, _ptr(d_arena) if d_arena is not None else None,
+                                     d_arena.numel() if d_arena is not None else 0), "bzg_server_keep")
+
+    def submit(self, d_buf, level: int) -> int:
+        t = ctypes.c_int64(0)
+        _check(lib().bzg_server_submit(self._h, _ptr(d_buf), d_buf.numel(), level, ctypes.byref(t)),
+               "bzg_server_submit")
+        return int(t.value)
+
+    def wait(self, ticket: int):
+        """(output bytes, arena offset or -1, latency ns)."""
+        b, o, l = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+        _check(lib().bzg_server_wait(self._h, ticket, ctypes.byref(b), ctypes.byref(o), ctypes.byref(l)),
+               "bzg_server_wait")
+        return int(b.value), int(o.value), int(l.value)
+
+    def replay(self, d_pool, offsets, lengths, levels, arrival_ns):
+        """Open-loop replay; returns (done_ns, out_bytes, out_off, batch_size) arrays."""
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        ln = np.ascontiguousarray(lengths, dtype=np.uint64)
+        lv = np.ascontiguousarray(levels, dtype=np.int32)
+        arr = np.ascontiguousarray(arrival_ns, dtype=np.int64)
+        n = off.size
+        done = np.zeros(n, np.int64)
+        ob = np.zeros(n, np.int64)
+        oo = np.zeros(n, np.int64)
+        bs = np.zeros(n, np.int32)
+        c = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        _check(lib().bzg_server_replay(self._h, ctypes.c_void_p(d_pool.data_ptr()), n, c(off), c(ln), c(lv), c(arr),
+                                       c(done), c(ob), c(oo), c(bs)), "bzg_server_replay")
+        return done, ob, oo, bs

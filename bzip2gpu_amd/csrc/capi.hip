@@ -6,10 +6,12 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/bzg.h"
 #include "engine.h"
+#include "server.h"
 
 using namespace bzg;
 
@@ -106,35 +108,6 @@ int ensure_result(Ctx* c, size_t bytes) {
   return 0;
 }
 
-// MSB-first host bit writer (the reference's BitWriter, compress.cu:26-58).
-struct HostBits {
-  std::vector<uint8_t>& out;
-  uint64_t buf = 0;
-  int n = 0;
-  explicit HostBits(std::vector<uint8_t>& o) : out(o) {}
-  void put(uint32_t v, int k) {
-    if (k == 0) return;
-    v &= (uint32_t)((1ull << k) - 1);
-    buf = (buf << k) | v;
-    n += k;
-    while (n >= 8) {
-      n -= 8;
-      out.push_back((uint8_t)(buf >> n));
-    }
-  }
-  void put_words(const std::vector<uint32_t>& w, int64_t bits) {
-    int64_t rem = bits;
-    for (size_t i = 0; i < w.size() && rem > 0; i++) {
-      int take = rem < 32 ? (int)rem : 32;
-      put(w[i] >> (32 - take), take);
-      rem -= take;
-    }
-  }
-  void pad() {
-    if (n > 0) put(0, 8 - n);
-  }
-};
-
 int clamp_level(int level) { return level < 1 ? 1 : (level > 9 ? 9 : level); }
 
 }  // namespace
@@ -197,34 +170,17 @@ namespace {
 // or the context's staging buffer refilled per block from h_in.
 bzg_status plain_run(Ctx* c, const uint8_t* h_in, const uint8_t* d_in, size_t len, int level, int32_t block_bytes,
                      uint8_t* h_out, size_t out_cap, size_t* out_len, hipStream_t s) {
-  level = clamp_level(level);
   std::vector<uint8_t> out;
-  out.reserve(len / 2 + 64);
-  HostBits bw(out);
-  bw.put(0x42, 8); bw.put(0x5A, 8); bw.put(0x68, 8); bw.put((uint32_t)(0x30 + level), 8);
-  uint32_t combined = 0;
   int e;
-  if (h_in && (e = ensure_input(c, (size_t)block_bytes + 16))) return hip_fail(e, "input buffer");
-  BlockResult r;
-  for (size_t off = 0; off < len; off += (size_t)block_bytes) {
-    size_t n = len - off < (size_t)block_bytes ? len - off : (size_t)block_bytes;
-    const uint8_t* src = d_in ? d_in + off : c->d_in;
-    if (h_in) {
-      hipError_t he = hipMemcpyAsync(c->d_in, h_in + off, n, hipMemcpyHostToDevice, s);
-      if (he != hipSuccess) return hip_fail((int)he, "H2D");
-    }
-    std::vector<StreamSpec> ss{StreamSpec{0, n, level, block_bytes}};
-    if ((e = c->eng.plan(ss))) return hip_fail(e, "plan");
-    if ((e = c->eng.setup(src, s))) return hip_fail(e, "setup");
-    if ((e = c->eng.run(s, false, nullptr))) return hip_fail(e, "run");
-    if ((e = c->eng.block_result(0, r, s))) return hip_fail(e, "block result");
-    combined = ((combined << 1) | (combined >> 31)) ^ r.crc;
-    bw.put_words(r.hdr, r.hdr_bits);
-    bw.put_words(r.words, r.bits);
+  if (d_in) {
+    if ((e = compress_plain_device(c->eng, d_in, len, level, block_bytes, out, s))) return hip_fail(e, "plain");
+  } else {
+    // host input: one copy into the context's staging buffer, then the same
+    // per-block device path
+    if ((e = ensure_input(c, len + 16))) return hip_fail(e, "input buffer");
+    if (len && (e = (int)hipMemcpyAsync(c->d_in, h_in, len, hipMemcpyHostToDevice, s))) return hip_fail(e, "H2D");
+    if ((e = compress_plain_device(c->eng, c->d_in, len, level, block_bytes, out, s))) return hip_fail(e, "plain");
   }
-  bw.put(0x17, 8); bw.put(0x72, 8); bw.put(0x45, 8); bw.put(0x38, 8); bw.put(0x50, 8); bw.put(0x90, 8);
-  bw.put(combined, 32);
-  bw.pad();
   *out_len = out.size();
   if (out.size() > out_cap) return fail(BZG_ERR_CAPACITY, "output buffer too small");
   memcpy(h_out, out.data(), out.size());
@@ -350,6 +306,79 @@ bzg_status bzg_set_max_batch_bytes(uint64_t bytes) {
   if (!c) return st;
   std::lock_guard<std::mutex> lk(c->mu);
   c->max_batch_bytes = bytes < (1u << 20) ? (1u << 20) : bytes;
+  return BZG_OK;
+}
+
+struct bzg_server {
+  std::unique_ptr<bzg::Server> s;
+};
+
+bzg_status bzg_server_create(int32_t mode, int32_t workers, uint64_t max_batch_bytes, uint64_t hbm_budget_bytes,
+                             bzg_server** out) {
+  if (!out || (mode != 0 && mode != 1) || workers < 1) return fail(BZG_ERR_INVALID, "bad argument");
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  bzg_server* srv = new bzg_server;
+  srv->s.reset(new bzg::Server(c->device, mode, workers, max_batch_bytes, hbm_budget_bytes));
+  *out = srv;
+  return BZG_OK;
+}
+
+void bzg_server_destroy(bzg_server* srv) { delete srv; }
+
+bzg_status bzg_server_submit(bzg_server* srv, const uint8_t* d_in, uint64_t len, int32_t level, int64_t* h_ticket) {
+  if (!srv || !h_ticket || (len && !d_in)) return fail(BZG_ERR_INVALID, "bad argument");
+  *h_ticket = srv->s->submit(d_in, len, level, -1);
+  return BZG_OK;
+}
+
+bzg_status bzg_server_wait(bzg_server* srv, int64_t ticket, int64_t* h_out_bytes, int64_t* h_out_off,
+                           int64_t* h_latency_ns) {
+  if (!srv) return fail(BZG_ERR_INVALID, "bad argument");
+  bzg::Server::Req r;
+  int e = srv->s->wait(ticket, &r);
+  if (e) return hip_fail(e, "server request");
+  if (h_out_bytes) *h_out_bytes = r.out_bytes;
+  if (h_out_off) *h_out_off = r.out_off;
+  if (h_latency_ns) *h_latency_ns = r.done_ns - r.arrival_ns;
+  return BZG_OK;
+}
+
+bzg_status bzg_server_keep(bzg_server* srv, uint8_t* d_arena, uint64_t arena_bytes) {
+  if (!srv) return fail(BZG_ERR_INVALID, "bad argument");
+  srv->s->set_keep(d_arena != nullptr, d_arena, arena_bytes);
+  return BZG_OK;
+}
+
+bzg_status bzg_server_replay(bzg_server* srv, const uint8_t* d_pool, int32_t n, const uint64_t* in_off,
+                             const uint64_t* lens, const int32_t* levels, const int64_t* arrival_ns,
+                             int64_t* h_done_ns, int64_t* h_out_bytes, int64_t* h_out_off, int32_t* h_batch) {
+  if (!srv || n < 0 || (n && (!d_pool || !in_off || !lens || !levels || !arrival_ns || !h_done_ns)))
+    return fail(BZG_ERR_INVALID, "bad argument");
+  bzg::Server& S = *srv->s;
+  S.reset_clock();
+  std::vector<int64_t> ids((size_t)n);
+  for (int32_t i = 0; i < n; i++) {
+    // open loop: submit at the scheduled arrival time, whatever the backlog
+    int64_t t = arrival_ns[i];
+    for (;;) {
+      int64_t now = S.now_ns();
+      if (now >= t) break;
+      int64_t dt = t - now;
+      if (dt > 200000) std::this_thread::sleep_for(std::chrono::nanoseconds(dt - 100000));
+    }
+    ids[i] = S.submit(d_pool + in_off[i], lens[i], levels[i], t);
+  }
+  for (int32_t i = 0; i < n; i++) {
+    bzg::Server::Req r;
+    int e = S.wait(ids[i], &r);
+    if (e) return hip_fail(e, "server request");
+    h_done_ns[i] = r.done_ns;
+    if (h_out_bytes) h_out_bytes[i] = r.out_bytes;
+    if (h_out_off) h_out_off[i] = r.out_off;
+    if (h_batch) h_batch[i] = r.batch;
+  }
   return BZG_OK;
 }
 

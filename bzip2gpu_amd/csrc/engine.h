@@ -38,6 +38,14 @@ struct StageTimes {
   int64_t bwt_elems = 0;  // rows keyed and sorted, summed over rounds
 };
 
+class Engine;
+
+// Plain path: the blocks of one stream, each through its own pipeline run
+// on `s`, results copied back and serialised on the host (the reference's
+// compress_block + BitWriter structure).  `d_in` is device memory.
+int compress_plain_device(Engine& eng, const uint8_t* d_in, size_t len, int level, int32_t block_bytes,
+                          std::vector<uint8_t>& out, hipStream_t s);
+
 // per-kernel HIP-event probe (one target kernel name at a time)
 void probe_set(const char* kernel);
 int probe_read(float* total_ms, int* launches);

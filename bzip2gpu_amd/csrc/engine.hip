@@ -438,4 +438,62 @@ int Engine::block_result(int b, BlockResult& r, hipStream_t s) {
   return 0;
 }
 
+namespace {
+// MSB-first host bit writer (the reference's BitWriter, compress.cu:26-58).
+struct HostBits {
+  std::vector<uint8_t>& out;
+  uint64_t buf = 0;
+  int n = 0;
+  explicit HostBits(std::vector<uint8_t>& o) : out(o) {}
+  void put(uint32_t v, int k) {
+    if (k == 0) return;
+    v &= (uint32_t)((1ull << k) - 1);
+    buf = (buf << k) | v;
+    n += k;
+    while (n >= 8) {
+      n -= 8;
+      out.push_back((uint8_t)(buf >> n));
+    }
+  }
+  void put_words(const std::vector<uint32_t>& w, int64_t bits) {
+    int64_t rem = bits;
+    for (size_t i = 0; i < w.size() && rem > 0; i++) {
+      int take = rem < 32 ? (int)rem : 32;
+      put(w[i] >> (32 - take), take);
+      rem -= take;
+    }
+  }
+  void pad() {
+    if (n > 0) put(0, 8 - n);
+  }
+};
+}  // namespace
+
+int compress_plain_device(Engine& eng, const uint8_t* d_in, size_t len, int level, int32_t block_bytes,
+                          std::vector<uint8_t>& out, hipStream_t s) {
+  level = std::min(9, std::max(1, level));
+  out.clear();
+  out.reserve(len / 2 + 64);
+  HostBits bw(out);
+  bw.put(0x42, 8); bw.put(0x5A, 8); bw.put(0x68, 8); bw.put((uint32_t)(0x30 + level), 8);
+  uint32_t combined = 0;
+  BlockResult r;
+  int e;
+  for (size_t off = 0; off < len; off += (size_t)block_bytes) {
+    size_t n = len - off < (size_t)block_bytes ? len - off : (size_t)block_bytes;
+    std::vector<StreamSpec> ss{StreamSpec{0, n, level, block_bytes}};
+    if ((e = eng.plan(ss))) return e;
+    if ((e = eng.setup(d_in + off, s))) return e;
+    if ((e = eng.run(s, false, nullptr))) return e;
+    if ((e = eng.block_result(0, r, s))) return e;
+    combined = ((combined << 1) | (combined >> 31)) ^ r.crc;
+    bw.put_words(r.hdr, r.hdr_bits);
+    bw.put_words(r.words, r.bits);
+  }
+  bw.put(0x17, 8); bw.put(0x72, 8); bw.put(0x45, 8); bw.put(0x38, 8); bw.put(0x50, 8); bw.put(0x90, 8);
+  bw.put(combined, 32);
+  bw.pad();
+  return 0;
+}
+
 }  // namespace bzg

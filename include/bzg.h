@@ -110,6 +110,39 @@ int32_t bzg_last_round_stats(int64_t *h_rows, float *h_ms, int32_t max_rounds);
 bzg_status bzg_probe_kernel(const char *kernel_name);
 bzg_status bzg_probe_read(float *h_total_ms, int32_t *h_launches);
 
+/* ---------------------------------------------------------------- serving
+ * A request loop with continuous batching (no counterpart in the reference,
+ * which compresses one input at a time).  Requests are device buffers; a
+ * pool of `workers` pipelines, each on its own HIP stream, takes every
+ * queued request that fits its batch (at most max_batch_bytes raw bytes,
+ * and the device memory of all in-flight batches within hbm_budget_bytes)
+ * and compresses them in one batched pipeline run, so batches of different
+ * workers overlap on the device.  mode 1 serves each request alone on the
+ * plain per-block path instead.  Block size is 100000*level per request.
+ */
+typedef struct bzg_server bzg_server;
+bzg_status bzg_server_create(int32_t mode, int32_t workers, uint64_t max_batch_bytes,
+                             uint64_t hbm_budget_bytes, bzg_server **out);
+void bzg_server_destroy(bzg_server *srv);
+/* Queues one request; *h_ticket identifies it. */
+bzg_status bzg_server_submit(bzg_server *srv, const uint8_t *d_in, uint64_t len, int32_t level,
+                             int64_t *h_ticket);
+/* Waits for a request; its stream is d_arena[*h_out_off .. +*h_out_bytes)
+ * when outputs are kept (bzg_server_keep), else only the size is set. */
+bzg_status bzg_server_wait(bzg_server *srv, int64_t ticket, int64_t *h_out_bytes, int64_t *h_out_off,
+                           int64_t *h_latency_ns);
+/* Keep outputs in the caller's device arena d_arena (arena_bytes). */
+bzg_status bzg_server_keep(bzg_server *srv, uint8_t *d_arena, uint64_t arena_bytes);
+/* Open-loop replay: request i = d_pool[in_off[i] .. +lens[i]) at level
+ * levels[i] is submitted at arrival_ns[i] (nanoseconds from the start,
+ * non-decreasing); returns when all are done.  Fills completion times,
+ * output sizes, arena offsets (-1 when not kept) and the size of the
+ * pipeline run that served each request. */
+bzg_status bzg_server_replay(bzg_server *srv, const uint8_t *d_pool, int32_t n, const uint64_t *in_off,
+                             const uint64_t *lens, const int32_t *levels, const int64_t *arrival_ns,
+                             int64_t *h_done_ns, int64_t *h_out_bytes, int64_t *h_out_off,
+                             int32_t *h_batch);
+
 /* ------------------------------------------------------------- stage level
  * One block at a time, device buffers in and out, like the reference's
  * stage functions (include/compression.h:17-94).  Outputs are written to
