@@ -471,8 +471,16 @@ __global__ void __launch_bounds__(256) k_lscan(Dev d, int cur) {
   for (int t = 0; t < nt; t++) d.hist[(int64_t)(t0 + t) * 256 + dg] += add;
 }
 
+// One LSD pass over a large-group tile: stable rank of every row by digit,
+// rows placed in LDS in tile-sorted order, then written out so that rows of
+// one digit go to consecutive global addresses (coalesced runs).
 __global__ void __launch_bounds__(256) k_lscatter(Dev d, int cur, int shift, int src) {
   __shared__ uint32_t cnt[4][256];
+  __shared__ uint32_t lstart[256];   // tile-local start of each digit
+  __shared__ uint32_t gstart[256];   // global destination of each digit's first row
+  __shared__ uint32_t skey[kTile];
+  __shared__ uint32_t sval[kTile];
+  __shared__ int32_t tmp[8];
   const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const Seg g = d.seg[cur][d.ltile_seg[t]];
@@ -497,22 +505,33 @@ __global__ void __launch_bounds__(256) k_lscatter(Dev d, int cur, int shift, int
   }
   __syncthreads();
   {
-    uint32_t c0 = cnt[0][tid], c1 = cnt[1][tid], c2 = cnt[2][tid];
-    uint32_t gb = d.hist[(int64_t)t * 256 + tid];
-    cnt[0][tid] = gb;
-    cnt[1][tid] = gb + c0;
-    cnt[2][tid] = gb + c0 + c1;
-    cnt[3][tid] = gb + c0 + c1 + c2;
+    uint32_t c0 = cnt[0][tid], c1 = cnt[1][tid], c2 = cnt[2][tid], c3 = cnt[3][tid];
+    int tot;
+    int ls = wg_excl_sum<256, int>((int)(c0 + c1 + c2 + c3), tmp, &tot);
+    lstart[tid] = (uint32_t)ls;
+    gstart[tid] = d.hist[(int64_t)t * 256 + tid];
+    cnt[0][tid] = ls;
+    cnt[1][tid] = ls + c0;
+    cnt[2][tid] = ls + c0 + c1;
+    cnt[3][tid] = ls + c0 + c1 + c2;
   }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < 16; q++) {
     int e = wv * 1024 + q * 64 + lane;
     if (e < n) {
-      uint32_t dst = cnt[wv][(key[q] >> shift) & 255u] + rk[q];
-      Kd[off + dst] = key[q];
-      Sd[off + dst] = val[q];
+      uint32_t p = cnt[wv][(key[q] >> shift) & 255u] + rk[q];
+      skey[p] = key[q];
+      sval[p] = val[q];
     }
+  }
+  __syncthreads();
+  for (int p = tid; p < n; p += 256) {
+    uint32_t k = skey[p];
+    uint32_t dg = (k >> shift) & 255u;
+    uint32_t dst = gstart[dg] + ((uint32_t)p - lstart[dg]);
+    Kd[off + dst] = k;
+    Sd[off + dst] = sval[p];
   }
 }
 
