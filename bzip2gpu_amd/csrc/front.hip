@@ -2,16 +2,19 @@
 //
 // CRC-32/BZIP2 of each raw block (reference: src/compression/crc32.cpp:22-29,
 // called per block at src/compression/compress.cu:259) is computed tile-
-// parallel: every 4096-byte tile yields the CRC register of its bytes
-// started from zero, and tiles are folded with GF(2) shifts
-// (reg(A||B) = reg(A) * x^(8|B|) + reg0(B) mod P).
+// parallel: every thread forms the CRC register of its 16 bytes started from
+// zero and shifts it by the bytes that follow it
+// (reg(A||B) = reg(A) * x^(8|B|) + reg0(B) mod P), so a tile's register is
+// the XOR of its threads' and a block's the XOR of its shifted tiles'.
 //
 // RLE1 (reference: src/compression/rle1.cu:13-113) is restated per byte
-// position: with the run [a, e) that contains position i and j = i - a,
-// position i emits 0, 1 or 2 bytes (see rle1_emit), so the output is a
-// stream compaction in input order: count per tile -> per-block scan ->
-// write.  Run ends that cross tiles are found from per-tile boundary
-// summaries instead of scanning bytes.
+// position: a run [a, e) is cut in pieces of 259; offset r = (i-a) mod 259
+// emits the byte for r < 3, the byte and the piece's count byte for r == 3,
+// nothing beyond.  The number of bytes a position emits depends only on its
+// distance to the run start, so one pass over the raw tiles gives each tile's
+// output count (positions before the tile's first run start are settled by
+// the per-block scan from the carried run start), and the write pass places
+// bytes in input order.
 #include "pipeline.h"
 
 namespace bzg {
@@ -40,6 +43,39 @@ __device__ inline uint32_t xpow8(uint32_t len) {
     k++;
   }
   return r;
+}
+
+// x^(8r) for byte shifts: x^(8q), x^(128m), x^(32768u) with q < 16, m, u < 256
+struct ShiftTables {
+  uint32_t q1[16];
+  uint32_t q16[256];
+  uint32_t tile[256];
+  constexpr ShiftTables() : q1(), q16(), tile() {
+    uint32_t a = 1u;
+    for (int q = 0; q < 16; q++) { q1[q] = a; a = gf2_mulmod(a, 0x100u); }
+    uint32_t b = 1u;
+    for (int m = 0; m < 256; m++) { q16[m] = b; b = gf2_mulmod(b, a); }
+    uint32_t c = 1u;
+    for (int u = 0; u < 256; u++) { tile[u] = c; c = gf2_mulmod(c, b); }
+  }
+};
+static_assert(kTile == 4096, "shift tables assume 4096-byte tiles");
+__constant__ ShiftTables c_shift = ShiftTables();
+
+// reg * x^(8r) mod P
+__device__ inline uint32_t crc_shift(uint32_t reg, uint32_t r) {
+  if (r >= 256u * kTile) return gf2_mulmod(reg, xpow8(r));
+  if (r >> 12) reg = gf2_mulmod(reg, c_shift.tile[r >> 12]);
+  if ((r >> 4) & 255u) reg = gf2_mulmod(reg, c_shift.q16[(r >> 4) & 255u]);
+  if (r & 15u) reg = gf2_mulmod(reg, c_shift.q1[r & 15u]);
+  return reg;
+}
+
+// Bytes emitted by the first j positions of a run: 5 per 259-piece, and
+// min(o, 3) or 5 for the o positions of a started piece.
+__device__ inline int rle1_prefix(int j) {
+  int q = j / 259, o = j - 259 * q;
+  return 5 * q + (o <= 3 ? o : 5);
 }
 
 __global__ void __launch_bounds__(256) k_block_init(Dev d) {
@@ -73,15 +109,16 @@ __device__ inline int load16(const uint8_t* src, int len, int tid, uint8_t x[16]
   return c;
 }
 
-// Tile pass 1: CRC register of the tile (from zero) and the first/last run
-// boundary (block-relative) of the tile.
+// Tile pass 1: CRC register of the tile (from zero), first/last run start
+// (block-relative) and the RLE1 output count of the positions whose run
+// starts inside the tile.
 __global__ void __launch_bounds__(256) k_front1(Dev d) {
   __shared__ uint32_t tab[256];
-  __shared__ uint32_t regs[256];
-  __shared__ uint32_t lens[256];
   __shared__ uint8_t lastb[256];
-  __shared__ int32_t red[8];
-  const int tid = threadIdx.x;
+  __shared__ int32_t xs[256];
+  __shared__ int32_t tmp[8];
+  __shared__ uint32_t red[16];
+  const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int b = d.tn_blk[t];
   const BlockDesc D = d.desc[b];
@@ -91,141 +128,155 @@ __global__ void __launch_bounds__(256) k_front1(Dev d) {
   const uint8_t* src = d.in + D.in_off + start;
   tab[tid] = c_crc.t[tid];
   uint8_t x[16];
-  int c = load16(src, len, tid, x);
+  const int c = load16(src, len, tid, x);
   lastb[tid] = c > 0 ? x[c - 1] : 0;
   __syncthreads();
   uint32_t reg = 0;
   for (int q = 0; q < c; q++) reg = (reg << 8) ^ tab[(reg >> 24) ^ x[q]];
-  regs[tid] = reg;
-  lens[tid] = (uint32_t)c;
-  // run boundaries
+  if (c > 0) reg = crc_shift(reg, (uint32_t)(len - tid * 16 - c));
+  // run starts
+  const int p0 = start + tid * 16;
+  uint32_t bmask = 0;
   int fb = 0x7fffffff, lb = -1;
   if (c > 0) {
-    int p0 = start + tid * 16;
     uint8_t prev = 0;
     bool has_prev = false;
     if (tid > 0) { prev = lastb[tid - 1]; has_prev = true; }
     else if (start > 0) { prev = src[-1]; has_prev = true; }
     for (int q = 0; q < c; q++) {
       bool bnd = (q == 0) ? (!has_prev || x[0] != prev) : (x[q] != x[q - 1]);
-      if (bnd) { int p = p0 + q; fb = min(fb, p); lb = max(lb, p); }
+      if (bnd) { bmask |= 1u << q; lb = p0 + q; if (fb == 0x7fffffff) fb = p0 + q; }
     }
   }
+  xs[tid] = wg_incl_max<256, int>(lb, tmp, -1);
   __syncthreads();
-  // tree-combine CRC registers: left * x^(8*len(right)) ^ right
-  for (int s = 1; s < 256; s <<= 1) {
-    if ((tid & (2 * s - 1)) == 0) {
-      uint32_t rl = lens[tid + s];
-      if (rl) regs[tid] = gf2_mulmod(regs[tid], xpow8(rl)) ^ regs[tid + s];
-      lens[tid] += rl;
+  int run = tid > 0 ? xs[tid - 1] : -1;  // -1: run started before the tile
+  int cnt = 0;
+  for (int q = 0; q < c; q++) {
+    if (bmask & (1u << q)) run = p0 + q;
+    if (run >= 0) {
+      int j = p0 + q - run;
+      int r = j - 259 * (j / 259);
+      cnt += r < 3 ? 1 : (r == 3 ? 2 : 0);
     }
-    __syncthreads();
+  }
+  // reductions: xor of registers, sum of counts, min/max run starts
+  for (int o = 32; o >= 1; o >>= 1) {
+    reg ^= __shfl_xor(reg, o, 64);
+    cnt += __shfl_xor(cnt, o, 64);
   }
   fb = wave_min(fb);
   lb = wave_max(lb);
-  if (lane_id() == 0) { red[tid >> 6] = fb; red[4 + (tid >> 6)] = lb; }
+  if (lane == 0) {
+    red[wv] = reg;
+    red[4 + wv] = (uint32_t)cnt;
+    red[8 + wv] = (uint32_t)fb;
+    red[12 + wv] = (uint32_t)lb;
+  }
   __syncthreads();
   if (tid == 0) {
-    int f = red[0], l = red[4];
-    for (int w = 1; w < 4; w++) { f = min(f, red[w]); l = max(l, red[4 + w]); }
-    d.tn_crc[t] = regs[0];
+    uint32_t r2 = red[0];
+    int c2 = (int)red[4], f = (int)red[8], l = (int)red[12];
+    for (int w = 1; w < 4; w++) {
+      r2 ^= red[w];
+      c2 += (int)red[4 + w];
+      f = min(f, (int)red[8 + w]);
+      l = max(l, (int)red[12 + w]);
+    }
+    d.tn_crc[t] = r2;
+    d.tn_cnt[t] = c2;
     d.tn_fb[t] = f;
     d.tn_lb[t] = l;
   }
 }
 
-// Fold tile CRC registers into the block CRC (one wave per block).
-__global__ void __launch_bounds__(64) k_crc_final(Dev d) {
+// Per-block pass over the tile summaries: block CRC, RLE1 length m, each
+// tile's output offset, and the carried run bounds the write pass needs
+// (tn_lb <- last run start before the tile, tn_fb <- first run start at or
+// after the tile end, or n).
+__global__ void __launch_bounds__(256) k_front_scan(Dev d) {
+  __shared__ int32_t tmp[8];
+  __shared__ int32_t xs[256];
+  __shared__ uint32_t red[4];
+  const int tid = threadIdx.x, lane = lane_id();
   const int b = blockIdx.x;
   const BlockDesc D = d.desc[b];
   const int nt = tiles_for(D.n);
-  const int l = lane_id();
-  const int per = (nt + 63) / 64;
-  int t0 = l * per, t1 = min(nt, t0 + per);
-  uint32_t reg = 0, len = 0;
-  for (int t = t0; t < t1; t++) {
-    uint32_t tl = (uint32_t)min(kTile, D.n - t * kTile);
-    reg = gf2_mulmod(reg, xpow8(tl)) ^ d.tn_crc[D.tile_n0 + t];
-    len += tl;
-  }
-  for (int s = 1; s < 64; s <<= 1) {
-    uint32_t r2 = __shfl_down(reg, s, 64);
-    uint32_t l2 = __shfl_down(len, s, 64);
-    if ((l & (2 * s - 1)) == 0 && l + s < 64) {
-      if (l2) reg = gf2_mulmod(reg, xpow8(l2)) ^ r2;
-      len += l2;
+  int run_lb = -1, run_cnt = 0;
+  uint32_t acc = 0;
+  for (int c0 = 0; c0 < nt; c0 += 256) {
+    const int tk = c0 + tid;
+    const bool v = tk < nt;
+    const int ti = D.tile_n0 + tk;
+    const int lb = v ? d.tn_lb[ti] : -1;
+    const int fb = v ? d.tn_fb[ti] : 0x7fffffff;
+    int cnt = v ? d.tn_cnt[ti] : 0;
+    const uint32_t reg = v ? d.tn_crc[ti] : 0u;
+    xs[tid] = wg_incl_max<256, int>(lb, tmp, run_lb);
+    __syncthreads();
+    const int ca = tid > 0 ? xs[tid - 1] : run_lb;
+    const int next_lb = xs[255];
+    const int start = tk * kTile, end = min(start + kTile, D.n);
+    if (v && ca >= 0) {
+      const int pe = min(fb, end);  // positions [start, pe) continue the run from ca
+      if (pe > start) cnt += rle1_prefix(pe - ca) - rle1_prefix(start - ca);
     }
+    int tot;
+    const int ex = wg_excl_sum<256, int>(cnt, tmp, &tot);
+    if (v) {
+      d.tn_off[ti] = run_cnt + ex;
+      d.tn_lb[ti] = ca;
+      acc ^= crc_shift(reg, (uint32_t)(D.n - end));
+    }
+    run_cnt += tot;
+    run_lb = next_lb;
+    __syncthreads();
   }
-  if (l == 0) {
-    uint32_t full = gf2_mulmod(0xFFFFFFFFu, xpow8((uint32_t)D.n)) ^ reg;
-    d.st[b].crc = ~full;
+  int run_fb = D.n;
+  for (int c0 = ((nt - 1) / 256) * 256; c0 >= 0 && nt > 0; c0 -= 256) {
+    const int tk = c0 + tid;
+    const bool v = tk < nt;
+    const int ti = D.tile_n0 + tk;
+    const int fb = v ? d.tn_fb[ti] : 0x7fffffff;
+    xs[tid] = wg_rev_incl_min<256, int>(fb, tmp, run_fb);
+    __syncthreads();
+    const int ce = tid < 255 ? xs[tid + 1] : run_fb;
+    const int next_fb = xs[0];
+    if (v) d.tn_fb[ti] = ce;
+    run_fb = next_fb;
+    __syncthreads();
+  }
+  for (int o = 32; o >= 1; o >>= 1) acc ^= __shfl_xor(acc, o, 64);
+  if (lane == 0) red[tid >> 6] = acc;
+  __syncthreads();
+  if (tid == 0) {
+    const uint32_t reg = red[0] ^ red[1] ^ red[2] ^ red[3];
+    d.st[b].m = run_cnt;
+    d.st[b].crc = ~(gf2_mulmod(0xFFFFFFFFu, xpow8((uint32_t)D.n)) ^ reg);
   }
 }
 
-// Bytes emitted by raw position i of a run [a, e) of byte x
-// (reference semantics: src/compression/rle1.cu:13-58).
-__device__ inline int rle1_emit(int i, int a, int e, uint8_t x, uint8_t& v0, uint8_t& v1) {
-  int c = e - a, j = i - a;
-  int q = j / 259, r = j - 259 * q;
-  int full = c / 259, rem = c - 259 * full;
-  v0 = x;
-  v1 = 0;
-  if (q < full) {
-    if (r < 4) return 1;
-    if (r == 4) { v0 = 255; return 1; }
-    return 0;
-  }
-  if (rem < 4) return r < rem ? 1 : 0;
-  if (r < 3) return 1;
-  if (r == 3) return rem == 4 ? 2 : 1;
-  if (r == 4) { v0 = (uint8_t)(rem - 4); return 1; }
-  return 0;
-}
-
-// Shared body of the RLE1 count and write passes.
-template <bool WRITE>
-__device__ inline void rle1_tile(const Dev& d) {
+// RLE1 write pass: every position's bytes are staged in LDS in input order,
+// then copied to the block's RLE1 text with word stores (byte stores at the
+// two edge words shared with the neighbouring tiles).
+__global__ void __launch_bounds__(256) k_rle1_write(Dev d) {
   __shared__ uint8_t lastb[256];
   __shared__ int32_t tmp[8];
-  __shared__ int32_t carry[2];
-  __shared__ uint32_t pres[8];
+  __shared__ int32_t xa[256], xe[256];
+  __shared__ uint8_t seen[256];
+  __shared__ uint8_t ob[2 * kTile];
   const int tid = threadIdx.x;
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int b = d.tn_blk[t];
   const BlockDesc D = d.desc[b];
   const int k = t - D.tile_n0;
-  const int nt = tiles_for(D.n);
   const int start = k * kTile;
   const int len = min(kTile, D.n - start);
   const uint8_t* src = d.in + D.in_off + start;
-  if (WRITE && tid < 8) pres[tid] = 0;
-  // carries: last boundary before the tile (prefix max of tile lb),
-  // first boundary at/after the tile end (suffix min of tile fb)
-  {
-    int lbv = -1;
-    for (int j = k - 1 - tid; j >= 0; j -= 256) {
-      int v = d.tn_lb[D.tile_n0 + j];
-      if (v >= 0) { lbv = v; break; }
-    }
-    int fbv = 0x7fffffff;
-    for (int j = k + 1 + tid; j < nt; j += 256) {
-      int v = d.tn_fb[D.tile_n0 + j];
-      if (v != 0x7fffffff) { fbv = v; break; }
-    }
-    lbv = wave_max(lbv);
-    fbv = wave_min(fbv);
-    if (lane_id() == 0) { tmp[tid >> 6] = lbv; tmp[4 + (tid >> 6)] = fbv; }
-    __syncthreads();
-    if (tid == 0) {
-      int a = tmp[0], f = tmp[4];
-      for (int w = 1; w < 4; w++) { a = max(a, tmp[w]); f = min(f, tmp[4 + w]); }
-      carry[0] = a;
-      carry[1] = f == 0x7fffffff ? D.n : f;
-    }
-    __syncthreads();
-  }
+  const int ca = d.tn_lb[t], ce = d.tn_fb[t];
+  seen[tid] = 0;
   uint8_t x[16];
-  int c = load16(src, len, tid, x);
+  const int c = load16(src, len, tid, x);
   lastb[tid] = c > 0 ? x[c - 1] : 0;
   __syncthreads();
   const int p0 = start + tid * 16;
@@ -243,93 +294,99 @@ __device__ inline void rle1_tile(const Dev& d) {
       if (firstB == 0x7fffffff) firstB = p0 + q;
     }
   }
-  // exclusive prefix max of lastB over threads, exclusive suffix min of firstB
-  int incl_a = wg_incl_max<256, int>(lastB, tmp, carry[0]);
-  int incl_e = wg_rev_incl_min<256, int>(firstB, tmp, carry[1]);
-  __shared__ int32_t xa[256], xe[256];
-  xa[tid] = incl_a;
-  xe[tid] = incl_e;
+  xa[tid] = wg_incl_max<256, int>(lastB, tmp, ca);
+  xe[tid] = wg_rev_incl_min<256, int>(firstB, tmp, ce);
   __syncthreads();
-  int ap = tid > 0 ? xa[tid - 1] : carry[0];
-  int es = tid < 255 ? xe[tid + 1] : carry[1];
-  // per-position runs
-  int a_of[16], e_of[16];
-  {
-    int run = ap;
-    for (int q = 0; q < 16; q++) {
-      if (bmask & (1u << q)) run = p0 + q;
-      a_of[q] = run;
-    }
-    int nb = es;
-    for (int q = 15; q >= 0; q--) {
-      e_of[q] = nb;
-      if (bmask & (1u << q)) nb = p0 + q;
-    }
-  }
+  int a = tid > 0 ? xa[tid - 1] : ca;
   int cnt = 0;
-  uint8_t v0s[16], v1s[16], cs[16];
+  int8_t rr[16];
+#pragma unroll
   for (int q = 0; q < 16; q++) {
-    uint8_t v0 = 0, v1 = 0;
-    int cc = q < c ? rle1_emit(p0 + q, a_of[q], e_of[q], x[q], v0, v1) : 0;
-    cs[q] = (uint8_t)cc;
-    v0s[q] = v0;
-    v1s[q] = v1;
-    cnt += cc;
+    if (bmask & (1u << q)) a = p0 + q;
+    int r = 4;
+    if (q < c) {
+      int j = p0 + q - a;
+      r = j - 259 * (j / 259);
+    }
+    rr[q] = (int8_t)(r < 4 ? r : 4);
+    cnt += r < 3 ? 1 : (r == 3 ? 2 : 0);
   }
   int total;
-  int excl = wg_excl_sum<256, int>(cnt, tmp, &total);
-  if (!WRITE) {
-    if (tid == 0) d.tn_cnt[t] = total;
-    return;
-  }
-  uint8_t* dst = d.T + D.off + d.tn_off[t] + excl;
-  int o = 0;
-  for (int q = 0; q < 16; q++) {
-    if (cs[q] >= 1) {
-      dst[o++] = v0s[q];
-      atomicOr(&pres[v0s[q] >> 5], 1u << (v0s[q] & 31));
+  int o = wg_excl_sum<256, int>(cnt, tmp, &total);
+  {
+    // run end of every position: next run start after it
+    int e = tid < 255 ? xe[tid + 1] : ce;
+    uint32_t ends[16];
+#pragma unroll
+    for (int q = 15; q >= 0; q--) {
+      ends[q] = (uint32_t)e;
+      if (bmask & (1u << q)) e = p0 + q;
     }
-    if (cs[q] == 2) {
-      dst[o++] = v1s[q];
-      atomicOr(&pres[v1s[q] >> 5], 1u << (v1s[q] & 31));
+    a = tid > 0 ? xa[tid - 1] : ca;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      if (bmask & (1u << q)) a = p0 + q;
+      if (rr[q] < 4) {
+        ob[o++] = x[q];
+        seen[x[q]] = 1;
+        if (rr[q] == 3) {
+          int runlen = (int)ends[q] - a, j = p0 + q - a;
+          uint8_t cb = (j / 259 < runlen / 259) ? 255 : (uint8_t)(runlen - 259 * (runlen / 259) - 4);
+          ob[o++] = cb;
+          seen[cb] = 1;
+        }
+      }
     }
   }
   __syncthreads();
-  if (tid < 8 && pres[tid]) atomicOr(&d.st[b].present[tid], pres[tid]);
+  uint8_t* dst = d.T + D.off + d.tn_off[t];
+  const uintptr_t A0 = reinterpret_cast<uintptr_t>(dst) & ~(uintptr_t)3;
+  const int lead = (int)(reinterpret_cast<uintptr_t>(dst) - A0);
+  const int nw = (lead + total + 3) >> 2;
+  for (int w = tid; w < nw; w += 256) {
+    const int oo = 4 * w - lead;
+    if (oo >= 0 && oo + 4 <= total) {
+      uint32_t v = (uint32_t)ob[oo] | ((uint32_t)ob[oo + 1] << 8) | ((uint32_t)ob[oo + 2] << 16) |
+                   ((uint32_t)ob[oo + 3] << 24);
+      *reinterpret_cast<uint32_t*>(A0 + 4 * (uintptr_t)w) = v;
+    } else {
+      for (int q = 0; q < 4; q++) {
+        const int oq = oo + q;
+        if (oq >= 0 && oq < total) dst[oq] = ob[oq];
+      }
+    }
+  }
+  const uint64_t m = __ballot(seen[tid] != 0);
+  if (lane_id() == 0) {
+    const int w2 = (tid >> 6) * 2;
+    if ((uint32_t)m) atomicOr(&d.st[b].present[w2], (uint32_t)m);
+    if ((uint32_t)(m >> 32)) atomicOr(&d.st[b].present[w2 + 1], (uint32_t)(m >> 32));
+  }
 }
 
-__global__ void __launch_bounds__(256) k_rle1_count(Dev d) { rle1_tile<false>(d); }
-__global__ void __launch_bounds__(256) k_rle1_write(Dev d) { rle1_tile<true>(d); }
-
-// Per-block exclusive scan of tile counts.  which = 0: raw tiles -> st.m;
-// which = 1: capacity tiles -> st.s (adds the end-of-block symbol).
-__global__ void __launch_bounds__(256) k_tile_scan(Dev d, int which) {
+// Per-block exclusive scan of the RLE2 counts of the capacity tiles -> st.s
+// (adds the end-of-block symbol).
+__global__ void __launch_bounds__(256) k_tile_scan_c(Dev d) {
   __shared__ int32_t tmp[8];
   const int b = blockIdx.x;
   const BlockDesc D = d.desc[b];
-  const int t0 = which == 0 ? D.tile_n0 : D.tile_c0;
-  const int nt = which == 0 ? tiles_for(D.n) : tiles_for(D.cap);
-  const int32_t* cnt = which == 0 ? d.tn_cnt : d.tc_cnt;
-  int32_t* off = which == 0 ? d.tn_off : d.tc_off;
+  const int t0 = D.tile_c0;
+  const int nt = tiles_for(D.cap);
   int run = 0;
   for (int base = 0; base < nt; base += 256) {
     int j = base + threadIdx.x;
-    int v = j < nt ? cnt[t0 + j] : 0;
+    int v = j < nt ? d.tc_cnt[t0 + j] : 0;
     int tot;
     int ex = wg_excl_sum<256, int>(v, tmp, &tot);
-    if (j < nt) off[t0 + j] = run + ex;
+    if (j < nt) d.tc_off[t0 + j] = run + ex;
     run += tot;
   }
   if (threadIdx.x == 0) {
-    if (which == 0) {
-      d.st[b].m = run;
-    } else {
-      // end-of-block symbol = n_in_use + 1 (src/compression/rle2.cu:71-74)
-      int eob = d.st[b].n_in_use + 1;
-      d.SYM[D.off + run] = (uint16_t)eob;
-      atomicAdd(&d.freq[(int64_t)b * 260 + eob], 1);
-      d.st[b].s = run + 1;
-    }
+    // end-of-block symbol = n_in_use + 1 (src/compression/rle2.cu:71-74)
+    int eob = d.st[b].n_in_use + 1;
+    d.SYM[D.off + run] = (uint16_t)eob;
+    atomicAdd(&d.freq[(int64_t)b * 260 + eob], 1);
+    d.st[b].s = run + 1;
   }
 }
 
@@ -338,15 +395,13 @@ void launch_block_init(const Dev& d, hipStream_t s) {
 }
 
 void launch_crc_rle1(const Dev& d, hipStream_t s) {
-  k_front1<<<d.ntile_n, 256, 0, s>>>(d);
-  k_crc_final<<<d.nblk, 64, 0, s>>>(d);
-  k_rle1_count<<<d.ntile_n, 256, 0, s>>>(d);
-  k_tile_scan<<<d.nblk, 256, 0, s>>>(d, 0);
-  k_rle1_write<<<d.ntile_n, 256, 0, s>>>(d);
+  if (d.ntile_n > 0) k_front1<<<d.ntile_n, 256, 0, s>>>(d);
+  k_front_scan<<<d.nblk, 256, 0, s>>>(d);
+  if (d.ntile_n > 0) k_rle1_write<<<d.ntile_n, 256, 0, s>>>(d);
 }
 
 void launch_tile_scan_c(const Dev& d, hipStream_t s) {
-  k_tile_scan<<<d.nblk, 256, 0, s>>>(d, 1);
+  k_tile_scan_c<<<d.nblk, 256, 0, s>>>(d);
 }
 
 }  // namespace bzg

@@ -139,100 +139,161 @@ __global__ void __launch_bounds__(256) k_huff_select(Dev d) {
   }
 }
 
-// libbz2 code-length method, one thread per table (huffman.cu:51-124):
-// binary min-heap on weight = freq<<8 | depth with the same sift rules, so
-// ties resolve exactly as in the reference.  Heap entries carry their
-// weight next to the node id (one LDS load per comparison), and depths are
-// assigned top-down over the node list (parents have larger ids than their
-// children) instead of walking each leaf's parent chain.
+// libbz2 code-length method (huffman.cu:51-124): binary min-heap on
+// weight = freq<<8 | depth with the same sift rules, so ties resolve exactly
+// as in the reference.  One wave per (block, table); the heap lives in LDS
+// and every heap operation is resolved a window of levels at a time: the
+// lanes load the window, and sibling minima, path membership and the
+// "tmp < entry" stop test become ballots, so one LDS round trip covers five
+// levels of a sift.  Depths come from pointer jumping over the parent array.
+namespace {
+
+// Sift-down of `tmp` from the root (DOWNHEAP(1)); heap holds nHeap entries.
+// Lane l < 62 covers window level k = log2(l+2), index j = l+2-2^k below zz.
+// An entry lies on the min-child path when it and all its window ancestors
+// are the smaller of their sibling pair; the path stops above the first path
+// entry that tmp is strictly lighter than.
+__device__ inline void heap_down(int2* heap, int nHeap, int2 tmp, int lane) {
+  const int k = 31 - __clz(lane + 2);
+  const int j = lane + 2 - (1 << k);
+  const bool act = lane < 62;
+  int zz = 1;
+  while (true) {
+    const int p = (zz << k) + j;
+    const int pl = p & ~1;  // left sibling of the pair
+    const bool exl = act && pl <= nHeap, exr = act && pl + 1 <= nHeap;
+    const int4 pr = exl ? *reinterpret_cast<const int4*>(heap + pl)
+                        : make_int4(0x7fffffff, 0, 0x7fffffff, 0);
+    const bool isr = j & 1;
+    const int2 e = isr ? make_int2(pr.z, pr.w) : make_int2(pr.x, pr.y);
+    const bool ex = isr ? exr : exl;
+    // the left child is taken unless the right one exists and is strictly lighter
+    const bool rlight = exr && pr.z < pr.x;
+    const bool win = isr ? rlight : (exl && !rlight);
+    const uint64_t W = __ballot(win);
+    bool on = win;
+#pragma unroll
+    for (int kp = 1; kp < 5; kp++) {
+      if (kp < k) on = on && ((W >> ((1 << kp) - 2 + (j >> (k - kp)))) & 1ull);
+    }
+    const uint64_t P = __ballot(on);
+    const uint64_t S = P & __ballot(ex && tmp.x < e.x);
+    uint64_t moved;
+    bool done;
+    if (S) {
+      moved = P & ((S & (~S + 1)) - 1);  // path entries above the first stop
+      done = true;
+    } else {
+      moved = P;
+      done = (P >> 30) == 0;  // path ended inside the window
+    }
+    int cur = zz;
+    if (moved) {
+      const int l = 63 - __builtin_clzll(moved);
+      const int kl = 31 - __clz(l + 2);
+      cur = (zz << kl) + (l + 2 - (1 << kl));
+    }
+    // the next window reads only below cur, which these writes never touch
+    if ((moved >> lane) & 1ull) heap[p >> 1] = e;
+    if (done) {
+      if (lane == 0) heap[cur] = tmp;
+      __syncthreads();
+      return;
+    }
+    zz = cur;
+  }
+}
+
+// Sift-up of `tmp` placed at position zz (UPHEAP(zz)): lane q holds the
+// ancestor zz >> (q+1); every ancestor heavier than tmp moves one level down.
+__device__ inline void heap_up(int2* heap, int zz, int2 tmp, int lane) {
+  const int a = lane < 9 ? (zz >> (lane + 1)) : 0;
+  const bool ex = a >= 1;
+  const int2 e = ex ? heap[a] : make_int2(0, 0);
+  const uint64_t M = __ballot(ex && tmp.x < e.x);
+  const int c = __builtin_ctzll(~M);
+  if (lane < c) heap[zz >> lane] = e;
+  if (lane == 0) heap[zz >> c] = tmp;
+  __syncthreads();
+}
+
+}  // namespace
+
 __global__ void __launch_bounds__(64) k_huff_lengths(Dev d) {
-  __shared__ int2 heap_s[6][260];
-  __shared__ int32_t parent_s[6][520];
-  __shared__ int32_t leafw_s[6][260];
-  const int b = blockIdx.x;
-  const int tt = threadIdx.x;
-  const int ng = d.st[b].n_groups;
-  if (tt >= ng) return;
+  __shared__ __align__(16) int2 heap[264];
+  __shared__ int32_t anc[2][520];
+  __shared__ int32_t dep[2][520];
+  __shared__ int32_t leafw[260];
+  const int b = blockIdx.x / 6, tt = blockIdx.x - 6 * (blockIdx.x / 6);
+  const int lane = lane_id();
+  if (tt >= d.st[b].n_groups) return;
   const int alpha = d.st[b].n_in_use + 2;
-  int2* heap = heap_s[tt];
-  int32_t* parent = parent_s[tt];  // reused as depth in the length pass
-  int32_t* leafw = leafw_s[tt];
   int32_t* fr = d.rfreq + ((int64_t)b * 6 + tt) * 260;
   uint8_t* len = d.len + ((int64_t)b * 6 + tt) * 260;
-  for (int i = 0; i < alpha; i++) {
+  for (int i = lane; i < alpha; i += 64) {
     int f = fr[i];
     leafw[i + 1] = (f == 0 ? 1 : f) << 8;
     fr[i] = 0;  // ready for the next iteration
   }
+  __syncthreads();
   while (true) {
     int nNodes = alpha, nHeap = 0;
-    heap[0] = make_int2(0, 0);
     for (int i = 1; i <= alpha; i++) {
-      parent[i] = -1;
       nHeap++;
-      int2 tmp = make_int2(leafw[i], i);
-      int zz = nHeap;
-      while (true) {
-        int2 up = heap[zz >> 1];
-        if (!(tmp.x < up.x)) break;
-        heap[zz] = up;
-        zz >>= 1;
-      }
-      heap[zz] = tmp;
+      heap_up(heap, nHeap, make_int2(leafw[i], i), lane);
     }
     while (nHeap > 1) {
-      int2 pick[2];
-      for (int q = 0; q < 2; q++) {
-        pick[q] = heap[1];
-        int2 tmp = heap[nHeap];
-        nHeap--;
-        int zz = 1;
-        while (true) {
-          int yy = zz << 1;
-          if (yy > nHeap) break;
-          int2 c = heap[yy];
-          if (yy < nHeap) {
-            int2 c2 = heap[yy + 1];
-            if (c2.x < c.x) { yy++; c = c2; }
-          }
-          if (tmp.x < c.x) break;
-          heap[zz] = c;
-          zz = yy;
-        }
-        heap[zz] = tmp;
-      }
+      const int2 n1 = heap[1];
+      const int2 t1 = heap[nHeap];
+      __syncthreads();
+      nHeap--;
+      heap_down(heap, nHeap, t1, lane);
+      const int2 n2 = heap[1];
+      const int2 t2 = heap[nHeap];
+      __syncthreads();
+      nHeap--;
+      heap_down(heap, nHeap, t2, lane);
       nNodes++;
-      parent[pick[0].y] = nNodes;
-      parent[pick[1].y] = nNodes;
-      int w1 = pick[0].x, w2 = pick[1].x;
-      int d1 = w1 & 0xff, d2 = w2 & 0xff;
-      int2 tmp = make_int2(((w1 & 0xffffff00) + (w2 & 0xffffff00)) | (1 + (d1 > d2 ? d1 : d2)), nNodes);
-      parent[nNodes] = -1;
-      nHeap++;
-      int zz = nHeap;
-      while (true) {
-        int2 up = heap[zz >> 1];
-        if (!(tmp.x < up.x)) break;
-        heap[zz] = up;
-        zz >>= 1;
+      if (lane == 0) {
+        anc[0][n1.y] = nNodes;
+        anc[0][n2.y] = nNodes;
       }
-      heap[zz] = tmp;
+      const int w1 = n1.x, w2 = n2.x;
+      const int d1 = w1 & 0xff, d2 = w2 & 0xff;
+      const int2 nn = make_int2(((w1 & 0xffffff00) + (w2 & 0xffffff00)) | (1 + (d1 > d2 ? d1 : d2)), nNodes);
+      nHeap++;
+      heap_up(heap, nHeap, nn, lane);
     }
-    // depth of every node, root (nNodes) first
+    // depth of every node by pointer jumping (root: ancestor itself, depth 0)
+    for (int v = lane + 1; v <= nNodes; v += 64) {
+      const bool root = v == nNodes;
+      if (root) anc[0][v] = v;
+      dep[0][v] = root ? 0 : 1;
+    }
+    __syncthreads();
+    int src = 0;
+    for (int it = 0; it < 9; it++) {
+      for (int v = lane + 1; v <= nNodes; v += 64) {
+        const int a = anc[src][v];
+        dep[src ^ 1][v] = dep[src][v] + dep[src][a];
+        anc[src ^ 1][v] = anc[src][a];
+      }
+      __syncthreads();
+      src ^= 1;
+    }
     bool tooLong = false;
-    parent[nNodes] = 0;
-    for (int v = nNodes - 1; v >= 1; v--) parent[v] = parent[parent[v]] + 1;
-    for (int i = 1; i <= alpha; i++) {
-      int j = parent[i];
+    for (int i = lane + 1; i <= alpha; i += 64) {
+      const int j = dep[src][i];
       len[i - 1] = (uint8_t)j;
       if (j > 17) tooLong = true;
     }
-    if (!tooLong) break;
-    for (int i = 1; i <= alpha; i++) {
+    if (!__ballot(tooLong)) break;
+    for (int i = lane + 1; i <= alpha; i += 64) {
       int j = leafw[i] >> 8;
       j = 1 + (j / 2);
       leafw[i] = j << 8;
     }
+    __syncthreads();
   }
 }
 
@@ -573,7 +634,7 @@ void launch_huffman(const Dev& d, hipStream_t s) {
     k_huff_select<<<d.ntile_c, 256, 0, s>>>(d);
     probe_end("k_huff_select", s);
     probe_begin("k_huff_lengths", s);
-    k_huff_lengths<<<d.nblk, 64, 0, s>>>(d);
+    k_huff_lengths<<<d.nblk * 6, 64, 0, s>>>(d);
     probe_end("k_huff_lengths", s);
   }
   k_huff_codes<<<d.nblk, 64, 0, s>>>(d);
