@@ -85,17 +85,20 @@ __device__ inline uint64_t bit_transpose64(uint64_t x) {
   return x;
 }
 
-// One wave per segment: the MTF proper.  The list is kept as sym2pos[]
-// (position of every present byte).  After each 64-symbol step the K
-// distinct bytes of the step take positions 0..K-1 by last occurrence, and
-// every other present byte moves from position p to
-// K + p - #(step bytes whose old position is below p).
+// One wave per segment: the MTF proper.  The segment is staged in LDS and
+// the list is kept as s2p[] (list position of every present byte, one byte
+// each).  After each 64-symbol step the K distinct bytes of the step take
+// positions 0..K-1 by last occurrence, and every other present byte moves
+// from position p to K + p - rank[p], rank[p] = #(step bytes whose old
+// position is below p).
 __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
   __shared__ int32_t E[256];
-  __shared__ int32_t sym2pos[256];
-  __shared__ int32_t newpos[256];
-  __shared__ uint32_t cmask[8];   // old positions of this step's bytes
-  __shared__ uint32_t smask[8];   // this step's bytes
+  __shared__ __align__(16) uint8_t buf[kTile];
+  __shared__ __align__(4) uint8_t s2p[256];
+  __shared__ __align__(4) uint8_t pflag[256];  // old positions of this step's bytes
+  __shared__ __align__(4) uint8_t sflag[256];  // this step's bytes
+  __shared__ __align__(4) uint8_t rank[256];
+  __shared__ uint8_t newpos[256];
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int b = d.tc_blk[t];
   const BlockDesc D = d.desc[b];
@@ -104,27 +107,42 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
   const int start = k * kTile;
   const int lane = lane_id();
   if (start >= m) return;
-  for (int q = lane; q < 256; q += 64) E[q] = d.lo[(int64_t)t * 256 + q];
-  if (lane < 8) { cmask[lane] = 0; smask[lane] = 0; }
-  __syncthreads();
-  // list position of every present byte: # present bytes more recent
-  for (int q = lane; q < 256; q += 64) {
-    int e = E[q];
-    int p = -1;  // absent from the block
-    if (e != (int)0x80000000) {
-      p = 0;
-      for (int r = 0; r < 256; r++) p += (E[r] > e) ? 1 : 0;
+  const int n = min(kTile, m - start);
+  uint8_t* mtf = d.MTF + D.off + start;  // 256-byte aligned segment
+  for (int o = lane * 16; o < n; o += 1024) {
+    if (o + 16 <= n) {
+      *reinterpret_cast<uint4*>(buf + o) = *reinterpret_cast<const uint4*>(mtf + o);
+    } else {
+      for (int q = o; q < n; q++) buf[q] = mtf[q];
     }
-    sym2pos[q] = p;
+  }
+  for (int q = lane; q < 256; q += 64) {
+    E[q] = d.lo[(int64_t)t * 256 + q];
+    pflag[q] = 0;
+    sflag[q] = 0;
   }
   __syncthreads();
-  const int n = min(kTile, m - start);
-  uint8_t* mtf = d.MTF + D.off + start;
+  // list position of every present byte: # present bytes more recent
+  {
+    int e4[4], p4[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < 4; r++) e4[r] = E[4 * lane + r];
+    for (int r = 0; r < 256; r++) {
+      const int er = E[r];
+#pragma unroll
+      for (int q = 0; q < 4; q++) p4[q] += er > e4[q] ? 1 : 0;
+    }
+    reinterpret_cast<uint32_t*>(s2p)[lane] =
+        (uint32_t)(p4[0] & 255) | ((uint32_t)(p4[1] & 255) << 8) | ((uint32_t)(p4[2] & 255) << 16) |
+        ((uint32_t)(p4[3] & 255) << 24);
+  }
+  __syncthreads();
   int fnz = 0x7fffffff;
+  const uint64_t below = lanemask_lt();
   for (int base = 0; base < n; base += 64) {
     const int i = base + lane;
     const bool valid = i < n;
-    const uint32_t c = valid ? mtf[i] : 0u;
+    const uint32_t c = valid ? buf[i] : 0u;
     uint64_t peers = ballot(valid);
 #pragma unroll
     for (int bb = 0; bb < 8; bb++) {
@@ -132,13 +150,12 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
       uint64_t mm = ballot(bit);
       peers &= bit ? mm : ~mm;
     }
-    const uint64_t below = lanemask_lt();
     const uint64_t pb = peers & below;
     const bool has_prev = valid && pb != 0;
     const int prev = has_prev ? 63 - __clzll(pb) : -1;
     const uint64_t above = valid ? (peers & ~below & ~(1ull << lane)) : 0ull;
     const int nxt = above ? __builtin_ctzll(above) : 64;
-    const int mypos = valid ? sym2pos[c] : 0;
+    const int mypos = valid ? (int)s2p[c] : 0;
     const bool first = valid && !has_prev;
     // Repeated byte: distinct bytes strictly between its previous lane and
     // this lane = lanes j in (prev, lane) whose next occurrence is >= lane.
@@ -150,64 +167,72 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
       row = upto & ~((2ull << lane) - 1ull);
       if (lane == 63) row = 0;
     }
-    uint64_t col = bit_transpose64(row);
-    int cnt = 0;
-    if (has_prev) {
-      uint64_t between = lanemask_lt() & ~((2ull << prev) - 1ull);
-      cnt = __popcll(col & between);
-    }
-    // First occurrence: its old position, plus the first occurrences in
-    // earlier lanes whose old position was behind it.
-    uint64_t fm = ballot(first);
+    const uint64_t col = bit_transpose64(row);
+    // First occurrence: earlier first occurrences whose old position is
+    // behind this one, by a bit-serial compare of the positions (MSB first).
+    const uint64_t fm = ballot(first);
     const int K = __popcll(fm);
-    {
-      uint64_t f = fm;
-      while (f) {
-        int j = __builtin_ctzll(f);
-        f &= f - 1;
-        int fj = __builtin_amdgcn_readlane(mypos, j);
-        if (first && j < lane && fj > mypos) cnt++;
-      }
+    uint64_t gt = 0, eq = fm;
+#pragma unroll
+    for (int bb = 7; bb >= 0; bb--) {
+      const bool mb = (mypos >> bb) & 1;
+      const uint64_t B = ballot(first && mb);
+      gt = mb ? gt : (gt | (eq & B));
+      eq = mb ? (eq & B) : (eq & ~B);
     }
-    const int out = has_prev ? cnt : mypos + cnt;
-    if (valid) mtf[i] = (uint8_t)out;
-    uint64_t nz = ballot(valid && out != 0);
+    int out;
+    if (has_prev) {
+      const uint64_t between = below & ~((2ull << prev) - 1ull);
+      out = __popcll(col & between);
+    } else {
+      out = mypos + __popcll(gt & below);
+    }
+    if (valid) buf[i] = (uint8_t)out;
+    const uint64_t nz = ballot(valid && out != 0);
     if (nz && fnz == 0x7fffffff) fnz = start + base + __builtin_ctzll(nz);
     // list update
     const bool last = valid && nxt == 64;
     const uint64_t lastm = ballot(last);
     if (first) {
-      atomicOr(&cmask[mypos >> 5], 1u << (mypos & 31));
-      atomicOr(&smask[c >> 5], 1u << (c & 31));
+      pflag[mypos] = 1;
+      sflag[c] = 1;
     }
-    if (last) newpos[c] = __popcll(lastm >> lane) - 1;
+    if (last) newpos[c] = (uint8_t)(__popcll(lastm >> lane) - 1);
     __syncthreads();
-    uint32_t cm[8];
+    {
+      const uint32_t f4 = reinterpret_cast<const uint32_t*>(pflag)[lane];
+      const int c4 = (int)(f4 & 1u) + (int)((f4 >> 8) & 1u) + (int)((f4 >> 16) & 1u) + (int)(f4 >> 24);
+      const int ex = wave_incl_sum(c4) - c4;
+      const int r1 = ex + (int)(f4 & 1u), r2 = r1 + (int)((f4 >> 8) & 1u), r3 = r2 + (int)((f4 >> 16) & 1u);
+      reinterpret_cast<uint32_t*>(rank)[lane] =
+          (uint32_t)ex | ((uint32_t)r1 << 8) | ((uint32_t)r2 << 16) | ((uint32_t)r3 << 24);
+    }
+    __syncthreads();
+    {
+      const uint32_t sp4 = reinterpret_cast<const uint32_t*>(s2p)[lane];
+      const uint32_t sf4 = reinterpret_cast<const uint32_t*>(sflag)[lane];
+      uint32_t np4 = 0;
 #pragma unroll
-    for (int w = 0; w < 8; w++) cm[w] = cmask[w];
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int sy = lane * 4 + r;
-      const int op = sym2pos[sy];
-      if (op < 0) continue;
-      int np;
-      if ((smask[sy >> 5] >> (sy & 31)) & 1u) {
-        np = newpos[sy];
-      } else {
-        int lower = 0;
-#pragma unroll
-        for (int w = 0; w < 8; w++) {
-          int lo_bit = w * 32;
-          if (lo_bit + 32 <= op) lower += __popc(cm[w]);
-          else if (lo_bit < op) lower += __popc(cm[w] & ((1u << (op - lo_bit)) - 1u));
-        }
-        np = K + op - lower;
+      for (int r = 0; r < 4; r++) {
+        const int op = (int)((sp4 >> (8 * r)) & 255u);
+        const int np = ((sf4 >> (8 * r)) & 1u) ? (int)newpos[4 * lane + r] : K + op - (int)rank[op];
+        np4 |= (uint32_t)(np & 255) << (8 * r);
       }
-      sym2pos[sy] = np;
+      __syncthreads();
+      reinterpret_cast<uint32_t*>(s2p)[lane] = np4;
+      if (first) {
+        pflag[mypos] = 0;
+        sflag[c] = 0;
+      }
     }
     __syncthreads();
-    if (lane < 8) { cmask[lane] = 0; smask[lane] = 0; }
-    __syncthreads();
+  }
+  for (int o = lane * 16; o < n; o += 1024) {
+    if (o + 16 <= n) {
+      *reinterpret_cast<uint4*>(mtf + o) = *reinterpret_cast<const uint4*>(buf + o);
+    } else {
+      for (int q = o; q < n; q++) mtf[q] = buf[q];
+    }
   }
   if (lane == 0) d.tc_fnz[t] = fnz;
 }
