@@ -174,11 +174,11 @@ __global__ void __launch_bounds__(256) k_mk_large(Dev d, int cur, int nseg) {
 // ------------------------------------------------------------ pack loading
 struct PackLds {
   int32_t start[1024];
-  int32_t len[1024];
   int32_t blk[1024];
   int32_t lofs[1024];
   int32_t count;
   int32_t total;
+  int32_t last_len;
   int32_t tmp[8];
 };
 
@@ -197,11 +197,12 @@ __device__ inline void load_pack(const Dev& d, int cur, int k, PackLds& P) {
     if (small) {
       int idx = count + ex;
       P.start[idx] = g.start;
-      P.len[idx] = g.len;
       P.blk[idx] = g.blk;
       P.lofs[idx] = d.pa[i];
+      if (ex == tot - 1) P.last_len = g.len;
     }
     count += tot;
+    __syncthreads();
   }
   __syncthreads();
   if (count > 0) {
@@ -209,7 +210,7 @@ __device__ inline void load_pack(const Dev& d, int cur, int k, PackLds& P) {
     __syncthreads();
     for (int j = threadIdx.x; j < count; j += 256) P.lofs[j] -= base_pa;
     __syncthreads();
-    if (threadIdx.x == 0) P.total = P.lofs[count - 1] + P.len[count - 1];
+    if (threadIdx.x == 0) P.total = P.lofs[count - 1] + P.last_len;
   }
   if (threadIdx.x == 0) P.count = count;
   __syncthreads();
@@ -313,34 +314,56 @@ __device__ inline void emit_groups(const Dev& d, int nxt, bool has, Seg g, int32
 }
 
 // ----------------------------------------------------------- small groups
-// One workgroup sorts a pack of <= kPack rows in LDS.
-__global__ void __launch_bounds__(256) k_sort_small(Dev d, int cur, int keybits) {
+// One workgroup sorts a pack of <= kPack rows in LDS.  KT holds the
+// composite key (group in pack << keybits | key): 32 bits whenever they fit.
+template <typename KT>
+__device__ inline void sort_small_body(const Dev& d, int cur, int keybits, int k) {
   __shared__ PackLds L;
-  __shared__ uint64_t K0[kPack], K1[kPack];
+  __shared__ KT K0[kPack], K1[kPack];
   __shared__ uint32_t V0[kPack], V1[kPack];
   __shared__ uint32_t cnt[4][256];
+  __shared__ int32_t xs[256];
   __shared__ int32_t s_base;
   __shared__ int32_t tmp[8];
   const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
-  const int k = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int nxt = cur ^ 1;
   load_pack(d, cur, k, L);
   if (L.count == 0) return;
   const int total = L.total;
-  // load composite keys (group-in-pack << keybits | key) and SA values
-  for (int e = tid; e < kPack; e += 256) {
-    if (e < total) {
-      int s = find_seg(L, e);
-      uint64_t off = d.desc[L.blk[s]].off;
-      int j = L.start[s] + (e - L.lofs[s]);
-      K0[e] = ((uint64_t)s << keybits) | d.KEY[off + j];
-      V0[e] = d.SA[off + j];
+  // group of every row: mark group starts, then a running max over rows
+  uint16_t* sid = reinterpret_cast<uint16_t*>(K1);  // free until the first pass
+  for (int e = tid; e < kPack; e += 256) sid[e] = 0;
+  __syncthreads();
+  for (int j = tid; j < L.count; j += 256) sid[L.lofs[j]] = (uint16_t)j;
+  __syncthreads();
+  {
+    const int e0 = tid * (kPack / 256);
+    int mx = 0;
+#pragma unroll
+    for (int q = 0; q < kPack / 256; q++) mx = max(mx, (int)sid[e0 + q]);
+    const int incl = wg_incl_max<256, int>(mx, tmp, 0);
+    xs[tid] = incl;
+    __syncthreads();
+    int run = tid > 0 ? xs[tid - 1] : 0;
+#pragma unroll
+    for (int q = 0; q < kPack / 256; q++) {
+      run = max(run, (int)sid[e0 + q]);
+      sid[e0 + q] = (uint16_t)run;
     }
+  }
+  __syncthreads();
+  // composite keys and SA values
+  for (int e = tid; e < total; e += 256) {
+    const int sg = sid[e];
+    const uint64_t off = d.desc[L.blk[sg]].off;
+    const int j = L.start[sg] + (e - L.lofs[sg]);
+    K0[e] = ((KT)sg << keybits) | (KT)d.KEY[off + j];
+    V0[e] = d.SA[off + j];
   }
   __syncthreads();
   const int nbits = keybits + bits_for((uint32_t)(L.count - 1));
   const int npass = (nbits + 7) / 8;
-  uint64_t* Ks = K0; uint64_t* Kd = K1;
+  KT* Ks = K0; KT* Kd = K1;
   uint32_t* Vs = V0; uint32_t* Vd = V1;
   for (int pass = 0; pass < npass; pass++) {
     const int shift = pass * 8;
@@ -376,62 +399,81 @@ __global__ void __launch_bounds__(256) k_sort_small(Dev d, int cur, int keybits)
       }
     }
     __syncthreads();
-    uint64_t* tk = Ks; Ks = Kd; Kd = tk;
+    KT* tk = Ks; Ks = Kd; Kd = tk;
     uint32_t* tv = Vs; Vs = Vd; Vd = tv;
   }
   // heads: thread owns rows [8*tid, 8*tid+8)
   const int e0 = tid * 8;
   int lastB = -1;
   uint32_t bm = 0;
+  KT kk[9];
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    int e = e0 + q;
-    if (e < total && (e == 0 || Ks[e] != Ks[e - 1])) { bm |= 1u << q; lastB = e; }
+  for (int q = 0; q < 9; q++) kk[q] = (e0 + q < total) ? Ks[e0 + q] : (KT)0;
+  {
+    KT prev = e0 > 0 ? Ks[e0 - 1] : (KT)0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      int e = e0 + q;
+      if (e < total && (e == 0 || kk[q] != prev)) { bm |= 1u << q; lastB = e; }
+      prev = kk[q];
+    }
   }
   int incl = wg_incl_max<256, int>(lastB, tmp, -1);
-  __shared__ int32_t xs[256];
   xs[tid] = incl;
   __syncthreads();
   int run = tid > 0 ? xs[tid - 1] : -1;
-  bool has = false;
-  Seg ng{0, 0, 0, 0};
   int head_of[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     if (bm & (1u << q)) run = e0 + q;
     head_of[q] = run;
   }
+  uint32_t endm = 0;
+  int nout = 0;
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     int e = e0 + q;
     if (e < total) {
-      int s = (int)(Ks[e] >> keybits);
-      int b = L.blk[s];
+      int sg = (int)(kk[q] >> keybits);
+      int b = L.blk[sg];
       uint64_t off = d.desc[b].off;
-      int j = L.start[s] + (e - L.lofs[s]);
-      int hj = L.start[s] + (head_of[q] - L.lofs[s]);
+      int j = L.start[sg] + (e - L.lofs[sg]);
+      int hj = L.start[sg] + (head_of[q] - L.lofs[sg]);
       uint32_t v = Vs[e];
       d.SA[off + j] = v;
       d.RK[off + v] = (uint32_t)hj;
-    }
-  }
-  Seg out[4];
-  int nout = 0;
-  (void)has;
-  (void)ng;
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    int e = e0 + q;
-    if (e < total) {
-      bool end = (e == total - 1) || (Ks[e + 1] != Ks[e]);
-      int glen = e - head_of[q] + 1;
-      if (end && glen > 1) {  // >= 2 rows each, so <= 4 end in 8 rows
-        int s = (int)(Ks[e] >> keybits);
-        out[nout++ & 3] = Seg{L.blk[s], L.start[s] + (head_of[q] - L.lofs[s]), glen, 0};
+      // a group of >= 2 rows ends here
+      if (((e == total - 1) || (kk[q + 1] != kk[q])) && e > head_of[q]) {
+        endm |= 1u << q;
+        nout++;
       }
     }
   }
-  emit_groups_n<4>(d, nxt, nout, out, tmp, &s_base);
+  int tot;
+  const int ex = wg_excl_sum<256, int>(nout, tmp, &tot);
+  if (tid == 0 && tot > 0) s_base = atomicAdd(&d.ctr[nxt], tot);
+  __syncthreads();
+  if (endm) {
+    int idx = s_base + ex;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      if ((endm >> q) & 1u) {
+        const int sg = (int)(kk[q] >> keybits);
+        const Seg ng{L.blk[sg], L.start[sg] + (head_of[q] - L.lofs[sg]), e0 + q - head_of[q] + 1, 0};
+        if (idx < d.seg_cap) d.seg[nxt][idx] = ng;
+        else d.ctr[CTR_ERR] = 1;
+        idx++;
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sort_small(Dev d, int cur, int keybits) {
+  sort_small_body<uint32_t>(d, cur, keybits, (int)xcd_remap(blockIdx.x, gridDim.x));
+}
+
+__global__ void __launch_bounds__(256) k_sort_small64(Dev d, int cur, int keybits) {
+  sort_small_body<uint64_t>(d, cur, keybits, (int)xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // ----------------------------------------------------------- large groups
@@ -565,6 +607,8 @@ __global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
   __shared__ int32_t tmp[8];
   __shared__ int32_t s_carry, s_base;
   __shared__ int32_t xs[256];
+  __shared__ uint32_t sk[kTile + 2 + (kTile + 2) / 16 + 1];
+  __shared__ uint32_t ss[kTile + kTile / 16];
   const int tid = threadIdx.x;
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int si = d.ltile_seg[t];
@@ -591,33 +635,46 @@ __global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
     if (tid == 0) s_carry = max(max(tmp[0], tmp[1]), max(tmp[2], tmp[3]));
     __syncthreads();
   }
+  // stage the tile's keys (with the row before and after) and SA values in
+  // LDS; row e sits at e + e/16 (one word of skew per 16 rows), so the
+  // 16-row runs the threads own below are read without bank conflicts
+  for (int e = tid - 1; e <= n; e += 256) {
+    const int p = pos + e;
+    const int ix = e + 1;
+    sk[ix + (ix >> 4)] = (p >= 0 && p < g.len) ? K[base + p] : 0u;
+  }
+  for (int e = tid; e < n; e += 256) {
+    const uint32_t v = S[base + pos + e];
+    ss[e + (e >> 4)] = v;
+    if (fin) d.SA[base + pos + e] = v;
+  }
+  __syncthreads();
   // thread owns rows [16*tid, 16*tid+16) of the tile
   const int e0 = tid * 16;
   uint32_t kk[17];
   uint32_t bm = 0;
   int lastB = -1;
-#pragma unroll
-  for (int q = 0; q < 16; q++) {
-    int e = e0 + q;
-    int p = pos + e;
-    kk[q] = e < n ? K[base + p] : 0u;
-    if (e < n) {
-      uint32_t prev = q > 0 ? kk[q - 1] : (p > 0 ? K[base + p - 1] : 0u);
-      if (p == 0 || kk[q] != prev) { bm |= 1u << q; lastB = p; }
-    }
-  }
   {
-    int e = e0 + 16;
-    int p = pos + e;
-    kk[16] = (e0 + 15 < n && p < g.len) ? K[base + p] : 0u;
+    const int ix = e0;  // row e0 - 1
+    uint32_t prev = sk[ix + (ix >> 4)];
+#pragma unroll
+    for (int q = 0; q < 17; q++) {
+      const int jx = e0 + q + 1;
+      kk[q] = sk[jx + (jx >> 4)];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int e = e0 + q;
+      const int p = pos + e;
+      if (e < n && (p == 0 || kk[q] != prev)) { bm |= 1u << q; lastB = p; }
+      prev = kk[q];
+    }
   }
   int incl = wg_incl_max<256, int>(lastB, tmp, s_carry);
   xs[tid] = incl;
   __syncthreads();
   int run = tid > 0 ? xs[tid - 1] : s_carry;
   run = max(run, s_carry);
-  bool has = false;
-  Seg ng{0, 0, 0, 0};
   int head_of[16];
 #pragma unroll
   for (int q = 0; q < 16; q++) {
@@ -626,31 +683,36 @@ __global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
   }
 #pragma unroll
   for (int q = 0; q < 16; q++) {
-    int e = e0 + q;
-    if (e < n) {
-      int p = pos + e;
-      uint32_t v = S[base + p];
-      if (fin) d.SA[base + p] = v;
-      d.RK[off + v] = (uint32_t)(g.start + head_of[q]);
-    }
+    const int e = e0 + q;
+    if (e < n) d.RK[off + ss[e + (e >> 4)]] = (uint32_t)(g.start + head_of[q]);
   }
-  Seg out[8];
+  // groups of >= 2 rows ending in this thread's rows
+  uint32_t endm = 0;
   int nout = 0;
-  (void)has;
-  (void)ng;
 #pragma unroll
   for (int q = 0; q < 16; q++) {
-    int e = e0 + q;
-    if (e < n) {
-      int p = pos + e;
-      bool end = (p == g.len - 1) || (kk[q + 1] != kk[q]);
-      int glen = p - head_of[q] + 1;
-      // a group ending here has >= 2 rows, and only the first one can
-      // start before this thread's rows, so <= 8 end in 16 rows
-      if (end && glen > 1) out[nout++ & 7] = Seg{g.blk, g.start + head_of[q], glen, 0};
+    const int e = e0 + q;
+    const int p = pos + e;
+    if (e < n && ((p == g.len - 1) || (kk[q + 1] != kk[q])) && p > head_of[q]) {
+      endm |= 1u << q;
+      nout++;
     }
   }
-  emit_groups_n<8>(d, nxt, nout, out, tmp, &s_base);
+  int tot;
+  const int ex = wg_excl_sum<256, int>(nout, tmp, &tot);
+  if (tid == 0 && tot > 0) s_base = atomicAdd(&d.ctr[nxt], tot);
+  __syncthreads();
+  if (endm) {
+    int idx = s_base + ex;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      if ((endm >> q) & 1u) {
+        if (idx < d.seg_cap) d.seg[nxt][idx] = Seg{g.blk, g.start + head_of[q], pos + e0 + q - head_of[q] + 1, 0};
+        else d.ctr[CTR_ERR] = 1;
+        idx++;
+      }
+    }
+  }
 }
 
 void launch_bwt_init(const Dev& d, hipStream_t s) {
@@ -668,8 +730,10 @@ void launch_bwt_round(const Dev& d, int cur, int round, int32_t P, int key_bits,
     probe_end("k_keys", s);
   }
   if (npacks > 0) {
+    // composite key: group index in the pack (< 1024) above the key bits
     probe_begin("k_sort_small", s);
-    k_sort_small<<<npacks, 256, 0, s>>>(d, cur, key_bits);
+    if (key_bits + 10 <= 32) k_sort_small<<<npacks, 256, 0, s>>>(d, cur, key_bits);
+    else k_sort_small64<<<npacks, 256, 0, s>>>(d, cur, key_bits);
     probe_end("k_sort_small", s);
   }
   if (ntiles > 0) {
