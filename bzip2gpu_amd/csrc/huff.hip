@@ -91,49 +91,61 @@ __global__ void __launch_bounds__(64) k_huff_init(Dev d) {
   }
 }
 
-// One thread per 50-symbol group: costs, first strict minimum, counts.
+// One thread per 50-symbol group: costs, first strict minimum, counts.  A
+// workgroup covers the groups of kSelTiles capacity tiles; the six code
+// lengths of a symbol are packed in one 64-bit word (10 bits per table, a
+// group's cost is at most 50 * 17 < 1024), so a group's six costs are one
+// 64-bit sum.
+constexpr int kSelTiles = 16;
 __global__ void __launch_bounds__(256) k_huff_select(Dev d) {
-  __shared__ uint8_t L[6][260];
+  __shared__ uint64_t PL[260];
   __shared__ int32_t rf[6][260];
+  __shared__ uint16_t S[256 * kGSize];
+  const int tid = threadIdx.x;
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int b = d.tc_blk[t];
   const BlockDesc D = d.desc[b];
-  const int s = d.st[b].s;
   const int k = t - D.tile_c0;
+  if (k % kSelTiles) return;
+  const int s = d.st[b].s;
+  const int nsel = (s + kGSize - 1) / kGSize;
   const int g0 = (k * kTile + kGSize - 1) / kGSize;
-  const int g1 = min(((k + 1) * kTile + kGSize - 1) / kGSize, (s + kGSize - 1) / kGSize);
+  const int g1 = min(((k + kSelTiles) * kTile + kGSize - 1) / kGSize, nsel);
   if (g0 >= g1) return;
   const int ng = d.st[b].n_groups;
-  for (int q = threadIdx.x; q < 6 * 260; q += 256) {
-    (&L[0][0])[q] = d.len[(int64_t)b * 6 * 260 + q];
-    (&rf[0][0])[q] = 0;
-  }
-  __syncthreads();
-  // stage this tile's groups' symbols (coalesced) in LDS
-  __shared__ uint16_t S[(kTile / kGSize + 2) * kGSize];
-  const int sbase = g0 * kGSize, send = min(g1 * kGSize, s);
-  for (int i = sbase + (int)threadIdx.x; i < send; i += 256) S[i - sbase] = d.SYM[D.off + i];
-  __syncthreads();
-  const int g = g0 + (int)threadIdx.x;
-  if (g < g1) {
-    const int gs = g * kGSize, ge = min(gs + kGSize, s);
-    const uint16_t* sym = S - sbase;
-    uint32_t cost[6] = {0, 0, 0, 0, 0, 0};
-    for (int i = gs; i < ge; i++) {
-      uint16_t v = sym[i];
+  const uint8_t* len = d.len + (int64_t)b * 6 * 260;
+  for (int v = tid; v < 260; v += 256) {
+    uint64_t w = 0;
 #pragma unroll
-      for (int tt = 0; tt < 6; tt++) cost[tt] += L[tt][v];
-    }
-    uint32_t bc = 0xFFFFFFFFu;
-    int bt = 0;
-    for (int tt = 0; tt < ng; tt++)
-      if (cost[tt] < bc) { bc = cost[tt]; bt = tt; }
-    d.sel[D.sel_off + g] = (uint8_t)bt;
-    for (int i = gs; i < ge; i++) atomicAdd(&rf[bt][sym[i]], 1);
+    for (int tt = 0; tt < 6; tt++) w |= (uint64_t)len[tt * 260 + v] << (10 * tt);
+    PL[v] = w;
   }
+  for (int q = tid; q < 6 * 260; q += 256) (&rf[0][0])[q] = 0;
   __syncthreads();
+  for (int gb = g0; gb < g1; gb += 256) {
+    const int gend = min(gb + 256, g1);
+    const int sbase = gb * kGSize, send = min(gend * kGSize, s);
+    for (int i = sbase + tid; i < send; i += 256) S[i - sbase] = d.SYM[D.off + i];
+    __syncthreads();
+    const int g = gb + tid;
+    if (g < gend) {
+      const int gs = g * kGSize, ge = min(gs + kGSize, s);
+      const uint16_t* sym = S - sbase;
+      uint64_t acc = 0;
+      for (int i = gs; i < ge; i++) acc += PL[sym[i]];
+      uint32_t bc = (uint32_t)(acc & 1023u);
+      int bt = 0;
+      for (int tt = 1; tt < ng; tt++) {
+        const uint32_t c = (uint32_t)(acc >> (10 * tt)) & 1023u;
+        if (c < bc) { bc = c; bt = tt; }
+      }
+      d.sel[D.sel_off + g] = (uint8_t)bt;
+      for (int i = gs; i < ge; i++) atomicAdd(&rf[bt][sym[i]], 1);
+    }
+    __syncthreads();
+  }
   int32_t* grf = d.rfreq + (int64_t)b * 6 * 260;
-  for (int q = threadIdx.x; q < 6 * 260; q += 256) {
+  for (int q = tid; q < 6 * 260; q += 256) {
     int v = (&rf[0][0])[q];
     if (v) atomicAdd(&grf[q], v);
   }

@@ -21,9 +21,13 @@ namespace bzg {
 
 void launch_tile_scan_c(const Dev& d, hipStream_t s);
 
-// L[i] = T[(SA[i]-1) mod m] (fmtf.cu:55), orig_ptr, per-tile last occurrence
+// L[i] = T[(SA[i]-1) mod m] (fmtf.cu:55), orig_ptr, per-tile last occurrence.
+// Each wave walks 1024 consecutive rows 64 at a time; within a step the
+// last lane of each byte value (found from 8 ballots) records the position,
+// so later steps overwrite earlier ones without atomics.
 __global__ void __launch_bounds__(256) k_mtf_gather(Dev d) {
-  __shared__ int32_t lo[256];
+  __shared__ int32_t lo[4][256];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int b = d.tc_blk[t];
   const BlockDesc D = d.desc[b];
@@ -31,19 +35,32 @@ __global__ void __launch_bounds__(256) k_mtf_gather(Dev d) {
   const int k = t - D.tile_c0;
   const int start = k * kTile;
   if (start >= m) return;
-  lo[threadIdx.x] = -1;
+  lo[0][tid] = -1; lo[1][tid] = -1; lo[2][tid] = -1; lo[3][tid] = -1;
   __syncthreads();
   const int n = min(kTile, m - start);
-  for (int e = threadIdx.x; e < n; e += 256) {
-    int i = start + e;
-    uint32_t v = d.SA[D.off + i];
-    uint8_t c = d.T[D.off + (v == 0 ? (uint32_t)(m - 1) : v - 1)];
-    d.MTF[D.off + i] = c;
-    if (v == 0) d.st[b].orig_ptr = i;
-    atomicMax(&lo[c], i);
+  const uint64_t above = ~((2ull << lane) - 1ull);
+  for (int q = 0; q < 16; q++) {
+    const int e = wv * 1024 + q * 64 + lane;
+    const bool valid = e < n;
+    const int i = start + e;
+    uint32_t c = 0;
+    if (valid) {
+      const uint32_t v = d.SA[D.off + i];
+      c = d.T[D.off + (v == 0 ? (uint32_t)(m - 1) : v - 1)];
+      d.MTF[D.off + i] = (uint8_t)c;
+      if (v == 0) d.st[b].orig_ptr = i;
+    }
+    uint64_t peers = ballot(valid);
+#pragma unroll
+    for (int bb = 0; bb < 8; bb++) {
+      const bool bit = (c >> bb) & 1u;
+      const uint64_t mm = ballot(bit);
+      peers &= bit ? mm : ~mm;
+    }
+    if (valid && !(peers & above)) lo[wv][c] = i;
   }
   __syncthreads();
-  d.lo[(int64_t)t * 256 + threadIdx.x] = lo[threadIdx.x];
+  d.lo[(int64_t)t * 256 + tid] = max(max(lo[0][tid], lo[1][tid]), max(lo[2][tid], lo[3][tid]));
 }
 
 // Per block: entering list state of every segment (exclusive max-scan).
