@@ -60,9 +60,10 @@ __global__ void __launch_bounds__(256) k_bwt_init(Dev d) {
   const BlockDesc D = d.desc[b];
   const int m = d.st[b].m;
   const int k = t - D.tile_c0;
+  // SA entries carry the row's BWT byte T[(i-1) mod m] in their top 8 bits
   for (int e = threadIdx.x; e < kTile; e += 256) {
     int i = k * kTile + e;
-    if (i < m) d.SA[D.off + i] = (uint32_t)i;
+    if (i < m) d.SA[D.off + i] = (uint32_t)i | ((uint32_t)d.T[D.off + (i == 0 ? m - 1 : i - 1)] << 24);
   }
   if (k == 0 && threadIdx.x == 0) {
     if (m > 1) {
@@ -227,7 +228,7 @@ __device__ inline int find_seg(const PackLds& P, int e) {  // last j with lofs[j
 
 // key of the row at block-local SA index j
 __device__ inline uint32_t make_key(const Dev& d, uint64_t off, int m, int j, int round, int P) {
-  uint32_t v = d.SA[off + j];
+  uint32_t v = d.SA[off + j] & kSAMask;
   if (round == 0) {
     uint32_t k = 0;
     uint32_t p = v;
@@ -441,7 +442,7 @@ __device__ inline void sort_small_body(const Dev& d, int cur, int keybits, int k
       int hj = L.start[sg] + (head_of[q] - L.lofs[sg]);
       uint32_t v = Vs[e];
       d.SA[off + j] = v;
-      d.RK[off + v] = (uint32_t)hj;
+      d.RK[off + (v & kSAMask)] = (uint32_t)hj;
       // a group of >= 2 rows ends here
       if (((e == total - 1) || (kk[q + 1] != kk[q])) && e > head_of[q]) {
         endm |= 1u << q;
@@ -684,7 +685,7 @@ __global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
 #pragma unroll
   for (int q = 0; q < 16; q++) {
     const int e = e0 + q;
-    if (e < n) d.RK[off + ss[e + (e >> 4)]] = (uint32_t)(g.start + head_of[q]);
+    if (e < n) d.RK[off + (ss[e + (e >> 4)] & kSAMask)] = (uint32_t)(g.start + head_of[q]);
   }
   // groups of >= 2 rows ending in this thread's rows
   uint32_t endm = 0;

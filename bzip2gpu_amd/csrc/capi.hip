@@ -514,7 +514,7 @@ bzg_status bzg_bwt(const uint8_t* d_in, int32_t m, int32_t* d_sa, int32_t* h_rou
   if ((e = set_field<int32_t>(d, offsetof(BlockState, m), m, L.s))) return hip_fail(e, "bwt");
   int rounds = 0;
   if ((e = E.run_bwt(L.s, &rounds))) return hip_fail(e, "bwt");
-  ck(hipMemcpyAsync(d_sa, d.SA, (size_t)m * 4, hipMemcpyDeviceToDevice, L.s));
+  launch_sa_strip(d, d_sa, L.s);
   if ((e = sync(L.s))) return hip_fail(e, "bwt");
   if (h_rounds) *h_rounds = rounds;
   return BZG_OK;
@@ -557,8 +557,8 @@ bzg_status bzg_mtf(const uint8_t* d_rle1, const int32_t* d_sa, int32_t m, uint8_
   const Dev& d = E.dev();
   int e;
   ck(hipMemcpyAsync(d.T, d_rle1, (size_t)m, hipMemcpyDeviceToDevice, L.s));
-  ck(hipMemcpyAsync(d.SA, d_sa, (size_t)m * 4, hipMemcpyDeviceToDevice, L.s));
   if ((e = set_field<int32_t>(d, offsetof(BlockState, m), m, L.s))) return hip_fail(e, "mtf");
+  launch_sa_attach(d, d_sa, L.s);
   launch_presence(d, L.s);
   launch_mtf(d, L.s);
   if ((e = (int)hipGetLastError())) return hip_fail(e, "mtf");

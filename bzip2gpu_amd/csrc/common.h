@@ -16,6 +16,10 @@ constexpr int kMaxAlpha = 258;
 constexpr int kGSize = 50;            // symbols per selector group
 constexpr int kMaxBlock = 900000;     // raw bytes per block (reference chunk size)
 constexpr int kStageMax = 1 << 24;    // largest single block of a per-stage call
+// Suffix-array entries hold the start index in the low 24 bits and the
+// rotation's preceding byte (its BWT output byte) in the top 8 bits.
+constexpr uint32_t kSAMask = 0x00FFFFFFu;
+static_assert(kStageMax <= (int)kSAMask + 1, "start indices must fit 24 bits");
 constexpr uint32_t kCrcPoly = 0x04C11DB7u;
 
 // Capacity (elements) reserved for a block of n raw bytes: RLE1 can turn

@@ -21,7 +21,8 @@ namespace bzg {
 
 void launch_tile_scan_c(const Dev& d, hipStream_t s);
 
-// L[i] = T[(SA[i]-1) mod m] (fmtf.cu:55), orig_ptr, per-tile last occurrence.
+// L[i] = T[(SA[i]-1) mod m] (fmtf.cu:55) from the byte each SA entry carries,
+// orig_ptr, per-tile last occurrence.
 // Each wave walks 1024 consecutive rows 64 at a time; within a step the
 // last lane of each byte value (found from 8 ballots) records the position,
 // so later steps overwrite earlier ones without atomics.
@@ -46,9 +47,9 @@ __global__ void __launch_bounds__(256) k_mtf_gather(Dev d) {
     uint32_t c = 0;
     if (valid) {
       const uint32_t v = d.SA[D.off + i];
-      c = d.T[D.off + (v == 0 ? (uint32_t)(m - 1) : v - 1)];
+      c = v >> 24;  // T[(start - 1) mod m], carried by the sort
       d.MTF[D.off + i] = (uint8_t)c;
-      if (v == 0) d.st[b].orig_ptr = i;
+      if ((v & kSAMask) == 0) d.st[b].orig_ptr = i;
     }
     uint64_t peers = ballot(valid);
 #pragma unroll

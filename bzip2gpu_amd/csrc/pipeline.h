@@ -127,6 +127,8 @@ void launch_rle2(const Dev& d, hipStream_t s);       // RLE2 (needs tc_fnz, n_in
 void launch_tile_scan_c(const Dev& d, hipStream_t s);
 // helpers for the per-stage entry points
 void launch_presence(const Dev& d, hipStream_t s);   // st.present from T[0..m)
+void launch_sa_attach(const Dev& d, const int32_t* sa, hipStream_t s);  // caller SA -> SA entries
+void launch_sa_strip(const Dev& d, int32_t* sa, hipStream_t s);         // SA entries -> caller SA
 void launch_fnz(const Dev& d, hipStream_t s);        // tc_fnz from MTF[0..m)
 void launch_sym_hist(const Dev& d, hipStream_t s);   // freq from SYM[0..s)
 void launch_huffman(const Dev& d, hipStream_t s);    // tables, selectors, codes, headers

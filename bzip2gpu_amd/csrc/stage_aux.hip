@@ -56,6 +56,33 @@ __global__ void __launch_bounds__(256) k_sym_hist(Dev d) {
     if (h[q]) atomicAdd(&d.freq[(int64_t)b * 260 + q], h[q]);
 }
 
+// A caller's suffix array -> internal entries (start | preceding byte << 24).
+__global__ void __launch_bounds__(256) k_sa_attach(Dev d, const int32_t* sa) {
+  const int t = blockIdx.x;
+  const int b = d.tc_blk[t];
+  const BlockDesc D = d.desc[b];
+  const int m = d.st[b].m;
+  const int start = (t - D.tile_c0) * kTile;
+  for (int e = threadIdx.x; e < kTile && start + e < m; e += 256) {
+    const uint32_t v = (uint32_t)sa[start + e];
+    const uint32_t c = v < (uint32_t)m ? d.T[D.off + (v == 0 ? (uint32_t)(m - 1) : v - 1)] : 0u;
+    d.SA[D.off + start + e] = (v & kSAMask) | (c << 24);
+  }
+}
+
+// Internal entries -> plain start indices.
+__global__ void __launch_bounds__(256) k_sa_strip(Dev d, int32_t* sa) {
+  const int t = blockIdx.x;
+  const int b = d.tc_blk[t];
+  const BlockDesc D = d.desc[b];
+  const int m = d.st[b].m;
+  const int start = (t - D.tile_c0) * kTile;
+  for (int e = threadIdx.x; e < kTile && start + e < m; e += 256)
+    sa[start + e] = (int32_t)(d.SA[D.off + start + e] & kSAMask);
+}
+
+void launch_sa_attach(const Dev& d, const int32_t* sa, hipStream_t s) { k_sa_attach<<<d.ntile_c, 256, 0, s>>>(d, sa); }
+void launch_sa_strip(const Dev& d, int32_t* sa, hipStream_t s) { k_sa_strip<<<d.ntile_c, 256, 0, s>>>(d, sa); }
 void launch_presence(const Dev& d, hipStream_t s) { k_presence<<<d.ntile_c, 256, 0, s>>>(d); }
 void launch_fnz(const Dev& d, hipStream_t s) { k_fnz<<<d.ntile_c, 256, 0, s>>>(d); }
 void launch_sym_hist(const Dev& d, hipStream_t s) { k_sym_hist<<<d.ntile_c, 256, 0, s>>>(d); }
