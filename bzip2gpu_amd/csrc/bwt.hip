@@ -610,6 +610,9 @@ __global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
   __shared__ int32_t xs[256];
   __shared__ uint32_t sk[kTile + 2 + (kTile + 2) / 16 + 1];
   __shared__ uint32_t ss[kTile + kTile / 16];
+  // per-bucket counters reuse the key staging area once the keys are read
+  int32_t* bcnt = reinterpret_cast<int32_t*>(sk);
+  static_assert((kStageMax + kRkBucket - 1) / kRkBucket <= kTile, "bucket counters fit the key area");
   const int tid = threadIdx.x;
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int si = d.ltile_seg[t];
@@ -682,10 +685,36 @@ __global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
     if (bm & (1u << q)) run = pos + e0 + q;
     head_of[q] = run;
   }
+  // Rank updates RK[start] = head go to the block's buckets of kRkBucket
+  // starts (k_rk_apply writes them), so that a bucket's updates land in one
+  // 32 KiB window that is staged in LDS.
+  {
+    const BlockDesc D = d.desc[g.blk];
+    const int nbk = (d.st[g.blk].m + kRkBucket - 1) / kRkBucket;
+    for (int j = tid; j < nbk; j += 256) bcnt[j] = 0;
+    __syncthreads();
+    uint32_t lr[16];
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
-    const int e = e0 + q;
-    if (e < n) d.RK[off + (ss[e + (e >> 4)] & kSAMask)] = (uint32_t)(g.start + head_of[q]);
+    for (int q = 0; q < 16; q++) {
+      const int e = e0 + q;
+      lr[q] = e < n ? (uint32_t)atomicAdd(&bcnt[(ss[e + (e >> 4)] & kSAMask) / kRkBucket], 1) : 0u;
+    }
+    __syncthreads();
+    for (int j = tid; j < nbk; j += 256) {
+      const int c = bcnt[j];
+      if (c) bcnt[j] = atomicAdd(&d.rkb_cnt[D.tile_c0 + j * (kRkBucket / kTile)], c);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int e = e0 + q;
+      if (e < n) {
+        const uint32_t v = ss[e + (e >> 4)] & kSAMask;
+        const int bk = (int)(v / kRkBucket);
+        d.RKB[off + (uint64_t)bk * kRkBucket + bcnt[bk] + lr[q]] =
+            ((uint64_t)(uint32_t)(g.start + head_of[q]) << 32) | v;
+      }
+    }
   }
   // groups of >= 2 rows ending in this thread's rows
   uint32_t endm = 0;
@@ -714,6 +743,47 @@ __global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
       }
     }
   }
+}
+
+// Rank-update buckets: cleared before a round's large groups, applied after.
+__global__ void __launch_bounds__(256) k_rk_zero(Dev d, int ntile) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < ntile) d.rkb_cnt[i] = 0;
+}
+
+// A bucket's updates are applied to an LDS copy of its window of RK, then
+// the touched entries are written out in order.
+__global__ void __launch_bounds__(256) k_rk_apply(Dev d) {
+  __shared__ uint32_t win[kRkBucket];
+  __shared__ uint8_t touched[kRkBucket];
+  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
+  const int b = d.tc_blk[t];
+  const BlockDesc D = d.desc[b];
+  const int k = t - D.tile_c0;
+  if (k % (kRkBucket / kTile)) return;
+  const int cnt = d.rkb_cnt[t];
+  if (cnt == 0) return;
+  const uint32_t base = (uint32_t)k * kTile;  // first start of the bucket
+  for (int i = threadIdx.x; i < kRkBucket; i += 256) touched[i] = 0;
+  __syncthreads();
+  const uint64_t* src = d.RKB + D.off + base;
+  for (int i0 = threadIdx.x; i0 < cnt; i0 += 1024) {
+    uint64_t p[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) p[q] = i0 + 256 * q < cnt ? src[i0 + 256 * q] : ~0ull;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (p[q] != ~0ull) {
+        const uint32_t o = (uint32_t)p[q] - base;
+        win[o] = (uint32_t)(p[q] >> 32);
+        touched[o] = 1;
+      }
+    }
+  }
+  __syncthreads();
+  const int span = min(kRkBucket, d.st[b].m - (int)base);
+  for (int i = threadIdx.x; i < span; i += 256)
+    if (touched[i]) d.RK[D.off + base + i] = win[i];
 }
 
 void launch_bwt_init(const Dev& d, hipStream_t s) {
@@ -747,10 +817,12 @@ void launch_bwt_round(const Dev& d, int cur, int round, int32_t P, int key_bits,
       probe_end("k_lscatter", s);
     }
     const int fin = npass & 1;
+    k_rk_zero<<<(d.ntile_c + 255) / 256, 256, 0, s>>>(d, d.ntile_c);
     k_lsum<<<ntiles, 256, 0, s>>>(d, cur, fin);
     probe_begin("k_lheads", s);
     k_lheads<<<ntiles, 256, 0, s>>>(d, cur, fin);
     probe_end("k_lheads", s);
+    k_rk_apply<<<d.ntile_c, 256, 0, s>>>(d);
   }
 }
 

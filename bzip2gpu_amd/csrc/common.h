@@ -20,6 +20,8 @@ constexpr int kStageMax = 1 << 24;    // largest single block of a per-stage cal
 // rotation's preceding byte (its BWT output byte) in the top 8 bits.
 constexpr uint32_t kSAMask = 0x00FFFFFFu;
 static_assert(kStageMax <= (int)kSAMask + 1, "start indices must fit 24 bits");
+// Rank updates of a round are bucketed by start index, kRkBucket per bucket.
+constexpr int kRkBucket = 8192;
 constexpr uint32_t kCrcPoly = 0x04C11DB7u;
 
 // Capacity (elements) reserved for a block of n raw bytes: RLE1 can turn

@@ -223,6 +223,8 @@ int Engine::setup(const uint8_t* d_in, hipStream_t s) {
     d.T = c.take<uint8_t>(CAP + 64);
     d.SA = c.take<uint32_t>(CAP + 16);
     d.RK = c.take<uint32_t>(CAP + 16);
+    d.RKB = c.take<uint64_t>(CAP + 16);
+    d.rkb_cnt = c.take<int32_t>(NTc + 1);
     d.KEY = c.take<uint32_t>(CAP + 16);
     d.SA2 = c.take<uint32_t>(CAP + 16);
     d.KEY2 = c.take<uint32_t>(CAP + 16);

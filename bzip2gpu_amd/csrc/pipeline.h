@@ -57,6 +57,8 @@ struct Dev {
   uint8_t* T;          // RLE1 output
   uint32_t* SA;
   uint32_t* RK;
+  uint64_t* RKB;       // bucketed (head << 32 | start) rank updates, kRkBucket starts per bucket
+  int32_t* rkb_cnt;    // entries per bucket, indexed by the bucket's first capacity tile
   uint32_t* KEY;
   uint32_t* SA2;
   uint32_t* KEY2;
