@@ -311,67 +311,93 @@ __global__ void __launch_bounds__(64) k_huff_lengths(Dev d) {
 
 // Codes (huffman.cu:135-146, :352-362), selector MTF (compress.cu:119-137)
 // and the block header bits (compress.cu:290-356) into the block's header
-// words.  Lanes 0..ng-1 assign codes; the selector MTF runs as a wave scan
-// over 64 chunks (list state = last occurrence of each table id).
-__global__ void __launch_bounds__(64) k_huff_codes(Dev d) {
+// words, one 256-thread workgroup per block.  Codes are canonical: thread
+// (table, length) hands out consecutive codes to that length's symbols in
+// symbol order.  The selector MTF runs over 256 chunks whose entering list
+// state comes from a max-scan of per-chunk last occurrences.
+__global__ void __launch_bounds__(256) k_huff_codes(Dev d) {
+  __shared__ uint8_t L[6][260];
+  __shared__ int32_t lcount[6][32];
+  __shared__ uint32_t lbase[6][32];
   __shared__ int32_t tbits[8];
+  __shared__ int32_t tmp[8];
   const int b = blockIdx.x;
-  const int lane = lane_id();
+  const int tid = threadIdx.x;
   BlockState* st = d.st + b;
   const BlockDesc D = d.desc[b];
   const int ng = st->n_groups, nsel = st->n_sel;
   const int alpha = st->n_in_use + 2;
-  if (lane < ng) {
-    const uint8_t* len = d.len + ((int64_t)b * 6 + lane) * 260;
-    uint32_t* code = d.code + ((int64_t)b * 6 + lane) * 260;
-    int mn = 32, mx = 0;
-    for (int v = 0; v < alpha; v++) { mn = min(mn, (int)len[v]); mx = max(mx, (int)len[v]); }
+  for (int q = tid; q < 6 * 32; q += 256) (&lcount[0][0])[q] = 0;
+  __syncthreads();
+  for (int q = tid; q < 6 * 260; q += 256) {
+    const int tb = q / 260, v = q - 260 * tb;
+    const uint8_t l = d.len[(int64_t)b * 6 * 260 + q];
+    L[tb][v] = l;
+    if (tb < ng && v < alpha) atomicAdd(&lcount[tb][l & 31], 1);
+  }
+  __syncthreads();
+  if (tid < ng) {
+    // canonical first code of every length (lengths are 1..20)
     uint32_t vec = 0;
-    for (int l = mn; l <= mx; l++) {
-      for (int v = 0; v < alpha; v++)
-        if (len[v] == l) code[v] = vec++;
-      vec <<= 1;
+    for (int l = 1; l < 32; l++) {
+      lbase[tid][l] = vec;
+      vec = (vec + (uint32_t)lcount[tid][l]) << 1;
     }
     // coding-table bits: 5 + sum(1 + 2*|delta|)
-    int cur = len[0], nb = 5;
+    int cur = L[tid][0], nb = 5;
     for (int v = 0; v < alpha; v++) {
-      int tg = len[v];
+      const int tg = L[tid][v];
       nb += 1 + 2 * (tg > cur ? tg - cur : cur - tg);
       cur = tg;
     }
-    tbits[lane] = nb;
+    tbits[tid] = nb;
   }
-  // selector MTF: lane-chunked, entering state by max-scan of last occurrence
+  __syncthreads();
+  {
+    const int tb = tid >> 5, l = tid & 31;
+    if (tb < ng && l > 0 && lcount[tb][l] > 0) {
+      uint32_t* code = d.code + ((int64_t)b * 6 + tb) * 260;
+      uint32_t c = lbase[tb][l];
+      for (int v = 0; v < alpha; v++)
+        if (L[tb][v] == l) code[v] = c++;
+    }
+  }
+  // selector MTF
   const uint8_t* sel = d.sel + D.sel_off;
   uint8_t* smtf = d.selmtf + D.sel_off;
-  const int per = (nsel + 63) / 64;
-  const int c0 = min(nsel, lane * per), c1 = min(nsel, c0 + per);
+  const int per = (nsel + 255) / 256;
+  const int c0 = min(nsel, tid * per), c1 = min(nsel, c0 + per);
   int lastocc[6];
 #pragma unroll
   for (int q = 0; q < 6; q++) lastocc[q] = -1;
-  for (int i = c0; i < c1; i++) lastocc[sel[i]] = i;
+  for (int i = c0; i < c1; i++) {
+    const int v = sel[i];
+#pragma unroll
+    for (int q = 0; q < 6; q++) lastocc[q] = v == q ? i : lastocc[q];
+  }
   int ent[6];
 #pragma unroll
   for (int q = 0; q < 6; q++) {
-    int v = wave_incl_max(lastocc[q]);
-    int ex = __shfl_up(v, 1, 64);
-    ent[q] = lane == 0 ? -1 : ex;
-    // initial list 0,1,..,ng-1: encode as -(1+q)-1000000000 so never-seen ids
-    // rank after seen ones, in ascending order
+    __shared__ int32_t xs[256];
+    const int v = wg_incl_max<256, int>(lastocc[q], tmp, -1);
+    xs[tid] = v;
+    __syncthreads();
+    ent[q] = tid > 0 ? xs[tid - 1] : -1;
+    __syncthreads();
+    // never-seen ids rank after seen ones, in ascending order (initial list)
     if (ent[q] < 0) ent[q] = -1000000000 - q;
   }
   uint8_t lst[6];
   {
-    // order table ids by ent descending
     int ids[6] = {0, 1, 2, 3, 4, 5};
-    for (int a = 0; a < ng; a++)
-      for (int c = a + 1; c < ng; c++)
-        if (ent[ids[c]] > ent[ids[a]]) { int t2 = ids[a]; ids[a] = ids[c]; ids[c] = t2; }
-    for (int a = 0; a < 6; a++) lst[a] = (uint8_t)ids[a];
+    for (int a2 = 0; a2 < ng; a2++)
+      for (int c = a2 + 1; c < ng; c++)
+        if (ent[ids[c]] > ent[ids[a2]]) { int t2 = ids[a2]; ids[a2] = ids[c]; ids[c] = t2; }
+    for (int a2 = 0; a2 < 6; a2++) lst[a2] = (uint8_t)ids[a2];
   }
   int my_bits = 0;
   for (int i = c0; i < c1; i++) {
-    uint8_t v = sel[i];
+    const uint8_t v = sel[i];
     int j = 0;
     while (lst[j] != v) j++;
     for (int q = j; q > 0; q--) lst[q] = lst[q - 1];
@@ -379,7 +405,6 @@ __global__ void __launch_bounds__(64) k_huff_codes(Dev d) {
     smtf[i] = (uint8_t)j;
     my_bits += j + 1;
   }
-  __syncthreads();
   // header layout
   int nused = 0;
   uint32_t l1 = 0;
@@ -389,17 +414,14 @@ __global__ void __launch_bounds__(64) k_huff_codes(Dev d) {
     if (half) { l1 |= 1u << (15 - r); nused++; }
   }
   const int64_t fixed = 48 + 32 + 1 + 24 + 16 + 16 * nused + 3 + 15;
-  int sb_ex = wave_incl_sum(my_bits) - my_bits;
-  int sel_total = wave_sum(my_bits);
-  int tb_before = 0, tb_total = 0;
-  for (int q = 0; q < ng; q++) {
-    if (q < lane) tb_before += tbits[q];
-    tb_total += tbits[q];
-  }
+  int sel_total;
+  const int sb_ex = wg_excl_sum<256, int>(my_bits, tmp, &sel_total);
+  int tb_total = 0;
+  for (int q = 0; q < ng; q++) tb_total += tbits[q];
   uint32_t* hw = d.hdr + (int64_t)b * d.hdr_stride;
   const int64_t hdr_bits = fixed + sel_total + tb_total;
-  // fixed part (lane 0)
-  if (lane == 0) {
+  // fixed part
+  if (tid == 0) {
     BitOut o;
     o.init(hw, 0, fixed, false);
     o.put(0x314159u, 24); o.put(0x265359u, 24);
@@ -420,28 +442,28 @@ __global__ void __launch_bounds__(64) k_huff_codes(Dev d) {
     o.finish();
     st->hdr_bits = hdr_bits;
   }
-  // selectors: lane's chunk owns [fixed + sb_ex, + my_bits)
+  // selectors: this thread's chunk owns [fixed + sb_ex, + my_bits)
   {
     BitOut o;
-    int64_t a = fixed + sb_ex;
+    const int64_t a = fixed + sb_ex;
     o.init(hw, a, a + my_bits, false);
     for (int i = c0; i < c1; i++) {
-      int j = smtf[i];
-      // j ones then a zero
-      o.put(((1u << j) - 1) << 1, j + 1);
+      const int j = smtf[i];
+      o.put(((1u << j) - 1) << 1, j + 1);  // j ones then a zero
     }
     o.finish();
   }
-  // coding tables: lane q < ng owns its table's range
-  if (lane < ng) {
-    const uint8_t* len = d.len + ((int64_t)b * 6 + lane) * 260;
+  // coding tables: thread q < ng owns its table's range
+  if (tid < ng) {
+    int tb_before = 0;
+    for (int q = 0; q < tid; q++) tb_before += tbits[q];
     BitOut o;
-    int64_t a = fixed + sel_total + tb_before;
-    o.init(hw, a, a + tbits[lane], false);
-    int cur = len[0];
+    const int64_t a = fixed + sel_total + tb_before;
+    o.init(hw, a, a + tbits[tid], false);
+    int cur = L[tid][0];
     o.put((uint32_t)cur, 5);
     for (int v = 0; v < alpha; v++) {
-      int tg = len[v];
+      const int tg = L[tid][v];
       while (cur < tg) { o.put(2, 2); cur++; }
       while (cur > tg) { o.put(3, 2); cur--; }
       o.put(0, 1);
@@ -620,14 +642,19 @@ __device__ inline void place_bits(uint32_t* w0, const uint32_t* src, int64_t dst
   }
 }
 
+// One workgroup per capacity tile; the tiles of a block share its payload
+// words round-robin (256 consecutive words per workgroup and step).
 __global__ void __launch_bounds__(256) k_place(Dev d) {
-  const int b = blockIdx.x;
+  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
+  const int b = d.tc_blk[t];
   const BlockDesc D = d.desc[b];
+  const int k = t - D.tile_c0;
+  const int ntk = tiles_for(D.cap);
   const BlockState* st = d.st + b;
   const StreamState S = d.ss[D.stream];
   uint32_t* w0 = reinterpret_cast<uint32_t*>(d.out) + S.byte_off / 4;
-  place_bits(w0, d.hdr + (int64_t)b * d.hdr_stride, st->bit_off, st->hdr_bits, threadIdx.x, 256);
-  place_bits(w0, d.pay + D.w_off, st->bit_off + st->hdr_bits, st->bits, threadIdx.x, 256);
+  if (k == 0) place_bits(w0, d.hdr + (int64_t)b * d.hdr_stride, st->bit_off, st->hdr_bits, threadIdx.x, 256);
+  place_bits(w0, d.pay + D.w_off, st->bit_off + st->hdr_bits, st->bits, k * 256 + threadIdx.x, ntk * 256);
 }
 
 __global__ void __launch_bounds__(64) k_trailer(Dev d) {
@@ -649,7 +676,7 @@ void launch_huffman(const Dev& d, hipStream_t s) {
     k_huff_lengths<<<d.nblk * 6, 64, 0, s>>>(d);
     probe_end("k_huff_lengths", s);
   }
-  k_huff_codes<<<d.nblk, 64, 0, s>>>(d);
+  k_huff_codes<<<d.nblk, 256, 0, s>>>(d);
 }
 
 void launch_pack(const Dev& d, hipStream_t s) {
@@ -663,7 +690,7 @@ void launch_pack(const Dev& d, hipStream_t s) {
 void launch_assemble(const Dev& d, hipStream_t s) {
   if (d.nstream == 0) return;
   k_layout<<<1, 256, 0, s>>>(d);
-  if (d.nblk > 0) k_place<<<d.nblk, 256, 0, s>>>(d);
+  if (d.nblk > 0) k_place<<<d.ntile_c, 256, 0, s>>>(d);
   k_trailer<<<(d.nstream + 63) / 64, 64, 0, s>>>(d);
 }
 
