@@ -478,20 +478,30 @@ __global__ void __launch_bounds__(256) k_sort_small64(Dev d, int cur, int keybit
 }
 
 // ----------------------------------------------------------- large groups
+// Digit histogram of a large-group tile (per-wave counters, so that skewed
+// digits collide only within a wave).
 __global__ void __launch_bounds__(256) k_lhist(Dev d, int cur, int shift, int src) {
-  __shared__ uint32_t h[256];
+  __shared__ uint32_t h[4][256];
+  const int tid = threadIdx.x, wv = tid >> 6;
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const Seg g = d.seg[cur][d.ltile_seg[t]];
   const int pos = d.ltile_pos[t];
   const uint64_t off = d.desc[g.blk].off;
-  const uint32_t* K = src ? d.KEY2 : d.KEY;
-  h[threadIdx.x] = 0;
+  const uint32_t* K = (src ? d.KEY2 : d.KEY) + off + g.start + pos;
+  h[0][tid] = 0; h[1][tid] = 0; h[2][tid] = 0; h[3][tid] = 0;
   __syncthreads();
   const int n = min(kTile, g.len - pos);
-  for (int e = threadIdx.x; e < n; e += 256)
-    atomicAdd(&h[(K[off + g.start + pos + e] >> shift) & 255u], 1u);
+  uint32_t k[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int e = tid + 256 * q;
+    k[q] = e < n ? K[e] : 0u;
+  }
+#pragma unroll
+  for (int q = 0; q < 16; q++)
+    if (tid + 256 * q < n) atomicAdd(&h[wv][(k[q] >> shift) & 255u], 1u);
   __syncthreads();
-  d.hist[(int64_t)t * 256 + threadIdx.x] = h[threadIdx.x];
+  d.hist[(int64_t)t * 256 + tid] = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
 }
 
 // per large group: hist[t][digit] -> absolute destination base
@@ -501,29 +511,35 @@ __global__ void __launch_bounds__(256) k_lscan(Dev d, int cur) {
   const Seg g = d.seg[cur][si];
   const int t0 = d.pb[si], nt = tiles_for(g.len);
   const int dg = threadIdx.x;
+  uint32_t* hp = d.hist + (int64_t)t0 * 256 + dg;
   uint32_t run = 0;
-  for (int t = 0; t < nt; t++) {
-    uint32_t* p = &d.hist[(int64_t)(t0 + t) * 256 + dg];
-    uint32_t v = *p;
-    *p = run;
-    run += v;
+  for (int t = 0; t < nt; t += 8) {
+    uint32_t v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) v[q] = t + q < nt ? hp[(int64_t)(t + q) * 256] : 0u;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      if (t + q < nt) hp[(int64_t)(t + q) * 256] = run;
+      run += v[q];
+    }
   }
   int tot;
-  int base = wg_excl_sum<256, int>((int)run, tmp, &tot);
-  uint32_t add = (uint32_t)(g.start + base);
-  for (int t = 0; t < nt; t++) d.hist[(int64_t)(t0 + t) * 256 + dg] += add;
+  const int base = wg_excl_sum<256, int>((int)run, tmp, &tot);
+  const uint32_t add = (uint32_t)(g.start + base);
+  for (int t = 0; t < nt; t++) hp[(int64_t)t * 256] += add;
 }
 
 // One LSD pass over a large-group tile: stable rank of every row by digit,
 // rows placed in LDS in tile-sorted order, then written out so that rows of
-// one digit go to consecutive global addresses (coalesced runs).
-__global__ void __launch_bounds__(256) k_lscatter(Dev d, int cur, int shift, int src) {
-  __shared__ uint32_t cnt[4][256];
+// one digit go to consecutive global addresses (coalesced runs).  512
+// threads, 8 rows each.
+__global__ void __launch_bounds__(512) k_lscatter(Dev d, int cur, int shift, int src) {
+  __shared__ uint32_t cnt[8][256];
   __shared__ uint32_t lstart[256];   // tile-local start of each digit
   __shared__ uint32_t gstart[256];   // global destination of each digit's first row
   __shared__ uint32_t skey[kTile];
   __shared__ uint32_t sval[kTile];
-  __shared__ int32_t tmp[8];
+  __shared__ int32_t tmp[16];
   const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const Seg g = d.seg[cur][d.ltile_seg[t]];
@@ -535,44 +551,53 @@ __global__ void __launch_bounds__(256) k_lscatter(Dev d, int cur, int shift, int
   uint32_t* Sd = src ? d.SA : d.SA2;
   const int n = min(kTile, g.len - pos);
   const uint64_t base = off + g.start + pos;
-  cnt[0][tid] = 0; cnt[1][tid] = 0; cnt[2][tid] = 0; cnt[3][tid] = 0;
+  for (int q = tid; q < 8 * 256; q += 512) (&cnt[0][0])[q] = 0;
   __syncthreads();
-  uint32_t key[16], val[16], rk[16];
+  uint32_t key[8], val[8], rk[8];
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
-    int e = wv * 1024 + q * 64 + lane;
-    bool valid = e < n;
-    key[q] = valid ? Ks[base + e] : 0u;
-    val[q] = valid ? Ss[base + e] : 0u;
-    rk[q] = wave_rank_digit((key[q] >> shift) & 255u, valid, cnt[wv]);
+  for (int q = 0; q < 8; q++) {
+    const int e = wv * 512 + q * 64 + lane;
+    key[q] = e < n ? Ks[base + e] : 0u;
+    val[q] = e < n ? Ss[base + e] : 0u;
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int e = wv * 512 + q * 64 + lane;
+    rk[q] = wave_rank_digit((key[q] >> shift) & 255u, e < n, cnt[wv]);
   }
   __syncthreads();
   {
-    uint32_t c0 = cnt[0][tid], c1 = cnt[1][tid], c2 = cnt[2][tid], c3 = cnt[3][tid];
+    uint32_t c[8];
+    int sum = 0;
+    if (tid < 256) {
+#pragma unroll
+      for (int w = 0; w < 8; w++) { c[w] = cnt[w][tid]; sum += (int)c[w]; }
+    }
     int tot;
-    int ls = wg_excl_sum<256, int>((int)(c0 + c1 + c2 + c3), tmp, &tot);
-    lstart[tid] = (uint32_t)ls;
-    gstart[tid] = d.hist[(int64_t)t * 256 + tid];
-    cnt[0][tid] = ls;
-    cnt[1][tid] = ls + c0;
-    cnt[2][tid] = ls + c0 + c1;
-    cnt[3][tid] = ls + c0 + c1 + c2;
+    const int ls = wg_excl_sum<512, int>(sum, tmp, &tot);
+    if (tid < 256) {
+      lstart[tid] = (uint32_t)ls;
+      gstart[tid] = d.hist[(int64_t)t * 256 + tid];
+      uint32_t r = (uint32_t)ls;
+#pragma unroll
+      for (int w = 0; w < 8; w++) { cnt[w][tid] = r; r += c[w]; }
+    }
   }
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
-    int e = wv * 1024 + q * 64 + lane;
+  for (int q = 0; q < 8; q++) {
+    const int e = wv * 512 + q * 64 + lane;
     if (e < n) {
-      uint32_t p = cnt[wv][(key[q] >> shift) & 255u] + rk[q];
+      const uint32_t p = cnt[wv][(key[q] >> shift) & 255u] + rk[q];
       skey[p] = key[q];
       sval[p] = val[q];
     }
   }
   __syncthreads();
-  for (int p = tid; p < n; p += 256) {
-    uint32_t k = skey[p];
-    uint32_t dg = (k >> shift) & 255u;
-    uint32_t dst = gstart[dg] + ((uint32_t)p - lstart[dg]);
+  for (int p = tid; p < n; p += 512) {
+    const uint32_t k = skey[p];
+    const uint32_t dg = (k >> shift) & 255u;
+    const uint32_t dst = gstart[dg] + ((uint32_t)p - lstart[dg]);
     Kd[off + dst] = k;
     Sd[off + dst] = sval[p];
   }
@@ -813,7 +838,7 @@ void launch_bwt_round(const Dev& d, int cur, int round, int32_t P, int key_bits,
       k_lhist<<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
       k_lscan<<<nlarge, 256, 0, s>>>(d, cur);
       probe_begin("k_lscatter", s);
-      k_lscatter<<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
+      k_lscatter<<<ntiles, 512, 0, s>>>(d, cur, 8 * p, p & 1);
       probe_end("k_lscatter", s);
     }
     const int fin = npass & 1;
