@@ -60,10 +60,33 @@ __global__ void __launch_bounds__(256) k_bwt_init(Dev d) {
   const BlockDesc D = d.desc[b];
   const int m = d.st[b].m;
   const int k = t - D.tile_c0;
-  // SA entries carry the row's BWT byte T[(i-1) mod m] in their top 8 bits
-  for (int e = threadIdx.x; e < kTile; e += 256) {
-    int i = k * kTile + e;
-    if (i < m) d.SA[D.off + i] = (uint32_t)i | ((uint32_t)d.T[D.off + (i == 0 ? m - 1 : i - 1)] << 24);
+  // SA entries carry the row's BWT byte T[(i-1) mod m] in their top 8 bits;
+  // thread handles 16 consecutive rows
+  const int i0 = k * kTile + threadIdx.x * 16;
+  if (i0 < m) {
+    const uint8_t* T = d.T + D.off;
+    uint32_t* SA = d.SA + D.off;
+    uint32_t prev = T[i0 == 0 ? m - 1 : i0 - 1];
+    if (i0 + 16 <= m) {
+      const uint4 v = *reinterpret_cast<const uint4*>(T + i0);
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int q4 = 0; q4 < 4; q4++) {
+        uint32_t o[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int q = q4 * 4 + r;
+          o[r] = (uint32_t)(i0 + q) | (prev << 24);
+          prev = (w[q >> 2] >> (8 * (q & 3))) & 255u;
+        }
+        *reinterpret_cast<uint4*>(SA + i0 + q4 * 4) = make_uint4(o[0], o[1], o[2], o[3]);
+      }
+    } else {
+      for (int i = i0; i < m; i++) {
+        SA[i] = (uint32_t)i | (prev << 24);
+        prev = T[i];
+      }
+    }
   }
   if (k == 0 && threadIdx.x == 0) {
     if (m > 1) {

@@ -131,34 +131,39 @@ __global__ void __launch_bounds__(256) k_front1(Dev d) {
   const int c = load16(src, len, tid, x);
   lastb[tid] = c > 0 ? x[c - 1] : 0;
   __syncthreads();
+  // (fixed-count loops over x[] keep the register indexing static)
   uint32_t reg = 0;
-  for (int q = 0; q < c; q++) reg = (reg << 8) ^ tab[(reg >> 24) ^ x[q]];
+#pragma unroll
+  for (int q = 0; q < 16; q++)
+    if (q < c) reg = (reg << 8) ^ tab[(reg >> 24) ^ x[q]];
   if (c > 0) reg = crc_shift(reg, (uint32_t)(len - tid * 16 - c));
   // run starts
   const int p0 = start + tid * 16;
   uint32_t bmask = 0;
-  int fb = 0x7fffffff, lb = -1;
-  if (c > 0) {
-    uint8_t prev = 0;
+  {
+    uint32_t prev = 0;
     bool has_prev = false;
     if (tid > 0) { prev = lastb[tid - 1]; has_prev = true; }
     else if (start > 0) { prev = src[-1]; has_prev = true; }
-    for (int q = 0; q < c; q++) {
-      bool bnd = (q == 0) ? (!has_prev || x[0] != prev) : (x[q] != x[q - 1]);
-      if (bnd) { bmask |= 1u << q; lb = p0 + q; if (fb == 0x7fffffff) fb = p0 + q; }
-    }
+    if (c > 0 && (!has_prev || x[0] != prev)) bmask = 1u;
+#pragma unroll
+    for (int q = 1; q < 16; q++)
+      if (q < c && x[q] != x[q - 1]) bmask |= 1u << q;
   }
+  int fb = bmask ? p0 + __builtin_ctz(bmask) : 0x7fffffff;
+  int lb = bmask ? p0 + 31 - __builtin_clz(bmask) : -1;
   xs[tid] = wg_incl_max<256, int>(lb, tmp, -1);
   __syncthreads();
-  int run = tid > 0 ? xs[tid - 1] : -1;  // -1: run started before the tile
+  // offset r = (i - run start) mod 259, carried from position to position;
+  // -1 while the run start lies before the tile
+  const int run = tid > 0 ? xs[tid - 1] : -1;
+  int r = run >= 0 ? (p0 - run) % 259 : -1;
   int cnt = 0;
-  for (int q = 0; q < c; q++) {
-    if (bmask & (1u << q)) run = p0 + q;
-    if (run >= 0) {
-      int j = p0 + q - run;
-      int r = j - 259 * (j / 259);
-      cnt += r < 3 ? 1 : (r == 3 ? 2 : 0);
-    }
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    if (q > 0 && r >= 0) r = r == 258 ? 0 : r + 1;
+    if (bmask & (1u << q)) r = 0;
+    if (q < c && r >= 0) cnt += r < 3 ? 1 : (r == 3 ? 2 : 0);
   }
   // reductions: xor of registers, sum of counts, min/max run starts
   for (int o = 32; o >= 1; o >>= 1) {
@@ -280,36 +285,34 @@ __global__ void __launch_bounds__(256) k_rle1_write(Dev d) {
   lastb[tid] = c > 0 ? x[c - 1] : 0;
   __syncthreads();
   const int p0 = start + tid * 16;
-  uint8_t prev = 0;
+  uint32_t prev = 0;
   bool has_prev = false;
   if (tid > 0) { prev = lastb[tid - 1]; has_prev = true; }
   else if (start > 0) { prev = src[-1]; has_prev = true; }
   uint32_t bmask = 0;
-  int lastB = -1, firstB = 0x7fffffff;
-  for (int q = 0; q < c; q++) {
-    bool bnd = (q == 0) ? (!has_prev || x[0] != prev) : (x[q] != x[q - 1]);
-    if (bnd) {
-      bmask |= 1u << q;
-      lastB = p0 + q;
-      if (firstB == 0x7fffffff) firstB = p0 + q;
-    }
-  }
+  if (c > 0 && (!has_prev || x[0] != prev)) bmask = 1u;
+#pragma unroll
+  for (int q = 1; q < 16; q++)
+    if (q < c && x[q] != x[q - 1]) bmask |= 1u << q;
+  const int lastB = bmask ? p0 + 31 - __builtin_clz(bmask) : -1;
+  const int firstB = bmask ? p0 + __builtin_ctz(bmask) : 0x7fffffff;
   xa[tid] = wg_incl_max<256, int>(lastB, tmp, ca);
   xe[tid] = wg_rev_incl_min<256, int>(firstB, tmp, ce);
   __syncthreads();
   int a = tid > 0 ? xa[tid - 1] : ca;
   int cnt = 0;
   int8_t rr[16];
+  {
+    // offset in the 259-piece, carried from position to position
+    int r = c > 0 && a >= 0 ? (p0 - a) % 259 : 0;
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
-    if (bmask & (1u << q)) a = p0 + q;
-    int r = 4;
-    if (q < c) {
-      int j = p0 + q - a;
-      r = j - 259 * (j / 259);
+    for (int q = 0; q < 16; q++) {
+      if (q > 0) r = r == 258 ? 0 : r + 1;
+      if (bmask & (1u << q)) r = 0;
+      const int rq = q < c ? r : 4;
+      rr[q] = (int8_t)(rq < 4 ? rq : 4);
+      cnt += rq < 3 ? 1 : (rq == 3 ? 2 : 0);
     }
-    rr[q] = (int8_t)(r < 4 ? r : 4);
-    cnt += r < 3 ? 1 : (r == 3 ? 2 : 0);
   }
   int total;
   int o = wg_excl_sum<256, int>(cnt, tmp, &total);
@@ -330,8 +333,10 @@ __global__ void __launch_bounds__(256) k_rle1_write(Dev d) {
         ob[o++] = x[q];
         seen[x[q]] = 1;
         if (rr[q] == 3) {
-          int runlen = (int)ends[q] - a, j = p0 + q - a;
-          uint8_t cb = (j / 259 < runlen / 259) ? 255 : (uint8_t)(runlen - 259 * (runlen / 259) - 4);
+          // j = 259*piece + 3: a full piece when 259*(piece+1) <= run length,
+          // else the last piece, whose count is run length - (j - 3) - 4
+          const int runlen = (int)ends[q] - a, j = p0 + q - a;
+          const uint8_t cb = j + 256 <= runlen ? 255 : (uint8_t)(runlen - j - 1);
           ob[o++] = cb;
           seen[cb] = 1;
         }

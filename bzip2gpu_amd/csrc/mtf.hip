@@ -263,8 +263,9 @@ __device__ inline void rle2_tile(const Dev& d) {
   __shared__ int32_t tmp[8];
   __shared__ int32_t carry;
   __shared__ int32_t xe[256];
-  __shared__ int32_t hist[260];
-  const int tid = threadIdx.x;
+  __shared__ int32_t hist[4][260];
+  __shared__ uint16_t ob[WRITE ? 2 * kTile : 1];
+  const int tid = threadIdx.x, wv = tid >> 6;
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int b = d.tc_blk[t];
   const BlockDesc D = d.desc[b];
@@ -276,7 +277,8 @@ __device__ inline void rle2_tile(const Dev& d) {
     return;
   }
   const int nt = tiles_for(m);
-  if (WRITE) for (int q = tid; q < 260; q += 256) hist[q] = 0;
+  if (WRITE)
+    for (int q = tid; q < 4 * 260; q += 256) (&hist[0][0])[q] = 0;
   {
     int f = 0x7fffffff;
     for (int j = k + 1 + tid; j < nt; j += 256) {
@@ -293,27 +295,39 @@ __device__ inline void rle2_tile(const Dev& d) {
     __syncthreads();
   }
   const int n = min(kTile, m - start);
-  const uint8_t* src = d.MTF + D.off + start;
+  const uint8_t* src = d.MTF + D.off + start;  // 256-byte aligned
   const int q0 = tid * 16;
+  const int p0 = start + q0;
   uint8_t x[16];
-  int firstNZ = 0x7fffffff;
+  if (q0 + 16 <= n) {
+    const uint4 v = *reinterpret_cast<const uint4*>(src + q0);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
-    x[q] = (q0 + q < n) ? src[q0 + q] : 1;
+    for (int q = 0; q < 16; q++) x[q] = (uint8_t)(w[q >> 2] >> (8 * (q & 3)));
+  } else {
+#pragma unroll
+    for (int q = 0; q < 16; q++) x[q] = (q0 + q < n) ? src[q0 + q] : 1;
   }
-  for (int q = 15; q >= 0; q--)
-    if (q0 + q < n && x[q] != 0) firstNZ = start + q0 + q;
-  int incl = wg_rev_incl_min<256, int>(firstNZ, tmp, carry);
-  xe[tid] = incl;
+  // nonzero positions (positions past the tile count as nonzero)
+  uint32_t nzm = 0;
+#pragma unroll
+  for (int q = 0; q < 16; q++)
+    if (x[q] != 0) nzm |= 1u << q;
+  const uint32_t valid = q0 >= n ? 0u : (q0 + 16 <= n ? 0xFFFFu : ((1u << (n - q0)) - 1u));
+  const int firstNZ = (nzm & valid) ? p0 + __builtin_ctz(nzm & valid) : 0x7fffffff;
+  xe[tid] = wg_rev_incl_min<256, int>(firstNZ, tmp, carry);
   __syncthreads();
-  int es = tid < 255 ? xe[tid + 1] : carry;
-  uint8_t prevb = q0 > 0 ? src[q0 - 1] : (start > 0 ? src[-1] : 1);
+  const int es = tid < 255 ? xe[tid + 1] : carry;
+  const bool prev_nz = q0 > 0 ? src[q0 - 1] != 0 : (start > 0 ? src[-1] != 0 : true);
+  // symbols per position: nonzero v -> 1 (v+1); the first zero of a run of
+  // length L -> the bits of L+1 below its leading one
   int e_of[16];
   {
     int nb = es;
+#pragma unroll
     for (int q = 15; q >= 0; q--) {
       e_of[q] = nb;
-      if (q0 + q < n && x[q] != 0) nb = start + q0 + q;
+      if ((valid >> q) & (nzm >> q) & 1u) nb = p0 + q;
     }
   }
   int cs[16];
@@ -321,40 +335,44 @@ __device__ inline void rle2_tile(const Dev& d) {
 #pragma unroll
   for (int q = 0; q < 16; q++) {
     int c = 0;
-    if (q0 + q < n) {
-      uint8_t pv = q == 0 ? prevb : x[q - 1];
-      if (x[q] != 0) c = 1;
-      else if (pv != 0 || start + q0 + q == 0) c = ilog2((uint32_t)(e_of[q] - (start + q0 + q)) + 1);
+    if ((valid >> q) & 1u) {
+      const bool pnz = q == 0 ? prev_nz : ((nzm >> (q - 1)) & 1u);
+      if ((nzm >> q) & 1u) c = 1;
+      else if (pnz || p0 + q == 0) c = ilog2((uint32_t)(e_of[q] - (p0 + q)) + 1);
     }
     cs[q] = c;
     cnt += c;
   }
   int total;
-  int excl = wg_excl_sum<256, int>(cnt, tmp, &total);
+  int o = wg_excl_sum<256, int>(cnt, tmp, &total);
   if (!WRITE) {
     if (tid == 0) d.tc_cnt[t] = total;
     return;
   }
-  uint16_t* dst = d.SYM + D.off + d.tc_off[t] + excl;
-  int o = 0;
+  // symbols staged in LDS in order, then copied with coalesced stores
+#pragma unroll
   for (int q = 0; q < 16; q++) {
-    if (!cs[q]) continue;
-    if (x[q] != 0) {
-      uint16_t v = (uint16_t)(x[q] + 1);
-      dst[o++] = v;
-      atomicAdd(&hist[v], 1);
-    } else {
-      uint32_t kk = (uint32_t)(e_of[q] - (start + q0 + q)) + 1;
-      for (int r = 0; r < cs[q]; r++) {
-        uint16_t v = (uint16_t)((kk >> r) & 1u);
-        dst[o++] = v;
-        atomicAdd(&hist[v], 1);
+    if (cs[q]) {
+      if ((nzm >> q) & 1u) {
+        const uint16_t v = (uint16_t)(x[q] + 1);
+        ob[o++] = v;
+        atomicAdd(&hist[wv][v], 1);
+      } else {
+        const uint32_t kk = (uint32_t)(e_of[q] - (p0 + q)) + 1;
+        const int n1 = __popc(kk & ((1u << cs[q]) - 1u));
+        for (int r = 0; r < cs[q]; r++) ob[o++] = (uint16_t)((kk >> r) & 1u);
+        atomicAdd(&hist[wv][1], n1);
+        atomicAdd(&hist[wv][0], cs[q] - n1);
       }
     }
   }
   __syncthreads();
-  for (int q = tid; q < 260; q += 256)
-    if (hist[q]) atomicAdd(&d.freq[(int64_t)b * 260 + q], hist[q]);
+  uint16_t* dst = d.SYM + D.off + d.tc_off[t];
+  for (int i = tid; i < total; i += 256) dst[i] = ob[i];
+  for (int q = tid; q < 260; q += 256) {
+    const int h = hist[0][q] + hist[1][q] + hist[2][q] + hist[3][q];
+    if (h) atomicAdd(&d.freq[(int64_t)b * 260 + q], h);
+  }
 }
 
 __global__ void __launch_bounds__(256) k_rle2_count(Dev d) { rle2_tile<false>(d); }
