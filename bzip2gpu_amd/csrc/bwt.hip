@@ -61,29 +61,45 @@ __global__ void __launch_bounds__(256) k_bwt_init(Dev d) {
   const int m = d.st[b].m;
   const int k = t - D.tile_c0;
   // SA entries carry the row's BWT byte T[(i-1) mod m] in their top 8 bits;
-  // thread handles 16 consecutive rows
+  // KEY gets the round-0 key, the rotation's first 4 bytes.  A thread
+  // handles 16 consecutive rows.
   const int i0 = k * kTile + threadIdx.x * 16;
   if (i0 < m) {
     const uint8_t* T = d.T + D.off;
     uint32_t* SA = d.SA + D.off;
+    uint32_t* KEY = d.KEY + D.off;
     uint32_t prev = T[i0 == 0 ? m - 1 : i0 - 1];
-    if (i0 + 16 <= m) {
+    if (i0 + 19 <= m) {
       const uint4 v = *reinterpret_cast<const uint4*>(T + i0);
       const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+      uint32_t by[19];
+#pragma unroll
+      for (int q = 0; q < 16; q++) by[q] = (w[q >> 2] >> (8 * (q & 3))) & 255u;
+#pragma unroll
+      for (int q = 16; q < 19; q++) by[q] = T[i0 + q];
 #pragma unroll
       for (int q4 = 0; q4 < 4; q4++) {
-        uint32_t o[4];
+        uint32_t o[4], kk[4];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           const int q = q4 * 4 + r;
           o[r] = (uint32_t)(i0 + q) | (prev << 24);
-          prev = (w[q >> 2] >> (8 * (q & 3))) & 255u;
+          kk[r] = (by[q] << 24) | (by[q + 1] << 16) | (by[q + 2] << 8) | by[q + 3];
+          prev = by[q];
         }
         *reinterpret_cast<uint4*>(SA + i0 + q4 * 4) = make_uint4(o[0], o[1], o[2], o[3]);
+        *reinterpret_cast<uint4*>(KEY + i0 + q4 * 4) = make_uint4(kk[0], kk[1], kk[2], kk[3]);
       }
     } else {
-      for (int i = i0; i < m; i++) {
+      for (int i = i0; i < min(i0 + 16, m); i++) {
         SA[i] = (uint32_t)i | (prev << 24);
+        uint32_t kk = 0;
+        int p = i;
+        for (int q = 0; q < 4; q++) {
+          kk = (kk << 8) | T[p];
+          p = p + 1 == m ? 0 : p + 1;
+        }
+        KEY[i] = kk;
         prev = T[i];
       }
     }
@@ -92,6 +108,7 @@ __global__ void __launch_bounds__(256) k_bwt_init(Dev d) {
     if (m > 1) {
       int idx = atomicAdd(&d.ctr[CTR_LIST0], 1);
       d.seg[0][idx] = Seg{b, 0, m, 0};
+      d.st[b].rounds = 1;  // round 0 touches every block with m > 1
     } else {
       d.RK[D.off] = 0;
     }
@@ -843,7 +860,7 @@ void launch_bwt_round(const Dev& d, int cur, int round, int32_t P, int key_bits,
                       hipStream_t s) {
   k_mk_packs<<<(npacks + 1 + 255) / 256, 256, 0, s>>>(d, npacks, nseg);
   k_mk_large<<<(nseg + 255) / 256, 256, 0, s>>>(d, cur, nseg);
-  if (npacks + ntiles > 0) {
+  if (round > 0 && npacks + ntiles > 0) {  // round-0 keys come from k_bwt_init
     probe_begin("k_keys", s);
     k_keys<<<npacks + ntiles, 256, 0, s>>>(d, cur, round, P, npacks);
     probe_end("k_keys", s);
