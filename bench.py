@@ -41,7 +41,7 @@ def _args():
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-check", action="store_true")
-    p.add_argument("--probe", default="k_keys", help="kernel timed live with HIP events for the roofline")
+    p.add_argument("--probe", default="k_lscatter", help="kernel timed live with HIP events for the roofline")
     p.add_argument("--calib-copy", action="store_true",
                    help="also run one plain device copy of the input (PMC traffic calibration)")
     p.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
@@ -52,8 +52,9 @@ def _args():
 # Algorithmic HBM bytes of one unit of work of each probed kernel, and the
 # name of the unit (per-unit figures restated in DESIGN.md).
 ALG_BYTES = {
-    # suffix-sort key gather: read SA 4 + gather the rank at SA+P 4 + write key 4
-    "k_keys": (12, "bwt_rows"),
+    # one LSD radix pass of the large-group suffix sort: read key 4 + SA 4,
+    # write key 4 + SA 4 per row and pass
+    "k_lscatter": (16, "radix_pass_rows"),
     # LDS sort of a pack: read key 4 + SA 4, write SA 4 + scatter rank 4
     "k_sort_small": (16, "bwt_rows_small"),
     # MTF: read last-column byte, write MTF byte
@@ -329,8 +330,10 @@ def main():
     # bytes per row per round (read SA 4 + gather 2 ranks 8 + write SA 4 + rank 4)
     res["bwt_round_stats"] = [{"rows": r, "ms": round(t, 3), "gbps_alg": round(20 * r / (t * 1e-3) / 1e9, 1) if t else None}
                               for r, t in rs]
+    small_rows, large_rows, pass_rows = bz.last_sort_rows()
+    res["sort_rows"] = {"small_groups": small_rows, "large_groups": large_rows, "large_x_passes": pass_rows}
     per_unit, unit = ALG_BYTES.get(a.probe, (0, ""))
-    units = {"bwt_rows": rows, "rle1_bytes": 0}.get(unit, 0)
+    units = {"radix_pass_rows": pass_rows}.get(unit, 0)
     if probe_n and units and per_unit:
         alg = per_unit * units  # per step
         per_launch = alg * a.steps / probe_n
@@ -339,7 +342,7 @@ def main():
         traffic = None
         if a.traffic and os.path.exists(a.traffic):
             with open(a.traffic) as f:
-                traffic = json.load(f).get(a.probe)
+                traffic = json.load(f).get(f"{cfg['workload']}:{a.probe}")
         res["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
                            "frac": round(achieved / 8000.0, 4), "traffic": traffic,
                            "kernel": a.probe, "launches": probe_n, "avg_launch_us": round(probe_ms / probe_n * 1e3, 2),

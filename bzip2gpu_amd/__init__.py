@@ -23,6 +23,7 @@ from ._lib import (  # noqa: F401
     lib,
     library_path,
     last_round_stats,
+    last_sort_rows,
     last_stage_times,
     mtf,
     probe_kernel,

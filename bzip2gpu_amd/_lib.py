@@ -60,6 +60,7 @@ def lib():
         "bzg_server_replay": ([P, P, i32, P, P, P, P, P, P, P, P], i32),
         "bzg_set_max_batch_bytes": ([ctypes.c_uint64], i32),
         "bzg_last_stage_times": ([P, I32P, I64P], i32),
+        "bzg_last_sort_rows": ([I64P, I64P, I64P], i32),
         "bzg_probe_kernel": ([ctypes.c_char_p], i32),
         "bzg_probe_read": ([ctypes.POINTER(ctypes.c_float), I32P], i32),
         "bzg_crc32": ([P, i32, U32P, P], i32),
@@ -177,6 +178,14 @@ def last_stage_times():
     _check(lib().bzg_last_stage_times(ms, ctypes.byref(r), ctypes.byref(rows)), "bzg_last_stage_times")
     keys = ["front", "bwt", "mtf_rle2", "huffman", "pack", "assemble"]
     return dict(zip(keys, list(ms))), int(r.value), int(rows.value)
+
+
+def last_sort_rows():
+    """(small-group rows, large-group rows, large-group rows x radix passes)
+    of the last batched call, summed over suffix-sort rounds."""
+    a, b, c = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+    _check(lib().bzg_last_sort_rows(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "bzg_last_sort_rows")
+    return int(a.value), int(b.value), int(c.value)
 
 
 def probe_kernel(name) -> None:

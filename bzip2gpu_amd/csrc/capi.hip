@@ -254,6 +254,9 @@ bzg_status bzg_compress_batch_device(const uint8_t* d_in, int32_t n_inputs, cons
     sum.front += t.front; sum.bwt += t.bwt; sum.mtf += t.mtf; sum.huff += t.huff;
     sum.pack += t.pack; sum.assemble += t.assemble; sum.rounds = std::max(sum.rounds, t.rounds);
     sum.bwt_elems += t.bwt_elems;
+    sum.small_rows += t.small_rows;
+    sum.large_rows += t.large_rows;
+    sum.large_pass_rows += t.large_pass_rows;
     if ((e = c->eng.stream_layout(lay, s))) return hip_fail(e, "layout");
     const bool single = (i0 == 0 && i1 == n_inputs);
     if (single) {
@@ -406,6 +409,16 @@ bzg_status bzg_last_stage_times(float* h_ms6, int32_t* h_rounds, int64_t* h_bwt_
   }
   if (h_rounds) *h_rounds = c->last.rounds;
   if (h_bwt_rows) *h_bwt_rows = c->last.bwt_elems;
+  return BZG_OK;
+}
+
+bzg_status bzg_last_sort_rows(int64_t* h_small_rows, int64_t* h_large_rows, int64_t* h_large_pass_rows) {
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  if (h_small_rows) *h_small_rows = c->last.small_rows;
+  if (h_large_rows) *h_large_rows = c->last.large_rows;
+  if (h_large_pass_rows) *h_large_pass_rows = c->last.large_pass_rows;
   return BZG_OK;
 }
 

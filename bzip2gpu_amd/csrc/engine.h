@@ -36,6 +36,8 @@ struct StageTimes {
   float front = 0, bwt = 0, mtf = 0, huff = 0, pack = 0, assemble = 0;
   int rounds = 0;
   int64_t bwt_elems = 0;  // rows keyed and sorted, summed over rounds
+  int64_t small_rows = 0, large_rows = 0;  // of which in small / large groups
+  int64_t large_pass_rows = 0;             // large-group rows x radix passes
 };
 
 class Engine;
@@ -106,6 +108,7 @@ class Engine {
   int64_t total_cap_ = 0, sel_tot_ = 0, w_tot_ = 0, out_cap_ = 0;
   int32_t max_cap_ = 0;
   int64_t active_elems_ = 0;
+  int64_t small_rows_ = 0, large_rows_ = 0, large_pass_rows_ = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> rev_;
   std::vector<int64_t> round_rows_;
   std::vector<float> round_ms_;

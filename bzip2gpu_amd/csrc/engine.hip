@@ -301,6 +301,7 @@ int Engine::run_front(hipStream_t s) {
 
 int Engine::run_bwt(hipStream_t s, int* rounds_out) {
   active_elems_ = 0;
+  small_rows_ = large_rows_ = large_pass_rows_ = 0;
   round_rows_.clear();
   launch_bwt_init(d_, s);
   BZG_TRY(hipGetLastError());
@@ -322,6 +323,9 @@ int Engine::run_bwt(hipStream_t s, int* rounds_out) {
     const int32_t nlarge = h_ctr_[CTR_NLARGE];
     const int keybits = round == 0 ? 32 : keybits_late;
     active_elems_ += (int64_t)small + (int64_t)h_ctr_[CTR_LELEMS];
+    small_rows_ += small;
+    large_rows_ += h_ctr_[CTR_LELEMS];
+    large_pass_rows_ += (int64_t)h_ctr_[CTR_LELEMS] * ((keybits + 7) / 8);
     trace(s, "bwt round start", round, nseg, ntiles);
     if (rev_.size() < (size_t)round + 1) {
       hipEvent_t a = nullptr, b = nullptr;
@@ -410,6 +414,9 @@ int Engine::run(hipStream_t s, bool assemble, StageTimes* times) {
     for (int i = 0; i < 6; i++) BZG_TRY(hipEventElapsedTime(f[i], ev_[i], ev_[i + 1]));
     times->rounds = rounds;
     times->bwt_elems = active_elems_;
+    times->small_rows = small_rows_;
+    times->large_rows = large_rows_;
+    times->large_pass_rows = large_pass_rows_;
   }
   return 0;
 }

@@ -97,6 +97,13 @@ bzg_status bzg_set_max_batch_bytes(uint64_t bytes);
 bzg_status bzg_last_stage_times(float *h_ms6, int32_t *h_rounds,
                                 int64_t *h_bwt_rows);
 
+/* Suffix-sort rows of the last batched call, summed over rounds: rows in
+ * small groups (LDS sorts), rows in large groups (tile radix sort), and
+ * large-group rows times the radix passes they went through (benchmarking
+ * aid, no counterpart in the reference). */
+bzg_status bzg_last_sort_rows(int64_t *h_small_rows, int64_t *h_large_rows,
+                              int64_t *h_large_pass_rows);
+
 /* Rows sorted and GPU milliseconds of every prefix-doubling round of the
  * last pipeline run (its last sub-batch); returns the number of rounds and
  * fills at most max_rounds entries (either array may be NULL). */
