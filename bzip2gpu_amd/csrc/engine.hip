@@ -231,7 +231,6 @@ int Engine::setup(const uint8_t* d_in, hipStream_t s) {
     d.MTF = c.take<uint8_t>(CAP + 64);
     d.SYM = c.take<uint16_t>(CAP + 64);
     d.freq = c.take<int32_t>((B + 1) * 260);
-    d.rfreq = c.take<int32_t>((B + 1) * 6 * 260);
     d.len = c.take<uint8_t>((B + 1) * 6 * 260);
     d.code = c.take<uint32_t>((B + 1) * 6 * 260);
     d.sel = c.take<uint8_t>(sel_tot_ + 64);

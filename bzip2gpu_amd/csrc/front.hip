@@ -88,7 +88,6 @@ __global__ void __launch_bounds__(256) k_block_init(Dev d) {
   }
   if (threadIdx.x < 8) st->present[threadIdx.x] = 0;
   for (int i = threadIdx.x; i < 260; i += blockDim.x) d.freq[(int64_t)b * 260 + i] = 0;
-  for (int i = threadIdx.x; i < 6 * 260; i += blockDim.x) d.rfreq[(int64_t)b * 6 * 260 + i] = 0;
   if (b == 0 && threadIdx.x < CTR_N) d.ctr[threadIdx.x] = 0;
 }
 

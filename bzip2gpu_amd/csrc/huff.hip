@@ -91,66 +91,6 @@ __global__ void __launch_bounds__(64) k_huff_init(Dev d) {
   }
 }
 
-// One thread per 50-symbol group: costs, first strict minimum, counts.  A
-// workgroup covers the groups of kSelTiles capacity tiles; the six code
-// lengths of a symbol are packed in one 64-bit word (10 bits per table, a
-// group's cost is at most 50 * 17 < 1024), so a group's six costs are one
-// 64-bit sum.
-constexpr int kSelTiles = 16;
-__global__ void __launch_bounds__(256) k_huff_select(Dev d) {
-  __shared__ uint64_t PL[260];
-  __shared__ int32_t rf[6][260];
-  __shared__ uint16_t S[256 * kGSize];
-  const int tid = threadIdx.x;
-  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
-  const int b = d.tc_blk[t];
-  const BlockDesc D = d.desc[b];
-  const int k = t - D.tile_c0;
-  if (k % kSelTiles) return;
-  const int s = d.st[b].s;
-  const int nsel = (s + kGSize - 1) / kGSize;
-  const int g0 = (k * kTile + kGSize - 1) / kGSize;
-  const int g1 = min(((k + kSelTiles) * kTile + kGSize - 1) / kGSize, nsel);
-  if (g0 >= g1) return;
-  const int ng = d.st[b].n_groups;
-  const uint8_t* len = d.len + (int64_t)b * 6 * 260;
-  for (int v = tid; v < 260; v += 256) {
-    uint64_t w = 0;
-#pragma unroll
-    for (int tt = 0; tt < 6; tt++) w |= (uint64_t)len[tt * 260 + v] << (10 * tt);
-    PL[v] = w;
-  }
-  for (int q = tid; q < 6 * 260; q += 256) (&rf[0][0])[q] = 0;
-  __syncthreads();
-  for (int gb = g0; gb < g1; gb += 256) {
-    const int gend = min(gb + 256, g1);
-    const int sbase = gb * kGSize, send = min(gend * kGSize, s);
-    for (int i = sbase + tid; i < send; i += 256) S[i - sbase] = d.SYM[D.off + i];
-    __syncthreads();
-    const int g = gb + tid;
-    if (g < gend) {
-      const int gs = g * kGSize, ge = min(gs + kGSize, s);
-      const uint16_t* sym = S - sbase;
-      uint64_t acc = 0;
-      for (int i = gs; i < ge; i++) acc += PL[sym[i]];
-      uint32_t bc = (uint32_t)(acc & 1023u);
-      int bt = 0;
-      for (int tt = 1; tt < ng; tt++) {
-        const uint32_t c = (uint32_t)(acc >> (10 * tt)) & 1023u;
-        if (c < bc) { bc = c; bt = tt; }
-      }
-      d.sel[D.sel_off + g] = (uint8_t)bt;
-      for (int i = gs; i < ge; i++) atomicAdd(&rf[bt][sym[i]], 1);
-    }
-    __syncthreads();
-  }
-  int32_t* grf = d.rfreq + (int64_t)b * 6 * 260;
-  for (int q = tid; q < 6 * 260; q += 256) {
-    int v = (&rf[0][0])[q];
-    if (v) atomicAdd(&grf[q], v);
-  }
-}
-
 // libbz2 code-length method (huffman.cu:51-124): binary min-heap on
 // weight = freq<<8 | depth with the same sift rules, so ties resolve exactly
 // as in the reference.  One wave per (block, table); the heap lives in LDS
@@ -159,6 +99,14 @@ __global__ void __launch_bounds__(256) k_huff_select(Dev d) {
 // "tmp < entry" stop test become ballots, so one LDS round trip covers five
 // levels of a sift.  Depths come from pointer jumping over the parent array.
 namespace {
+
+// Orders one wave's LDS accesses across its lanes (a wave's LDS operations
+// execute in order; this keeps the compiler from moving them).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Sift-down of `tmp` from the root (DOWNHEAP(1)); heap holds nHeap entries.
 // Lane l < 62 covers window level k = log2(l+2), index j = l+2-2^k below zz.
@@ -209,7 +157,7 @@ __device__ inline void heap_down(int2* heap, int nHeap, int2 tmp, int lane) {
     if ((moved >> lane) & 1ull) heap[p >> 1] = e;
     if (done) {
       if (lane == 0) heap[cur] = tmp;
-      __syncthreads();
+      wave_lds_sync();
       return;
     }
     zz = cur;
@@ -226,87 +174,143 @@ __device__ inline void heap_up(int2* heap, int zz, int2 tmp, int lane) {
   const int c = __builtin_ctzll(~M);
   if (lane < c) heap[zz >> lane] = e;
   if (lane == 0) heap[zz >> c] = tmp;
-  __syncthreads();
+  wave_lds_sync();
 }
 
-}  // namespace
+// LDS of one table's code-length build.
+struct HeapLds {
+  __align__(16) int2 heap[264];
+  int16_t anc[2][520];  // pointer-jumping ancestors (node ids <= 2*258-1)
+  int16_t dep[2][520];
+  int32_t leafw[260];
+};
 
-__global__ void __launch_bounds__(64) k_huff_lengths(Dev d) {
-  __shared__ __align__(16) int2 heap[264];
-  __shared__ int32_t anc[2][520];
-  __shared__ int32_t dep[2][520];
-  __shared__ int32_t leafw[260];
-  const int b = blockIdx.x / 6, tt = blockIdx.x - 6 * (blockIdx.x / 6);
-  const int lane = lane_id();
-  if (tt >= d.st[b].n_groups) return;
-  const int alpha = d.st[b].n_in_use + 2;
-  int32_t* fr = d.rfreq + ((int64_t)b * 6 + tt) * 260;
-  uint8_t* len = d.len + ((int64_t)b * 6 + tt) * 260;
+// Code lengths of one table from its symbol counts fr[0..alpha) (LDS), by
+// one wave; len[0..alpha) (LDS) receives them.
+__device__ void build_lengths(HeapLds& H, const int32_t* fr, uint8_t* len, int alpha, int lane) {
   for (int i = lane; i < alpha; i += 64) {
-    int f = fr[i];
-    leafw[i + 1] = (f == 0 ? 1 : f) << 8;
-    fr[i] = 0;  // ready for the next iteration
+    const int f = fr[i];
+    H.leafw[i + 1] = (f == 0 ? 1 : f) << 8;
   }
-  __syncthreads();
+  wave_lds_sync();
   while (true) {
     int nNodes = alpha, nHeap = 0;
     for (int i = 1; i <= alpha; i++) {
       nHeap++;
-      heap_up(heap, nHeap, make_int2(leafw[i], i), lane);
+      heap_up(H.heap, nHeap, make_int2(H.leafw[i], i), lane);
     }
     while (nHeap > 1) {
-      const int2 n1 = heap[1];
-      const int2 t1 = heap[nHeap];
-      __syncthreads();
+      const int2 n1 = H.heap[1];
+      const int2 t1 = H.heap[nHeap];
+      wave_lds_sync();
       nHeap--;
-      heap_down(heap, nHeap, t1, lane);
-      const int2 n2 = heap[1];
-      const int2 t2 = heap[nHeap];
-      __syncthreads();
+      heap_down(H.heap, nHeap, t1, lane);
+      const int2 n2 = H.heap[1];
+      const int2 t2 = H.heap[nHeap];
+      wave_lds_sync();
       nHeap--;
-      heap_down(heap, nHeap, t2, lane);
+      heap_down(H.heap, nHeap, t2, lane);
       nNodes++;
       if (lane == 0) {
-        anc[0][n1.y] = nNodes;
-        anc[0][n2.y] = nNodes;
+        H.anc[0][n1.y] = (int16_t)nNodes;
+        H.anc[0][n2.y] = (int16_t)nNodes;
       }
       const int w1 = n1.x, w2 = n2.x;
       const int d1 = w1 & 0xff, d2 = w2 & 0xff;
       const int2 nn = make_int2(((w1 & 0xffffff00) + (w2 & 0xffffff00)) | (1 + (d1 > d2 ? d1 : d2)), nNodes);
       nHeap++;
-      heap_up(heap, nHeap, nn, lane);
+      heap_up(H.heap, nHeap, nn, lane);
     }
     // depth of every node by pointer jumping (root: ancestor itself, depth 0)
     for (int v = lane + 1; v <= nNodes; v += 64) {
       const bool root = v == nNodes;
-      if (root) anc[0][v] = v;
-      dep[0][v] = root ? 0 : 1;
+      if (root) H.anc[0][v] = (int16_t)v;
+      H.dep[0][v] = root ? 0 : 1;
     }
-    __syncthreads();
+    wave_lds_sync();
     int src = 0;
     for (int it = 0; it < 9; it++) {
       for (int v = lane + 1; v <= nNodes; v += 64) {
-        const int a = anc[src][v];
-        dep[src ^ 1][v] = dep[src][v] + dep[src][a];
-        anc[src ^ 1][v] = anc[src][a];
+        const int a = H.anc[src][v];
+        H.dep[src ^ 1][v] = (int16_t)(H.dep[src][v] + H.dep[src][a]);
+        H.anc[src ^ 1][v] = H.anc[src][a];
       }
-      __syncthreads();
+      wave_lds_sync();
       src ^= 1;
     }
     bool tooLong = false;
     for (int i = lane + 1; i <= alpha; i += 64) {
-      const int j = dep[src][i];
+      const int j = H.dep[src][i];
       len[i - 1] = (uint8_t)j;
       if (j > 17) tooLong = true;
     }
     if (!__ballot(tooLong)) break;
     for (int i = lane + 1; i <= alpha; i += 64) {
-      int j = leafw[i] >> 8;
+      int j = H.leafw[i] >> 8;
       j = 1 + (j / 2);
-      leafw[i] = j << 8;
+      H.leafw[i] = j << 8;
+    }
+    wave_lds_sync();
+  }
+  wave_lds_sync();
+}
+
+}  // namespace
+
+// The four selection / code-length iterations of one block in one
+// workgroup of six waves: all threads assign 50-symbol groups (costs from
+// 64-bit packed lengths, first strict minimum, counts in LDS), then wave t
+// rebuilds table t's lengths.  A block's tables never wait for another
+// block's.
+__global__ void __launch_bounds__(384) k_huff_build(Dev d) {
+  __shared__ uint64_t PL[260];
+  __shared__ int32_t rf[6][260];
+  __shared__ uint8_t L[6][260];
+  __shared__ HeapLds H[6];
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
+  const BlockDesc D = d.desc[b];
+  const BlockState* st = d.st + b;
+  const int ng = st->n_groups, nsel = st->n_sel, s = st->s;
+  const int alpha = st->n_in_use + 2;
+  uint8_t* glen = d.len + (int64_t)b * 6 * 260;
+  for (int q = tid; q < 6 * 260; q += 384) (&L[0][0])[q] = glen[q];
+  __syncthreads();
+  const uint32_t* sym2 = reinterpret_cast<const uint32_t*>(d.SYM + D.off);  // 2 symbols per word
+  for (int it = 0; it < 4; it++) {
+    for (int v = tid; v < 260; v += 384) {
+      uint64_t w = 0;
+#pragma unroll
+      for (int tt = 0; tt < 6; tt++) w |= (uint64_t)L[tt][v] << (10 * tt);
+      PL[v] = w;
+    }
+    for (int q = tid; q < 6 * 260; q += 384) (&rf[0][0])[q] = 0;
+    __syncthreads();
+    for (int g = tid; g < nsel; g += 384) {
+      const int gs = g * kGSize, ge = min(gs + kGSize, s);
+      uint32_t w[kGSize / 2];
+#pragma unroll
+      for (int q = 0; q < kGSize / 2; q++) w[q] = gs + 2 * q < ge ? sym2[(gs >> 1) + q] : 0u;
+      uint64_t acc = 0;
+#pragma unroll
+      for (int q = 0; q < kGSize; q++)
+        if (gs + q < ge) acc += PL[(w[q >> 1] >> (16 * (q & 1))) & 0xFFFFu];
+      uint32_t bc = (uint32_t)(acc & 1023u);
+      int bt = 0;
+      for (int tt = 1; tt < ng; tt++) {
+        const uint32_t c = (uint32_t)(acc >> (10 * tt)) & 1023u;
+        if (c < bc) { bc = c; bt = tt; }
+      }
+      if (it == 3) d.sel[D.sel_off + g] = (uint8_t)bt;
+#pragma unroll
+      for (int q = 0; q < kGSize; q++)
+        if (gs + q < ge) atomicAdd(&rf[bt][(w[q >> 1] >> (16 * (q & 1))) & 0xFFFFu], 1);
     }
     __syncthreads();
+    if (wv < ng) build_lengths(H[wv], rf[wv], L[wv], alpha, lane);
+    __syncthreads();
   }
+  for (int q = tid; q < 6 * 260; q += 384) glen[q] = (&L[0][0])[q];
 }
 
 // Codes (huffman.cu:135-146, :352-362), selector MTF (compress.cu:119-137)
@@ -668,14 +672,9 @@ __global__ void __launch_bounds__(64) k_trailer(Dev d) {
 
 void launch_huffman(const Dev& d, hipStream_t s) {
   k_huff_init<<<d.nblk, 64, 0, s>>>(d);
-  for (int it = 0; it < 4; it++) {
-    probe_begin("k_huff_select", s);
-    k_huff_select<<<d.ntile_c, 256, 0, s>>>(d);
-    probe_end("k_huff_select", s);
-    probe_begin("k_huff_lengths", s);
-    k_huff_lengths<<<d.nblk * 6, 64, 0, s>>>(d);
-    probe_end("k_huff_lengths", s);
-  }
+  probe_begin("k_huff_build", s);
+  k_huff_build<<<d.nblk, 384, 0, s>>>(d);
+  probe_end("k_huff_build", s);
   k_huff_codes<<<d.nblk, 256, 0, s>>>(d);
 }
 

@@ -66,7 +66,6 @@ struct Dev {
   uint16_t* SYM;
   // Huffman
   int32_t* freq;       // [nblk][260]
-  int32_t* rfreq;      // [nblk][6][260]
   uint8_t* len;        // [nblk][6][260]
   uint32_t* code;      // [nblk][6][260]
   uint8_t* sel;        // sel_off-indexed
