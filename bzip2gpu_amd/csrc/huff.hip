@@ -113,7 +113,16 @@ __device__ inline void wave_lds_sync() {
 // An entry lies on the min-child path when it and all its window ancestors
 // are the smaller of their sibling pair; the path stops above the first path
 // entry that tmp is strictly lighter than.
-__device__ inline void heap_down(int2* heap, int nHeap, int2 tmp, int lane) {
+// Window lanes of this lane's ancestors (levels 1..k-1 of its window).
+__device__ inline uint64_t window_ancestors(int lane) {
+  const int k = 31 - __clz(lane + 2);
+  const int j = lane + 2 - (1 << k);
+  uint64_t am = 0;
+  for (int kp = 1; kp < k; kp++) am |= 1ull << ((1 << kp) - 2 + (j >> (k - kp)));
+  return am;
+}
+
+__device__ inline void heap_down(int2* heap, int nHeap, int2 tmp, int lane, uint64_t am) {
   const int k = 31 - __clz(lane + 2);
   const int j = lane + 2 - (1 << k);
   const bool act = lane < 62;
@@ -131,11 +140,7 @@ __device__ inline void heap_down(int2* heap, int nHeap, int2 tmp, int lane) {
     const bool rlight = exr && pr.z < pr.x;
     const bool win = isr ? rlight : (exl && !rlight);
     const uint64_t W = __ballot(win);
-    bool on = win;
-#pragma unroll
-    for (int kp = 1; kp < 5; kp++) {
-      if (kp < k) on = on && ((W >> ((1 << kp) - 2 + (j >> (k - kp)))) & 1ull);
-    }
+    const bool on = win && (W & am) == am;
     const uint64_t P = __ballot(on);
     const uint64_t S = P & __ballot(ex && tmp.x < e.x);
     uint64_t moved;
@@ -188,6 +193,7 @@ struct HeapLds {
 // Code lengths of one table from its symbol counts fr[0..alpha) (LDS), by
 // one wave; len[0..alpha) (LDS) receives them.
 __device__ void build_lengths(HeapLds& H, const int32_t* fr, uint8_t* len, int alpha, int lane) {
+  const uint64_t am = window_ancestors(lane);
   for (int i = lane; i < alpha; i += 64) {
     const int f = fr[i];
     H.leafw[i + 1] = (f == 0 ? 1 : f) << 8;
@@ -204,12 +210,12 @@ __device__ void build_lengths(HeapLds& H, const int32_t* fr, uint8_t* len, int a
       const int2 t1 = H.heap[nHeap];
       wave_lds_sync();
       nHeap--;
-      heap_down(H.heap, nHeap, t1, lane);
+      heap_down(H.heap, nHeap, t1, lane, am);
       const int2 n2 = H.heap[1];
       const int2 t2 = H.heap[nHeap];
       wave_lds_sync();
       nHeap--;
-      heap_down(H.heap, nHeap, t2, lane);
+      heap_down(H.heap, nHeap, t2, lane, am);
       nNodes++;
       if (lane == 0) {
         H.anc[0][n1.y] = (int16_t)nNodes;
