@@ -102,6 +102,7 @@ def run_batch(a):
     def batched():
         return bz.compress_batch_device(d_pool, offs.astype(np.uint64), sizes.astype(np.uint64), levels, 0)
 
+    tune = _autotune(batched)
     for _ in range(max(0, a.warmup)):
         batched()
     lat = []
@@ -128,7 +129,7 @@ def run_batch(a):
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "stage_ms": {k: round(v, 3) for k, v in stages.items()}, "bwt_rounds": rounds,
-        "compressed_bytes": int(sum(by)),
+        "compressed_bytes": int(sum(by)), "autotune": tune,
     }
     if not a.no_plain:
         torch.cuda.synchronize()
@@ -232,6 +233,32 @@ def run_serve(a):
     print(json.dumps(res), flush=True)
 
 
+def _autotune(run_once, max_runs=8):
+    """Untimed set-up: lets the library's autotuner time each launch shape of
+    the radix scatter on this workload until every size class it met has
+    settled (each shape is timed twice per class)."""
+    import torch
+    import bzip2gpu_amd as bz
+
+    def unsettled():
+        seen = [(c, bz.tune_state(0, c)) for c in range(4)]
+        seen = [(c, st) for c, st in seen if any(n for _, _, n in st[1]) or st[0] >= 0]
+        return not seen or any(st[0] < 0 for _, st in seen)
+
+    runs = 0
+    while runs < max_runs and unsettled():
+        run_once()
+        runs += 1
+    torch.cuda.synchronize()
+    out = {"setup_runs": runs}
+    for c in range(4):
+        chosen, rows = bz.tune_state(0, c)
+        if chosen >= 0 or any(n for _, _, n in rows):
+            out[f"class{c}"] = {"radix_scatter_threads": rows[chosen][0] if chosen >= 0 else None,
+                                "ns_per_row_pass": {str(v): round(ns, 4) for v, ns, _ in rows}}
+    return out
+
+
 def _hip_copy_d2h(ptr, nbytes):
     import numpy as np
     hip = ctypes.CDLL("libamdhip64.so")
@@ -280,6 +307,7 @@ def main():
     def batched():
         return bz.compress_batch_device(d_in, [0], [n], [level], bb)
 
+    tune = _autotune(batched)
     for _ in range(max(0, a.warmup)):
         batched()
     torch.cuda.synchronize()
@@ -324,6 +352,7 @@ def main():
         "stage_ms": {k: round(v, 3) for k, v in stages.items()},
         "bwt_rounds": rounds,
         "compressed_bytes": int(by[0]),
+        "autotune": tune,
     }
     rs = bz.last_round_stats()
     # suffix-sort rounds: rows sorted, GPU ms, and GB/s at 20 algorithmic

@@ -32,5 +32,7 @@ from ._lib import (  # noqa: F401
     rle2,
     set_max_batch_bytes,
     symbols_table,
+    tune_force,
+    tune_state,
     version,
 )

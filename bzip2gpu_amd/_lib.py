@@ -61,6 +61,8 @@ def lib():
         "bzg_set_max_batch_bytes": ([ctypes.c_uint64], i32),
         "bzg_last_stage_times": ([P, I32P, I64P], i32),
         "bzg_last_sort_rows": ([I64P, I64P, I64P], i32),
+        "bzg_tune_force": ([i32, i32], i32),
+        "bzg_tune_state": ([i32, i32, P, P, P, i32], i32),
         "bzg_probe_kernel": ([ctypes.c_char_p], i32),
         "bzg_probe_read": ([ctypes.POINTER(ctypes.c_float), I32P], i32),
         "bzg_crc32": ([P, i32, U32P, P], i32),
@@ -186,6 +188,23 @@ def last_sort_rows():
     a, b, c = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
     _check(lib().bzg_last_sort_rows(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "bzg_last_sort_rows")
     return int(a.value), int(b.value), int(c.value)
+
+
+def tune_force(knob: int, variant: int) -> None:
+    """Pin an autotuned launch shape (-1: automatic); see bzg_tune_force."""
+    _check(lib().bzg_tune_force(int(knob), int(variant)), "bzg_tune_force")
+
+
+def tune_state(knob: int = 0, size_class: int = 0):
+    """(chosen variant or -1, [(launch value, mean ns per unit, samples)])."""
+    vals = np.zeros(4, dtype=np.int32)
+    ns = np.zeros(4, dtype=np.float32)
+    n = np.zeros(4, dtype=np.int32)
+    c = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    chosen = lib().bzg_tune_state(int(knob), int(size_class), c(vals), c(ns), c(n), 4)
+    if chosen == -2:
+        raise BzgError("bzg_tune_state: bad argument")
+    return int(chosen), [(int(vals[i]), float(ns[i]), int(n[i])) for i in range(4) if vals[i]]
 
 
 def probe_kernel(name) -> None:

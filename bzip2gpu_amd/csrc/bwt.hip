@@ -571,15 +571,18 @@ __global__ void __launch_bounds__(256) k_lscan(Dev d, int cur) {
 
 // One LSD pass over a large-group tile: stable rank of every row by digit,
 // rows placed in LDS in tile-sorted order, then written out so that rows of
-// one digit go to consecutive global addresses (coalesced runs).  512
-// threads, 8 rows each.
-__global__ void __launch_bounds__(512) k_lscatter(Dev d, int cur, int shift, int src) {
-  __shared__ uint32_t cnt[8][256];
+// one digit go to consecutive global addresses (coalesced runs).  NT threads
+// (NT/64 waves), 4096/NT rows each; the shape is an autotuned choice and
+// every shape gives the same result.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_lscatter(Dev d, int cur, int shift, int src) {
+  constexpr int NW = NT / 64, PER = kTile / NT;
+  __shared__ uint32_t cnt[NW][256];
   __shared__ uint32_t lstart[256];   // tile-local start of each digit
   __shared__ uint32_t gstart[256];   // global destination of each digit's first row
   __shared__ uint32_t skey[kTile];
   __shared__ uint32_t sval[kTile];
-  __shared__ int32_t tmp[16];
+  __shared__ int32_t tmp[NW + 1];
   const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const Seg g = d.seg[cur][d.ltile_seg[t]];
@@ -591,42 +594,42 @@ __global__ void __launch_bounds__(512) k_lscatter(Dev d, int cur, int shift, int
   uint32_t* Sd = src ? d.SA : d.SA2;
   const int n = min(kTile, g.len - pos);
   const uint64_t base = off + g.start + pos;
-  for (int q = tid; q < 8 * 256; q += 512) (&cnt[0][0])[q] = 0;
+  for (int q = tid; q < NW * 256; q += NT) (&cnt[0][0])[q] = 0;
   __syncthreads();
-  uint32_t key[8], val[8], rk[8];
+  uint32_t key[PER], val[PER], rk[PER];
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const int e = wv * 512 + q * 64 + lane;
+  for (int q = 0; q < PER; q++) {
+    const int e = wv * (64 * PER) + q * 64 + lane;
     key[q] = e < n ? Ks[base + e] : 0u;
     val[q] = e < n ? Ss[base + e] : 0u;
   }
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const int e = wv * 512 + q * 64 + lane;
+  for (int q = 0; q < PER; q++) {
+    const int e = wv * (64 * PER) + q * 64 + lane;
     rk[q] = wave_rank_digit((key[q] >> shift) & 255u, e < n, cnt[wv]);
   }
   __syncthreads();
   {
-    uint32_t c[8];
+    uint32_t c[NW];
     int sum = 0;
     if (tid < 256) {
 #pragma unroll
-      for (int w = 0; w < 8; w++) { c[w] = cnt[w][tid]; sum += (int)c[w]; }
+      for (int w = 0; w < NW; w++) { c[w] = cnt[w][tid]; sum += (int)c[w]; }
     }
     int tot;
-    const int ls = wg_excl_sum<512, int>(sum, tmp, &tot);
+    const int ls = wg_excl_sum<NT, int>(sum, tmp, &tot);
     if (tid < 256) {
       lstart[tid] = (uint32_t)ls;
       gstart[tid] = d.hist[(int64_t)t * 256 + tid];
       uint32_t r = (uint32_t)ls;
 #pragma unroll
-      for (int w = 0; w < 8; w++) { cnt[w][tid] = r; r += c[w]; }
+      for (int w = 0; w < NW; w++) { cnt[w][tid] = r; r += c[w]; }
     }
   }
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const int e = wv * 512 + q * 64 + lane;
+  for (int q = 0; q < PER; q++) {
+    const int e = wv * (64 * PER) + q * 64 + lane;
     if (e < n) {
       const uint32_t p = cnt[wv][(key[q] >> shift) & 255u] + rk[q];
       skey[p] = key[q];
@@ -634,7 +637,7 @@ __global__ void __launch_bounds__(512) k_lscatter(Dev d, int cur, int shift, int
     }
   }
   __syncthreads();
-  for (int p = tid; p < n; p += 512) {
+  for (int p = tid; p < n; p += NT) {
     const uint32_t k = skey[p];
     const uint32_t dg = (k >> shift) & 255u;
     const uint32_t dst = gstart[dg] + ((uint32_t)p - lstart[dg]);
@@ -857,7 +860,7 @@ void launch_bwt_init(const Dev& d, hipStream_t s) {
 
 void launch_bwt_round(const Dev& d, int cur, int round, int32_t P, int key_bits,
                       int32_t npacks, int32_t ntiles, int32_t nlarge, int32_t nseg,
-                      hipStream_t s) {
+                      hipStream_t s, const int* scatter_nt, const hipEvent_t* pass_ev) {
   k_mk_packs<<<(npacks + 1 + 255) / 256, 256, 0, s>>>(d, npacks, nseg);
   k_mk_large<<<(nseg + 255) / 256, 256, 0, s>>>(d, cur, nseg);
   if (round > 0 && npacks + ntiles > 0) {  // round-0 keys come from k_bwt_init
@@ -875,12 +878,16 @@ void launch_bwt_round(const Dev& d, int cur, int round, int32_t P, int key_bits,
   if (ntiles > 0) {
     const int npass = (key_bits + 7) / 8;
     for (int p = 0; p < npass; p++) {
+      if (pass_ev) (void)hipEventRecord(pass_ev[p], s);
       k_lhist<<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
       k_lscan<<<nlarge, 256, 0, s>>>(d, cur);
       probe_begin("k_lscatter", s);
-      k_lscatter<<<ntiles, 512, 0, s>>>(d, cur, 8 * p, p & 1);
+      if (scatter_nt[p] == 256) k_lscatter<256><<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
+      else if (scatter_nt[p] == 1024) k_lscatter<1024><<<ntiles, 1024, 0, s>>>(d, cur, 8 * p, p & 1);
+      else k_lscatter<512><<<ntiles, 512, 0, s>>>(d, cur, 8 * p, p & 1);
       probe_end("k_lscatter", s);
     }
+    if (pass_ev) (void)hipEventRecord(pass_ev[npass], s);
     const int fin = npass & 1;
     k_rk_zero<<<(d.ntile_c + 255) / 256, 256, 0, s>>>(d, d.ntile_c);
     k_lsum<<<ntiles, 256, 0, s>>>(d, cur, fin);

@@ -12,6 +12,7 @@
 #include "../../include/bzg.h"
 #include "engine.h"
 #include "server.h"
+#include "tuner.h"
 
 using namespace bzg;
 
@@ -410,6 +411,28 @@ bzg_status bzg_last_stage_times(float* h_ms6, int32_t* h_rounds, int64_t* h_bwt_
   if (h_rounds) *h_rounds = c->last.rounds;
   if (h_bwt_rows) *h_bwt_rows = c->last.bwt_elems;
   return BZG_OK;
+}
+
+bzg_status bzg_tune_force(int32_t knob, int32_t variant) {
+  if (knob < 0 || knob >= bzg::TUNE_KNOBS || variant < -1 || variant >= bzg::Tuner::get().variants(knob))
+    return fail(BZG_ERR_INVALID, "bad tuning knob or variant");
+  bzg::Tuner::get().force(knob, variant);
+  return BZG_OK;
+}
+
+int32_t bzg_tune_state(int32_t knob, int32_t size_class, int32_t* h_values, float* h_ns_per_unit,
+                       int32_t* h_samples, int32_t max_variants) {
+  if (knob < 0 || knob >= bzg::TUNE_KNOBS || size_class < 0 || size_class >= bzg::kTuneClasses) return -2;
+  bzg::Tuner& t = bzg::Tuner::get();
+  double ns[bzg::kTuneMaxVariants];
+  int n[bzg::kTuneMaxVariants];
+  const int chosen = t.state(knob, size_class, ns, n);
+  for (int v = 0; v < t.variants(knob) && v < max_variants; v++) {
+    if (h_values) h_values[v] = t.value(knob, v);
+    if (h_ns_per_unit) h_ns_per_unit[v] = (float)ns[v];
+    if (h_samples) h_samples[v] = n[v];
+  }
+  return chosen;
 }
 
 bzg_status bzg_last_sort_rows(int64_t* h_small_rows, int64_t* h_large_rows, int64_t* h_large_pass_rows) {

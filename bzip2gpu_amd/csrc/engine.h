@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <utility>
+#include <array>
 #include <vector>
 
 #include "pipeline.h"
@@ -110,6 +111,7 @@ class Engine {
   int64_t active_elems_ = 0;
   int64_t small_rows_ = 0, large_rows_ = 0, large_pass_rows_ = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> rev_;
+  std::vector<std::array<hipEvent_t, 5>> tev_;  // around each radix pass, per round
   std::vector<int64_t> round_rows_;
   std::vector<float> round_ms_;
   hipEvent_t ev_[8] = {};

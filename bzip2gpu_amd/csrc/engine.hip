@@ -1,7 +1,9 @@
 // engine.hip - batch planning, workspace and stage sequencing (host code).
 #include "engine.h"
+#include "tuner.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -114,6 +116,8 @@ Engine::~Engine() {
     (void)hipEventDestroy(pr.first);
     (void)hipEventDestroy(pr.second);
   }
+  for (auto& ev : tev_)
+    for (auto e : ev) (void)hipEventDestroy(e);
 }
 
 int Engine::plan(const std::vector<StreamSpec>& streams, bool allow_big) {
@@ -308,10 +312,22 @@ int Engine::run_bwt(hipStream_t s, int* rounds_out) {
   int32_t P = 0;
   int keybits_late = 1;
   while ((1 << keybits_late) < max_cap_ + 1) keybits_late++;
+  // The radix scatter's launch shape is autotuned: a round with large groups
+  // runs the shape the tuner picks for its size class and reports the radix
+  // passes' time per row and pass once the next readback has synchronised.
+  struct PendingTune { int round = -1, cls = 0, npass = 0, var = 0; int64_t units = 0; } tune;
+  auto report_tune = [&]() {
+    if (tune.round < 0) return;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, tev_[tune.round][0], tev_[tune.round][tune.npass]) == hipSuccess && tune.units > 0)
+      Tuner::get().report(TUNE_RADIX_SHAPE, tune.cls, tune.var, (double)ms * 1e6 / (double)tune.units);
+    tune.round = -1;
+  };
   for (;;) {
     launch_bwt_plan(d_, cur, 0, s);
     BZG_TRY(hipMemcpyAsync(h_ctr_, d_.ctr, CTR_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     BZG_TRY(hipStreamSynchronize(s));
+    report_tune();
     const int32_t nseg = h_ctr_[CTR_NSEG];
     if (h_ctr_[CTR_ERR]) return (int)hipErrorIllegalState;
     if (nseg == 0) break;
@@ -333,7 +349,24 @@ int Engine::run_bwt(hipStream_t s, int* rounds_out) {
       rev_.emplace_back(a, b);
     }
     BZG_TRY(hipEventRecord(rev_[round].first, s));
-    launch_bwt_round(d_, cur, round, P, keybits, npacks, ntiles, nlarge, nseg, s);
+    int scatter_nt[4] = {512, 512, 512, 512};
+    const hipEvent_t* pass_ev = nullptr;
+    if (ntiles > 0) {
+      while (tev_.size() < (size_t)round + 1) {
+        std::array<hipEvent_t, 5> ev{};
+        for (auto& e : ev) BZG_TRY(hipEventCreate(&e));
+        tev_.push_back(ev);
+      }
+      Tuner& tu = Tuner::get();
+      tune.round = round;
+      tune.cls = tune_radix_class(h_ctr_[CTR_LELEMS], nlarge);
+      tune.npass = (keybits + 7) / 8;
+      tune.units = (int64_t)h_ctr_[CTR_LELEMS] * tune.npass;
+      tune.var = tu.pick(TUNE_RADIX_SHAPE, tune.cls);
+      for (int p = 0; p < 4; p++) scatter_nt[p] = tu.value(TUNE_RADIX_SHAPE, tune.var);
+      pass_ev = tev_[round].data();
+    }
+    launch_bwt_round(d_, cur, round, P, keybits, npacks, ntiles, nlarge, nseg, s, scatter_nt, pass_ev);
     BZG_TRY(hipGetLastError());
     BZG_TRY(hipEventRecord(rev_[round].second, s));
     round_rows_.push_back((int64_t)small + (int64_t)h_ctr_[CTR_LELEMS]);
@@ -342,6 +375,7 @@ int Engine::run_bwt(hipStream_t s, int* rounds_out) {
     P = round == 0 ? 4 : (P >= (1 << 29) ? P : 2 * P);
     round++;
   }
+  report_tune();  // (the loop exits right after a synchronised readback)
   if (rounds_out) *rounds_out = round;
   round_ms_.assign((size_t)round, 0.f);
   for (int r = 0; r < round; r++) BZG_TRY(hipEventElapsedTime(&round_ms_[r], rev_[r].first, rev_[r].second));

@@ -119,9 +119,11 @@ void launch_crc_rle1(const Dev& d, hipStream_t s);   // CRC + RLE1 -> T, m, pres
 void launch_bwt_init(const Dev& d, hipStream_t s);   // SA = identity, first group list
 // one suffix-sort round; returns via host the number of groups processed
 void launch_bwt_plan(const Dev& d, int cur, int nseg, hipStream_t s);
+// scatter_nt[p]: threads per radix-scatter workgroup of pass p (256, 512 or
+// 1024); pass_ev (may be null): npass + 1 events around the radix passes.
 void launch_bwt_round(const Dev& d, int cur, int round, int32_t P, int key_bits,
                       int32_t npacks, int32_t ntiles, int32_t nlarge, int32_t nseg,
-                      hipStream_t s);
+                      hipStream_t s, const int* scatter_nt, const hipEvent_t* pass_ev);
 void launch_mtf_rle2(const Dev& d, hipStream_t s);   // MTF, orig_ptr, RLE2 -> SYM, s, freq
 void launch_mtf(const Dev& d, hipStream_t s);        // L gather, MTF, orig_ptr, n_in_use
 void launch_rle2(const Dev& d, hipStream_t s);       // RLE2 (needs tc_fnz, n_in_use)

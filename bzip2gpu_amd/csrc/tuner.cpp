@@ -1,0 +1,108 @@
+// tuner.cpp - launch-shape autotuner (see tuner.h).
+#include "tuner.h"
+
+#include <cstdio>
+#include <cstdlib>
+
+namespace bzg {
+
+namespace {
+// TUNE_RADIX_SHAPE: threads per 4096-row tile (16, 8 or 4 rows per thread)
+const int kRadixShapes[] = {256, 512, 1024};
+}  // namespace
+
+Tuner& Tuner::get() {
+  static Tuner t;
+  return t;
+}
+
+Tuner::Tuner() {
+  for (int k = 0; k < TUNE_KNOBS; k++) forced_[k] = -1;
+  const char* path = std::getenv("BZG_TUNE_CACHE");
+  if (!path || !*path) return;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return;
+  int k, c, v;
+  while (std::fscanf(f, "%d %d %d", &k, &c, &v) == 3)
+    if (k >= 0 && k < TUNE_KNOBS && c >= 0 && c < kTuneClasses && v >= 0 && v < variants(k))
+      slot_[k][c].chosen = v;
+  std::fclose(f);
+}
+
+int Tuner::variants(int knob) const {
+  return knob == TUNE_RADIX_SHAPE ? (int)(sizeof(kRadixShapes) / sizeof(kRadixShapes[0])) : 0;
+}
+
+int Tuner::value(int knob, int variant) const {
+  if (knob == TUNE_RADIX_SHAPE && variant >= 0 && variant < variants(knob)) return kRadixShapes[variant];
+  return 0;
+}
+
+int Tuner::pick(int knob, int cls) {
+  std::lock_guard<std::mutex> lk(mu_);
+  if (forced_[knob] >= 0) return forced_[knob];
+  Slot& s = slot_[knob][cls];
+  if (s.chosen >= 0) return s.chosen;
+  int best = 0;
+  for (int v = 1; v < variants(knob); v++)
+    if (s.n[v] < s.n[best]) best = v;
+  return best;
+}
+
+void Tuner::report(int knob, int cls, int variant, double ns_per_unit) {
+  std::lock_guard<std::mutex> lk(mu_);
+  if (forced_[knob] >= 0 || variant < 0 || variant >= variants(knob) || !(ns_per_unit > 0)) return;
+  Slot& s = slot_[knob][cls];
+  if (s.chosen >= 0) return;
+  s.sum[variant] += ns_per_unit;
+  s.n[variant]++;
+  for (int v = 0; v < variants(knob); v++)
+    if (s.n[v] < kTuneTrials) return;
+  int best = 0;
+  for (int v = 1; v < variants(knob); v++)
+    if (s.sum[v] / s.n[v] < s.sum[best] / s.n[best]) best = v;
+  s.chosen = best;
+  save_locked();
+}
+
+void Tuner::force(int knob, int variant) {
+  std::lock_guard<std::mutex> lk(mu_);
+  forced_[knob] = variant;
+}
+
+int Tuner::state(int knob, int cls, double* ns_per_unit, int* samples) {
+  std::lock_guard<std::mutex> lk(mu_);
+  const Slot& s = slot_[knob][cls];
+  for (int v = 0; v < variants(knob); v++) {
+    if (ns_per_unit) ns_per_unit[v] = s.n[v] ? s.sum[v] / s.n[v] : 0.0;
+    if (samples) samples[v] = s.n[v];
+  }
+  return forced_[knob] >= 0 ? forced_[knob] : s.chosen;
+}
+
+void Tuner::reset() {
+  std::lock_guard<std::mutex> lk(mu_);
+  for (auto& row : slot_)
+    for (auto& s : row) s = Slot{};
+}
+
+void Tuner::save_locked() {
+  const char* path = std::getenv("BZG_TUNE_CACHE");
+  if (!path || !*path) return;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return;
+  for (int k = 0; k < TUNE_KNOBS; k++)
+    for (int c = 0; c < kTuneClasses; c++)
+      if (slot_[k][c].chosen >= 0) std::fprintf(f, "%d %d %d\n", k, c, slot_[k][c].chosen);
+  std::fclose(f);
+}
+
+int tune_radix_class(int64_t rows, int64_t groups) {
+  const int64_t mean = groups > 0 ? rows / groups : 0;
+  if (mean >= (1 << 18)) return 0;  // whole blocks (round 0)
+  if (mean >= (1 << 14)) return 1;
+  if (mean >= (1 << 12)) return 2;
+  return 3;
+}
+
+}  // namespace bzg

@@ -1,0 +1,54 @@
+// tuner.h - launch-shape autotuner (host side).
+//
+// Some kernels come in several launch shapes that compute identical results
+// at different speeds on different inputs.  The tuner picks a shape per
+// knob and size class: while exploring it hands out every variant in turn,
+// timed by the caller with HIP events (nanoseconds per unit of work), and
+// after kTuneTrials samples of each it keeps the fastest for the rest of the
+// process.  Choices can be forced (tests), inspected (benchmarks) and kept
+// across processes in a text file named by the BZG_TUNE_CACHE environment
+// variable ("knob class variant" per line).
+#pragma once
+
+#include <cstdint>
+#include <mutex>
+
+namespace bzg {
+
+enum TuneKnob : int {
+  TUNE_RADIX_SHAPE = 0,  // threads per large-group radix-scatter workgroup
+  TUNE_KNOBS = 1
+};
+constexpr int kTuneMaxVariants = 4;
+constexpr int kTuneClasses = 4;
+constexpr int kTuneTrials = 2;
+
+class Tuner {
+ public:
+  static Tuner& get();
+  int variants(int knob) const;
+  int value(int knob, int variant) const;  // the launch parameter of a variant
+  int pick(int knob, int cls);             // variant to run now
+  void report(int knob, int cls, int variant, double ns_per_unit);
+  void force(int knob, int variant);       // -1: automatic
+  // chosen variant (-1 while exploring); per-variant mean ns/unit and samples
+  int state(int knob, int cls, double* ns_per_unit, int* samples);
+  void reset();
+
+ private:
+  Tuner();
+  void save_locked();
+  struct Slot {
+    double sum[kTuneMaxVariants] = {};
+    int n[kTuneMaxVariants] = {};
+    int chosen = -1;
+  };
+  std::mutex mu_;
+  Slot slot_[TUNE_KNOBS][kTuneClasses];
+  int forced_[TUNE_KNOBS];
+};
+
+// size class of the radix knob from the mean large-group length
+int tune_radix_class(int64_t rows, int64_t groups);
+
+}  // namespace bzg

@@ -97,6 +97,21 @@ bzg_status bzg_set_max_batch_bytes(uint64_t bytes);
 bzg_status bzg_last_stage_times(float *h_ms6, int32_t *h_rounds,
                                 int64_t *h_bwt_rows);
 
+/* Launch-shape autotuner (no counterpart in the reference).  Knob 0 is the
+ * threads per workgroup of the large-group radix scatter, variants 0/1/2 =
+ * 256/512/1024; every variant gives identical output.  Choices are made per
+ * size class (0: groups of >= 256Ki rows, i.e. whole blocks; 1: >= 16Ki;
+ * 2: >= 4Ki; 3: smaller) after timing each variant twice, and are kept in
+ * the file named by BZG_TUNE_CACHE when that variable is set.
+ * bzg_tune_force pins a variant for every class (-1: automatic).
+ * bzg_tune_state returns the chosen variant of a class (-1 while exploring,
+ * -2 for a bad argument) and fills, per variant, its launch value, mean
+ * nanoseconds per row and pass, and sample count. */
+bzg_status bzg_tune_force(int32_t knob, int32_t variant);
+int32_t bzg_tune_state(int32_t knob, int32_t size_class, int32_t *h_values,
+                       float *h_ns_per_unit, int32_t *h_samples,
+                       int32_t max_variants);
+
 /* Suffix-sort rows of the last batched call, summed over rounds: rows in
  * small groups (LDS sorts), rows in large groups (tile radix sort), and
  * large-group rows times the radix passes they went through (benchmarking

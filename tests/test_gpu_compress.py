@@ -104,3 +104,32 @@ def test_batch_device_many_inputs(dev):
     for i, d in enumerate(datas):
         s = host[bo[i]:bo[i] + by[i]]
         _same(s, oracle.compress(d, int(levels[i]), block_bytes=100000 * int(levels[i])), f"input {i}")
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_radix_shapes_identical(dev, variant):
+    """Every autotuned launch shape of the radix scatter gives the same stream."""
+    import bzip2gpu_amd as bz
+    import bzip2gpu_amd.datagen as dg
+    data = dg.mix(3 << 20, 77).tobytes()
+    want = oracle.compress(data, 9)
+    try:
+        bz.tune_force(0, variant)
+        _case(f"radix shape variant {variant}")
+        _same(bz.compress(data, 9), want, f"variant {variant}")
+    finally:
+        bz.tune_force(0, -1)
+
+
+def test_autotuner_explores_then_chooses(dev):
+    import bzip2gpu_amd as bz
+    import bzip2gpu_amd.datagen as dg
+    data = dg.mix(2 << 20, 78).tobytes()
+    want = oracle.compress(data, 9)
+    _case("autotuner exploration")
+    for _ in range(8):  # 3 shapes x 2 samples for the whole-block class
+        _same(bz.compress(data, 9), want, "autotuned run")
+    chosen, rows = bz.tune_state(0, 0)
+    assert [v for v, _, _ in rows] == [256, 512, 1024]
+    assert chosen in (0, 1, 2)
+    assert all(n >= 2 for _, _, n in rows)
