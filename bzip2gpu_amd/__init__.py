@@ -32,6 +32,8 @@ from ._lib import (  # noqa: F401
     rle2,
     set_max_batch_bytes,
     symbols_table,
+    set_stage_graphs,
+    graph_counts,
     tune_force,
     tune_state,
     version,

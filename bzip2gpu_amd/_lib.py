@@ -62,6 +62,9 @@ def lib():
         "bzg_last_stage_times": ([P, I32P, I64P], i32),
         "bzg_last_sort_rows": ([I64P, I64P, I64P], i32),
         "bzg_tune_force": ([i32, i32], i32),
+        "bzg_set_stage_graphs": ([i32], i32),
+        "bzg_graph_captures": ([], i64),
+        "bzg_graph_replays": ([], i64),
         "bzg_tune_state": ([i32, i32, P, P, P, i32], i32),
         "bzg_probe_kernel": ([ctypes.c_char_p], i32),
         "bzg_probe_read": ([ctypes.POINTER(ctypes.c_float), I32P], i32),
@@ -188,6 +191,16 @@ def last_sort_rows():
     a, b, c = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
     _check(lib().bzg_last_sort_rows(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "bzg_last_sort_rows")
     return int(a.value), int(b.value), int(c.value)
+
+
+def set_stage_graphs(on: bool) -> None:
+    """Run the non-sort stage groups as replayed HIP graphs (default) or eagerly."""
+    _check(lib().bzg_set_stage_graphs(1 if on else 0), "bzg_set_stage_graphs")
+
+
+def graph_counts():
+    """(stage-group captures, graph launches) of the device context."""
+    return int(lib().bzg_graph_captures()), int(lib().bzg_graph_replays())
 
 
 def tune_force(knob: int, variant: int) -> None:

@@ -413,6 +413,29 @@ bzg_status bzg_last_stage_times(float* h_ms6, int32_t* h_rounds, int64_t* h_bwt_
   return BZG_OK;
 }
 
+bzg_status bzg_set_stage_graphs(int32_t on) {
+  bzg::set_stage_graphs(on != 0);
+  return BZG_OK;
+}
+
+int64_t bzg_graph_replays(void) {
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return -1;
+  int64_t n = 0;
+  for (int i = 0; i < 5; i++) n += c->eng.graph_replays(i);
+  return n;
+}
+
+int64_t bzg_graph_captures(void) {
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return -1;
+  int64_t n = 0;
+  for (int i = 0; i < 5; i++) n += c->eng.graph_captures(i);
+  return n;
+}
+
 bzg_status bzg_tune_force(int32_t knob, int32_t variant) {
   if (knob < 0 || knob >= bzg::TUNE_KNOBS || variant < -1 || variant >= bzg::Tuner::get().variants(knob))
     return fail(BZG_ERR_INVALID, "bad tuning knob or variant");

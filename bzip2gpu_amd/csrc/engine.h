@@ -49,6 +49,11 @@ class Engine;
 int compress_plain_device(Engine& eng, const uint8_t* d_in, size_t len, int level, int32_t block_bytes,
                           std::vector<uint8_t>& out, hipStream_t s);
 
+// Stage groups other than the suffix sort run as captured HIP graphs
+// (process-wide switch; BZG_NO_GRAPHS=1 starts with graphs off).
+void set_stage_graphs(bool on);
+bool stage_graphs();
+
 // per-kernel HIP-event probe (one target kernel name at a time)
 void probe_set(const char* kernel);
 int probe_read(float* total_ms, int* launches);
@@ -95,9 +100,25 @@ class Engine {
   const std::vector<int64_t>& round_rows() const { return round_rows_; }
   const std::vector<float>& round_ms() const { return round_ms_; }
   int device() const { return device_; }
+  // graph captures and replays of stage group i (0 front, 1 MTF+RLE2,
+  // 2 Huffman, 3 pack, 4 assembly)
+  int64_t graph_captures(int i) const { return sgraph_[i].captures; }
+  // per-engine switch (the plain per-block path plans a new batch per block,
+  // where a capture per block would not pay off)
+  void set_graphs(bool on) { use_graphs_ = on; }
+  bool graphs() const { return use_graphs_; }
+  int64_t graph_replays(int i) const { return sgraph_[i].replays; }
 
  private:
   int grow(size_t bytes);
+  int run_stage(int idx, hipStream_t s, int (Engine::*fn)(hipStream_t));
+  struct StageGraph {
+    hipGraphExec_t exec = nullptr;
+    Dev key{};
+    int64_t captures = 0, replays = 0;
+  };
+  StageGraph sgraph_[5];
+  bool use_graphs_ = true;
   int device_;
   Dev d_{};
   void* ws_ = nullptr;

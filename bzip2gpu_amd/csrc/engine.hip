@@ -3,6 +3,7 @@
 #include "tuner.h"
 
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <chrono>
 #include <cstdio>
@@ -33,6 +34,21 @@ void trace(hipStream_t s, const char* what, long a = -1, long b = -1, long c = -
 }
 }  // namespace
 
+// ------------------------------------------------------------------ graphs
+namespace {
+std::atomic<int> g_graphs{-1};  // -1: from the environment (BZG_NO_GRAPHS=1 turns off)
+}
+void set_stage_graphs(bool on) { g_graphs = on ? 1 : 0; }
+bool stage_graphs() {
+  int v = g_graphs.load();
+  if (v < 0) {
+    const char* e = getenv("BZG_NO_GRAPHS");
+    v = (e && e[0] == '1') ? 0 : 1;
+    g_graphs = v;
+  }
+  return v == 1 && !trace_on();
+}
+
 // ------------------------------------------------------------------ probe
 namespace {
 struct ProbeState {
@@ -47,9 +63,15 @@ ProbeState& probe() {
 }
 }  // namespace
 
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
 void probe_begin(const char* kernel, hipStream_t s) {
   ProbeState& P = probe();
   if (P.target.empty() || P.target != kernel) return;
+  if (capturing(s)) return;  // kernels inside captured stage graphs are not probed
   if (P.used == P.ev.size()) {
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess) return;
@@ -118,6 +140,8 @@ Engine::~Engine() {
   }
   for (auto& ev : tev_)
     for (auto e : ev) (void)hipEventDestroy(e);
+  for (auto& g : sgraph_)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
 }
 
 int Engine::plan(const std::vector<StreamSpec>& streams, bool allow_big) {
@@ -402,6 +426,39 @@ int Engine::run_assemble(hipStream_t s) {
   return (int)hipGetLastError();
 }
 
+// Runs a stage group as a captured HIP graph.  The graph is instantiated
+// once and replayed while the device-side plan (the Dev struct: workspace
+// pointers, input pointer, block/tile/stream counts, hence every grid size)
+// is unchanged; the stage groups have no host-sized launches, copies or
+// memsets, so a replay does exactly what the eager launches do.
+int Engine::run_stage(int idx, hipStream_t s, int (Engine::*fn)(hipStream_t)) {
+  // eager on the legacy stream, inside a caller's own capture, or when off
+  if (!use_graphs_ || !stage_graphs() || s == nullptr || capturing(s)) return (this->*fn)(s);
+  StageGraph& g = sgraph_[idx];
+  if (!g.exec || std::memcmp(&g.key, &d_, sizeof(Dev)) != 0) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+    hipGraph_t graph = nullptr;
+    BZG_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = (this->*fn)(s);
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    if (rc || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc ? rc : (int)e;
+    }
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      g.exec = nullptr;
+      return (int)ei;
+    }
+    std::memcpy(&g.key, &d_, sizeof(Dev));
+    g.captures++;
+  }
+  g.replays++;
+  return (int)hipGraphLaunch(g.exec, s);
+}
+
 int Engine::run(hipStream_t s, bool assemble, StageTimes* times) {
   BZG_TRY(hipSetDevice(device_));
   if (desc_.empty()) {
@@ -415,7 +472,7 @@ int Engine::run(hipStream_t s, bool assemble, StageTimes* times) {
   int rounds = 0;
   trace(s, "run start", (long)desc_.size(), (long)ss_.size());
   if (times) BZG_TRY(hipEventRecord(ev_[0], s));
-  int rc = run_front(s);
+  int rc = run_stage(0, s, &Engine::run_front);
   if (rc) return rc;
   trace(s, "front done");
   if (times) BZG_TRY(hipEventRecord(ev_[1], s));
@@ -423,20 +480,20 @@ int Engine::run(hipStream_t s, bool assemble, StageTimes* times) {
   if (rc) return rc;
   trace(s, "bwt done", rounds);
   if (times) BZG_TRY(hipEventRecord(ev_[2], s));
-  rc = run_mtf_rle2(s);
+  rc = run_stage(1, s, &Engine::run_mtf_rle2);
   if (rc) return rc;
   trace(s, "mtf+rle2 done");
   if (times) BZG_TRY(hipEventRecord(ev_[3], s));
-  rc = run_huffman(s);
+  rc = run_stage(2, s, &Engine::run_huffman);
   if (rc) return rc;
   trace(s, "huffman done");
   if (times) BZG_TRY(hipEventRecord(ev_[4], s));
-  rc = run_pack(s);
+  rc = run_stage(3, s, &Engine::run_pack);
   if (rc) return rc;
   trace(s, "pack done");
   if (times) BZG_TRY(hipEventRecord(ev_[5], s));
   if (assemble) {
-    rc = run_assemble(s);
+    rc = run_stage(4, s, &Engine::run_assemble);
     if (rc) return rc;
     trace(s, "assemble done");
   }
@@ -521,6 +578,13 @@ int compress_plain_device(Engine& eng, const uint8_t* d_in, size_t len, int leve
   uint32_t combined = 0;
   BlockResult r;
   int e;
+  // every block is a new one-block plan: the stage groups run eagerly
+  struct GraphsOff {
+    Engine& g;
+    bool was;
+    explicit GraphsOff(Engine& x) : g(x), was(x.graphs()) { g.set_graphs(false); }
+    ~GraphsOff() { g.set_graphs(was); }
+  } graphs_off(eng);
   for (size_t off = 0; off < len; off += (size_t)block_bytes) {
     size_t n = len - off < (size_t)block_bytes ? len - off : (size_t)block_bytes;
     std::vector<StreamSpec> ss{StreamSpec{0, n, level, block_bytes}};

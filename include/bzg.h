@@ -97,6 +97,18 @@ bzg_status bzg_set_max_batch_bytes(uint64_t bytes);
 bzg_status bzg_last_stage_times(float *h_ms6, int32_t *h_rounds,
                                 int64_t *h_bwt_rows);
 
+/* Stage graphs (no counterpart in the reference): the batched path runs the
+ * CRC/RLE1, MTF/RLE2, Huffman, pack and assembly stage groups as HIP graphs
+ * captured once per device-side plan and replayed while it is unchanged; the
+ * suffix-sort rounds, whose launch sizes come from per-round readbacks, run
+ * eagerly.  bzg_set_stage_graphs(0) runs everything eagerly (also
+ * BZG_NO_GRAPHS=1 at start-up).  bzg_graph_captures / bzg_graph_replays
+ * count stage-group captures and graph launches of this thread's device
+ * context (-1 on error). */
+bzg_status bzg_set_stage_graphs(int32_t on);
+int64_t bzg_graph_captures(void);
+int64_t bzg_graph_replays(void);
+
 /* Launch-shape autotuner (no counterpart in the reference).  Knob 0 is the
  * threads per workgroup of the large-group radix scatter, variants 0/1/2 =
  * 256/512/1024; every variant gives identical output.  Choices are made per

@@ -133,3 +133,33 @@ def test_autotuner_explores_then_chooses(dev):
     assert [v for v, _, _ in rows] == [256, 512, 1024]
     assert chosen in (0, 1, 2)
     assert all(n >= 2 for _, _, n in rows)
+
+
+def test_stage_graph_replays_match_eager(dev):
+    """Stage groups captured as HIP graphs: every replay, with eager work and
+    synchronisations between replays, equals the eager result."""
+    import torch
+    import bzip2gpu_amd as bz
+    import bzip2gpu_amd.datagen as dg
+    a = dg.mix(2 << 20, 90).tobytes()
+    b = dg.mix(1 << 20, 91).tobytes()
+    _case("stage graphs: eager references")
+    bz.set_stage_graphs(False)
+    eager_a, eager_b = bz.compress(a, 9), bz.compress(b, 9)
+    _same(eager_a, oracle.compress(a, 9), "eager a")
+    try:
+        bz.set_stage_graphs(True)
+        _same(bz.compress(a, 9), eager_a, "first graph run")  # captures
+        c0, r0 = bz.graph_counts()
+        for i in range(3):
+            _case(f"stage graphs: replay {i}")
+            _same(bz.compress(a, 9), eager_a, f"replay {i}")
+            torch.cuda.synchronize()
+            bz.set_stage_graphs(False)
+            _same(bz.compress(b, 9), eager_b, f"eager between replays {i}")
+            bz.set_stage_graphs(True)
+        c1, r1 = bz.graph_counts()
+        assert c1 == c0, "replays of an unchanged plan must not re-capture"
+        assert r1 - r0 >= 3 * 5
+    finally:
+        bz.set_stage_graphs(True)
