@@ -117,6 +117,7 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
   __shared__ __align__(4) uint8_t sflag[256];  // this step's bytes
   __shared__ __align__(4) uint8_t rank[256];
   __shared__ uint8_t newpos[256];
+  __shared__ unsigned long long pm[256];  // per-byte lane masks of the current step
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int b = d.tc_blk[t];
   const BlockDesc D = d.desc[b];
@@ -138,6 +139,7 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
     E[q] = d.lo[(int64_t)t * 256 + q];
     pflag[q] = 0;
     sflag[q] = 0;
+    pm[q] = 0ull;
   }
   __syncthreads();
   // list position of every present byte: # present bytes more recent
@@ -161,13 +163,11 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
     const int i = base + lane;
     const bool valid = i < n;
     const uint32_t c = valid ? buf[i] : 0u;
-    uint64_t peers = ballot(valid);
-#pragma unroll
-    for (int bb = 0; bb < 8; bb++) {
-      bool bit = (c >> bb) & 1u;
-      uint64_t mm = ballot(bit);
-      peers &= bit ? mm : ~mm;
-    }
+    // lanes holding the same byte: each lane ORs its bit into its byte's
+    // LDS mask and reads the mask back (cleared again below)
+    if (valid) atomicOr(&pm[c], 1ull << lane);
+    __syncthreads();
+    const uint64_t peers = valid ? pm[c] : 0ull;
     const uint64_t pb = peers & below;
     const bool has_prev = valid && pb != 0;
     const int prev = has_prev ? 63 - __clzll(pb) : -1;
@@ -216,6 +216,7 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
       sflag[c] = 1;
     }
     if (last) newpos[c] = (uint8_t)(__popcll(lastm >> lane) - 1);
+    if (valid) pm[c] = 0ull;  // every lane has read its mask (in program order)
     __syncthreads();
     {
       const uint32_t f4 = reinterpret_cast<const uint32_t*>(pflag)[lane];
