@@ -87,6 +87,9 @@ void Server::reset_clock() {
 void Server::worker(int k) {
   if (hipSetDevice(device_) != hipSuccess) return;
   Engine eng(device_);
+  // every admitted batch is a new plan, so capturing stage graphs per batch
+  // would not pay off: the workers launch eagerly
+  eng.set_graphs(false);
   hipStream_t s = nullptr;
   if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return;
   std::vector<int64_t> batch;
