@@ -355,6 +355,8 @@ __device__ inline void emit_groups(const Dev& d, int nxt, bool has, Seg g, int32
 }
 
 // ----------------------------------------------------------- small groups
+// packs whose groups are all at most this long are sorted by counting
+constexpr int kCountMax = 64;
 // One workgroup sorts a pack of <= kPack rows in LDS.  KT holds the
 // composite key (group in pack << keybits | key): 32 bits whenever they fit.
 template <typename KT>
@@ -400,6 +402,67 @@ __device__ inline void sort_small_body(const Dev& d, int cur, int keybits, int k
     const int j = L.start[sg] + (e - L.lofs[sg]);
     K0[e] = ((KT)sg << keybits) | (KT)d.KEY[off + j];
     V0[e] = d.SA[off + j];
+  }
+  // largest group of the pack
+  {
+    int mxl = 0;
+    for (int j = tid; j < L.count; j += 256) mxl = max(mxl, (j + 1 < L.count ? L.lofs[j + 1] : total) - L.lofs[j]);
+    mxl = wave_max(mxl);
+    if (lane == 0) xs[wv] = mxl;
+  }
+  __syncthreads();
+  if (max(max(xs[0], xs[1]), max(xs[2], xs[3])) <= kCountMax) {
+    // Small groups only: every row's place in its group by counting (rows
+    // with a smaller key, and equal keys before it); equal keys stay together
+    // in input order and form the next round's subgroup.
+    for (int e = tid; e < total; e += 256) {
+      const int sg = sid[e];
+      const int gs = L.lofs[sg];
+      const int ge = sg + 1 < L.count ? L.lofs[sg + 1] : total;
+      const KT my = K0[e];
+      int lt = 0, eq = 0, eqb = 0;
+      for (int j = gs; j < ge; j++) {
+        const KT kj = K0[j];
+        lt += kj < my ? 1 : 0;
+        const int same = kj == my ? 1 : 0;
+        eq += same;
+        eqb += (j < e) ? same : 0;
+      }
+      const int b = L.blk[sg];
+      const uint64_t off = d.desc[b].off;
+      const int st0 = L.start[sg];
+      const uint32_t v = V0[e];
+      d.SA[off + st0 + lt + eqb] = v;
+      d.RK[off + (v & kSAMask)] = (uint32_t)(st0 + lt);
+      V1[e] = (eq >= 2 && eqb == 0) ? (((uint32_t)lt << 16) | (uint32_t)eq) : 0u;
+    }
+    __syncthreads();
+    // subgroups of >= 2 rows, emitted by their first row, in row order
+    const int e0 = tid * (kPack / 256);
+    int nout = 0;
+#pragma unroll
+    for (int q = 0; q < kPack / 256; q++)
+      if (e0 + q < total && V1[e0 + q]) nout++;
+    int tot;
+    const int ex = wg_excl_sum<256, int>(nout, tmp, &tot);
+    if (tid == 0 && tot > 0) s_base = atomicAdd(&d.ctr[nxt], tot);
+    __syncthreads();
+    if (nout) {
+      int idx = s_base + ex;
+#pragma unroll
+      for (int q = 0; q < kPack / 256; q++) {
+        const int e = e0 + q;
+        const uint32_t w = e < total ? V1[e] : 0u;
+        if (w) {
+          const int sg = sid[e];
+          const Seg ng{L.blk[sg], L.start[sg] + (int)(w >> 16), (int)(w & 0xFFFFu), 0};
+          if (idx < d.seg_cap) d.seg[nxt][idx] = ng;
+          else d.ctr[CTR_ERR] = 1;
+          idx++;
+        }
+      }
+    }
+    return;
   }
   __syncthreads();
   const int nbits = keybits + bits_for((uint32_t)(L.count - 1));
