@@ -6,6 +6,7 @@ callers, tests and benchmarks.  There is no CPU fallback: if the library or
 a GPU is missing, calls raise.
 """
 from ._lib import (  # noqa: F401
+    BzgDataError,
     BzgError,
     REF_BLOCK_BYTES,
     Server,
@@ -17,11 +18,14 @@ from ._lib import (  # noqa: F401
     compress_plain,
     compress_plain_device,
     crc32,
+    decompress,
+    decompress_device,
     histogram,
     huff_build,
     huff_encode,
     lib,
     library_path,
+    last_decode_times,
     last_round_stats,
     last_sort_rows,
     last_stage_times,

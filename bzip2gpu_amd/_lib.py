@@ -23,6 +23,14 @@ class BzgError(RuntimeError):
     pass
 
 
+class BzgDataError(BzgError):
+    """Malformed or corrupt .bz2 input (status BZG_ERR_DATA)."""
+
+
+ERR_CAPACITY = 3
+ERR_DATA = 5
+
+
 def build(jobs: int = 8) -> None:
     """Compiles libbzg.so and the bzip2gpu CLI for gfx950 (in-tree)."""
     subprocess.run(["make", "-s", "-C", os.path.join(_HERE, "csrc"), f"-j{jobs}"], check=True)
@@ -78,6 +86,10 @@ def lib():
         "bzg_histogram": ([P, i32, P, P], i32),
         "bzg_huff_build": ([P, i32, i32, P, P, P, I32P, I32P, P], i32),
         "bzg_huff_encode": ([P, i32, P, P, P, i32, i32, P, I64P, P], i32),
+        "bzg_decompress_buffer": ([P, sz, P, sz, SZP], i32),
+        "bzg_decompress_fetch": ([P, sz, SZP], i32),
+        "bzg_decompress_device": ([P, sz, ctypes.POINTER(ctypes.c_void_p), SZP, P], i32),
+        "bzg_last_decode_times": ([P, I32P, I32P], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -90,7 +102,8 @@ def lib():
 def _check(st: int, what: str) -> None:
     if st != 0:
         msg = lib().bzg_last_error().decode(errors="replace")
-        raise BzgError(f"{what} failed (status {st}): {msg}")
+        cls = BzgDataError if st == ERR_DATA else BzgError
+        raise cls(f"{what} failed (status {st}): {msg}")
 
 
 def version() -> str:
@@ -161,6 +174,47 @@ def compress_batch_device(d_in, offsets, lengths, levels, block_bytes: int = 0, 
         ctypes.c_void_p(stream) if stream else None)
     _check(st, "bzg_compress_batch_device")
     return dout.value, bo, by
+
+
+def decompress(data) -> bytes:
+    """Decodes one or more concatenated .bz2 streams on the device (host
+    bytes in, host bytes out).  Raises BzgDataError on malformed or corrupt
+    input (bad header or magic, CRC mismatch, truncation)."""
+    d = _np_u8(data)
+    cap = max(4 * d.size, 1 << 20)
+    out = np.empty(cap, dtype=np.uint8)
+    n = ctypes.c_size_t(0)
+    L = lib()
+    st = L.bzg_decompress_buffer(d.ctypes.data_as(ctypes.c_void_p) if d.size else None, d.size,
+                                 out.ctypes.data_as(ctypes.c_void_p), cap, ctypes.byref(n))
+    if st == ERR_CAPACITY:
+        out = np.empty(max(n.value, 1), dtype=np.uint8)
+        st = L.bzg_decompress_fetch(out.ctypes.data_as(ctypes.c_void_p), out.size, ctypes.byref(n))
+    _check(st, "bzg_decompress_buffer")
+    return out[: n.value].tobytes()
+
+
+def decompress_device(d_in, stream=None):
+    """Decodes .bz2 data held in a device uint8 tensor.  Returns (device
+    address, length) of the raw output, a library-owned buffer valid until
+    the next decompression call."""
+    ptr = ctypes.c_void_p(0)
+    n = ctypes.c_size_t(0)
+    st = lib().bzg_decompress_device(_ptr(d_in) if d_in.numel() else None, d_in.numel(), ctypes.byref(ptr),
+                                     ctypes.byref(n), stream if stream is not None else _stream())
+    _check(st, "bzg_decompress_device")
+    return (ptr.value or 0), n.value
+
+
+def last_decode_times():
+    """(times_ms[5], blocks, streams) of the last decompression: Huffman
+    (incl. magic scan and chain walk), inverse MTF/RLE2, FL-map sort,
+    inverse BWT walks, inverse RLE1 + CRC."""
+    ms = (ctypes.c_float * 5)()
+    nb = ctypes.c_int32(0)
+    ns = ctypes.c_int32(0)
+    _check(lib().bzg_last_decode_times(ms, ctypes.byref(nb), ctypes.byref(ns)), "bzg_last_decode_times")
+    return list(ms), nb.value, ns.value
 
 
 def last_round_stats(max_rounds: int = 64):

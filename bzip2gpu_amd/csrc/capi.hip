@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/bzg.h"
+#include "decode.h"
 #include "engine.h"
 #include "server.h"
 #include "tuner.h"
@@ -22,9 +23,11 @@ thread_local std::string g_err;
 thread_local int g_dev = 0;
 
 struct Ctx {
-  explicit Ctx(int dev) : eng(dev), stage(dev), device(dev) {}
+  explicit Ctx(int dev) : eng(dev), stage(dev), dec(dev), device(dev) {}
   Engine eng;      // batched / plain paths
   Engine stage;    // per-stage entry points
+  Decoder dec;     // .bz2 -> raw
+  DecodeTimes dlast{};
   int device;
   std::mutex mu;
   uint8_t* d_in = nullptr;
@@ -468,7 +471,95 @@ bzg_status bzg_last_sort_rows(int64_t* h_small_rows, int64_t* h_large_rows, int6
   return BZG_OK;
 }
 
+// ------------------------------------------------------------ decompression
+
+namespace {
+
+bzg_status decode_status(Ctx* c, int e) {
+  if (e == kDecDataError) return fail(BZG_ERR_DATA, c->dec.error());
+  if (e) return hip_fail(e, "decompress");
+  return BZG_OK;
+}
+
+bzg_status copy_decoded(Ctx* c, uint8_t* h_out, size_t out_cap, size_t* out_len) {
+  const size_t n = c->dec.out_len();
+  *out_len = n;
+  if (n > out_cap) return fail(BZG_ERR_CAPACITY, "output buffer too small");
+  if (n) {
+    hipError_t he = hipMemcpyAsync(h_out, c->dec.out(), n, hipMemcpyDeviceToHost, c->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+    if (he != hipSuccess) return hip_fail((int)he, "D2H");
+  }
+  return BZG_OK;
+}
+
+}  // namespace
+
+bzg_status bzg_decompress_buffer(const uint8_t* h_in, size_t len, uint8_t* h_out, size_t out_cap,
+                                 size_t* out_len) {
+  if ((!h_in && len) || (!h_out && out_cap) || !out_len) return fail(BZG_ERR_INVALID, "null pointer");
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(BZG_ERR_HIP, "hipSetDevice failed");
+  int e = ensure_input(c, len + 16);
+  if (e) return hip_fail(e, "input buffer");
+  if (len) {
+    hipError_t he = hipMemcpyAsync(c->d_in, h_in, len, hipMemcpyHostToDevice, c->stream);
+    if (he != hipSuccess) return hip_fail((int)he, "H2D");
+  }
+  if ((st = decode_status(c, c->dec.run(c->d_in, len, c->stream, &c->dlast))) != BZG_OK) return st;
+  return copy_decoded(c, h_out, out_cap, out_len);
+}
+
+bzg_status bzg_decompress_fetch(uint8_t* h_out, size_t out_cap, size_t* out_len) {
+  if ((!h_out && out_cap) || !out_len) return fail(BZG_ERR_INVALID, "null pointer");
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(BZG_ERR_HIP, "hipSetDevice failed");
+  return copy_decoded(c, h_out, out_cap, out_len);
+}
+
+bzg_status bzg_decompress_device(const uint8_t* d_in, size_t len, uint8_t** d_out, size_t* out_len,
+                                 void* stream) {
+  if ((!d_in && len) || !d_out || !out_len) return fail(BZG_ERR_INVALID, "null pointer");
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(BZG_ERR_HIP, "hipSetDevice failed");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  *d_out = nullptr;
+  *out_len = 0;
+  if ((st = decode_status(c, c->dec.run(d_in, len, s, &c->dlast))) != BZG_OK) return st;
+  *d_out = c->dec.out();
+  *out_len = c->dec.out_len();
+  return BZG_OK;
+}
+
+bzg_status bzg_last_decode_times(float* h_ms5, int32_t* h_blocks, int32_t* h_streams) {
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  std::lock_guard<std::mutex> lk(c->mu);
+  const DecodeTimes& t = c->dlast;
+  if (h_ms5) {
+    h_ms5[0] = t.huff;
+    h_ms5[1] = t.mtf;
+    h_ms5[2] = t.sort;
+    h_ms5[3] = t.walk;
+    h_ms5[4] = t.rle1;
+  }
+  if (h_blocks) *h_blocks = t.blocks;
+  if (h_streams) *h_streams = t.streams;
+  return BZG_OK;
+}
+
 }  // extern "C"
+
 
 // ------------------------------------------------------------ stage level
 namespace {

@@ -15,61 +15,10 @@
 // output count (positions before the tile's first run start are settled by
 // the per-block scan from the carried run start), and the write pass places
 // bytes in input order.
+#include "crc.h"
 #include "pipeline.h"
 
 namespace bzg {
-
-struct Pow8Table {
-  uint32_t p[40];
-  constexpr Pow8Table() : p() {
-    uint32_t b = 0x100u;  // x^8
-    for (int k = 0; k < 40; k++) {
-      p[k] = b;
-      b = gf2_mulmod(b, b);
-    }
-  }
-};
-
-__constant__ CrcTable c_crc = CrcTable();
-__constant__ Pow8Table c_pow8 = Pow8Table();
-
-// x^(8*len) mod P from the constant table
-__device__ inline uint32_t xpow8(uint32_t len) {
-  uint32_t r = 1u;
-  int k = 0;
-  while (len) {
-    if (len & 1u) r = gf2_mulmod(r, c_pow8.p[k]);
-    len >>= 1;
-    k++;
-  }
-  return r;
-}
-
-// x^(8r) for byte shifts: x^(8q), x^(128m), x^(32768u) with q < 16, m, u < 256
-struct ShiftTables {
-  uint32_t q1[16];
-  uint32_t q16[256];
-  uint32_t tile[256];
-  constexpr ShiftTables() : q1(), q16(), tile() {
-    uint32_t a = 1u;
-    for (int q = 0; q < 16; q++) { q1[q] = a; a = gf2_mulmod(a, 0x100u); }
-    uint32_t b = 1u;
-    for (int m = 0; m < 256; m++) { q16[m] = b; b = gf2_mulmod(b, a); }
-    uint32_t c = 1u;
-    for (int u = 0; u < 256; u++) { tile[u] = c; c = gf2_mulmod(c, b); }
-  }
-};
-static_assert(kTile == 4096, "shift tables assume 4096-byte tiles");
-__constant__ ShiftTables c_shift = ShiftTables();
-
-// reg * x^(8r) mod P
-__device__ inline uint32_t crc_shift(uint32_t reg, uint32_t r) {
-  if (r >= 256u * kTile) return gf2_mulmod(reg, xpow8(r));
-  if (r >> 12) reg = gf2_mulmod(reg, c_shift.tile[r >> 12]);
-  if ((r >> 4) & 255u) reg = gf2_mulmod(reg, c_shift.q16[(r >> 4) & 255u]);
-  if (r & 15u) reg = gf2_mulmod(reg, c_shift.q1[r & 15u]);
-  return reg;
-}
 
 // Bytes emitted by the first j positions of a run: 5 per 259-piece, and
 // min(o, 3) or 5 for the o positions of a started piece.

@@ -27,6 +27,7 @@ typedef int32_t bzg_status;
 #define BZG_ERR_HIP 2      /* a HIP runtime call failed */
 #define BZG_ERR_CAPACITY 3 /* output buffer too small */
 #define BZG_ERR_INTERNAL 4 /* a device-side invariant failed */
+#define BZG_ERR_DATA 5     /* malformed or corrupt .bz2 input (decompression) */
 
 /* Raw bytes per block of the reference's stream driver: the input is cut
  * into fixed 900000-byte chunks at every level (src/main.cpp:26,36,
@@ -143,6 +144,37 @@ int32_t bzg_last_round_stats(int64_t *h_rows, float *h_ms, int32_t max_rounds);
  * last read. */
 bzg_status bzg_probe_kernel(const char *kernel_name);
 bzg_status bzg_probe_read(float *h_total_ms, int32_t *h_launches);
+
+/* ---------------------------------------------------------- decompression
+ * The reference lists decompression (iBWT, iMTF, RLE2, Huffman decoding,
+ * block-boundary discovery) as not built (TODO.md:28-34, SOURCES.md:5-6);
+ * these entry points decode the format its stream driver writes
+ * (compress.cu:184-375) and any stock bzip2 stream on the device.  Input is
+ * one or more concatenated .bz2 streams; bytes after the last stream that
+ * do not start a new "BZh" header are ignored (as Python's bz2 does).  Block
+ * and stream CRCs are checked; a mismatch or malformed input returns
+ * BZG_ERR_DATA.  Blocks may decode to up to 1125000 RLE1 bytes whatever the
+ * level digit says (the reference's fixed 900000-byte cut can exceed
+ * 100000*level); randomised blocks (bzip2 < 0.9.5) are rejected.
+ */
+/* Host buffers in and out.  If the output does not fit, returns
+ * BZG_ERR_CAPACITY with *out_len set to the decoded size; the decoded bytes
+ * stay on the device until the next decompression call on this device and
+ * bzg_decompress_fetch copies them out. */
+bzg_status bzg_decompress_buffer(const uint8_t *h_in, size_t len, uint8_t *h_out,
+                                 size_t out_cap, size_t *out_len);
+bzg_status bzg_decompress_fetch(uint8_t *h_out, size_t out_cap, size_t *out_len);
+
+/* Input already on the device.  *d_out points to a device buffer owned by
+ * the library (valid until the next decompression call on this device). */
+bzg_status bzg_decompress_device(const uint8_t *d_in, size_t len, uint8_t **d_out,
+                                 size_t *out_len, void *stream);
+
+/* Stage times of the last decompression on this device (ms, HIP events):
+ * [0] magic scan + Huffman decoding (incl. the block-chain walk on the host),
+ * [1] inverse RLE2/MTF, [2] FL-map counting sort, [3] inverse BWT walks,
+ * [4] inverse RLE1 + CRC; and the number of blocks and streams decoded. */
+bzg_status bzg_last_decode_times(float *h_ms5, int32_t *h_blocks, int32_t *h_streams);
 
 /* ---------------------------------------------------------------- serving
  * A request loop with continuous batching (no counterpart in the reference,
