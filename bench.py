@@ -31,7 +31,8 @@ def _args():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="mix256", choices=["mix256", "text1", "periodic64", "batch4096", "serve"])
+    p.add_argument("--workload", default="mix256",
+                   choices=["mix256", "text1", "periodic64", "batch4096", "serve", "decompress"])
     p.add_argument("--inputs", type=int, default=4096, help="batch4096: number of inputs")
     p.add_argument("--lambdas", default="2000,8000,32000", help="serve: Poisson arrival rates (req/s)")
     p.add_argument("--serve-seconds", type=float, default=10.0, help="serve: arrival window per rate")
@@ -259,6 +260,72 @@ def _autotune(run_once, max_runs=8):
     return out
 
 
+def run_decompress(a):
+    """Decompression of the mix workload's .bz2 stream (written once by the
+    batched compressor, untimed) with the compressed bytes resident in HBM;
+    value = raw output GB/s.  The output is checked against the input, and
+    stock libbz2 (Python's bz2, one thread) is timed on the same stream."""
+    import bz2
+    import numpy as np
+    import torch
+    import bzip2gpu_amd as bz
+
+    data, cfg = _workload("mix256", a.mib)
+    n = data.size
+    comp = bz.compress(data.tobytes(), 9, cfg["block_bytes"])
+    dev = torch.device("cuda:0")
+    d_in = torch.from_numpy(np.frombuffer(comp, dtype=np.uint8).copy()).to(dev)
+    torch.cuda.synchronize()
+    for _ in range(max(0, a.warmup)):
+        bz.decompress_device(d_in)
+    torch.cuda.synchronize()
+    if a.probe:
+        bz.probe_kernel(a.probe)
+        bz.probe_read()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ts = time.perf_counter()
+        ptr, nout = bz.decompress_device(d_in)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+    wall = time.perf_counter() - t0
+    probe_ms, probe_n = bz.probe_read() if a.probe else (0.0, 0)
+    bz.probe_kernel(None)
+    stages, nblk, nstr = bz.last_decode_times()
+    value = n / (wall / a.steps) / 1e9
+    res = {
+        "metric": "decompression output GB/s at 1 MI355X (bit-exact, CRC-checked)",
+        "value": round(value, 4), "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": dict(cfg, workload="decompress-mix", compressed_bytes=len(comp), blocks=nblk),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+        "stage_ms": dict(zip(["huffman", "mtf_rle2", "fl_sort", "bwt_walk", "rle1_crc"],
+                             [round(x, 3) for x in stages])),
+    }
+    if probe_n:
+        res["probe"] = {"kernel": a.probe, "launches": probe_n, "avg_launch_us": round(probe_ms / probe_n * 1e3, 2)}
+    if not a.no_check:
+        res["identical_to_input"] = nout == n and _hip_copy_d2h(ptr, nout) == data.tobytes()
+    if not a.no_cpu:
+        # stock libbz2 refuses some blocks of this stream (RLE1 beyond
+        # 900000 bytes under the fixed 900000-byte cut), so it decodes its
+        # own level-9 stream of a 32 MiB sample of the same input
+        sample = data[: min(n, 32 * MiB)].tobytes()
+        stock = bz2.compress(sample, 9)
+        t = time.perf_counter()
+        ref = bz2.decompress(stock)
+        dt = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": round(len(sample) / dt / 1e9, 4), "unit": "GB/s", "cores": 1,
+                               "kind": "libbz2",
+                               "sample": f"Python bz2 (libbz2 1.0.8) decoding the stock level-9 stream of the first "
+                                         f"{len(sample) // MiB} MiB of the input, {dt:.2f} s wall",
+                               "identical": ref == sample}
+    print(json.dumps(res), flush=True)
+
+
 def _hip_copy_d2h(ptr, nbytes):
     import numpy as np
     hip = ctypes.CDLL("libamdhip64.so")
@@ -293,6 +360,8 @@ def main():
         return run_batch(a)
     if a.workload == "serve":
         return run_serve(a)
+    if a.workload == "decompress":
+        return run_decompress(a)
     import numpy as np
     import torch
     import bzip2gpu_amd as bz
