@@ -23,6 +23,9 @@ constexpr int32_t kDecSymTile = 4096;
 // Rows between list-ranking splitters of the inverse BWT.
 constexpr int32_t kDecSplit = 256;
 constexpr int32_t kDecMaxNodes = kDecMaxM / kDecSplit + 2;
+// Bytes a walker keeps of its stretch (the rest, for the few longer
+// stretches, is walked again after ranking).
+constexpr int32_t kDecStage = 1024;
 // Selectors kept per block (enough for kDecMaxM symbols).
 constexpr int32_t kDecSelCap = (kDecMaxM + 49) / 50 + 32;
 
@@ -116,7 +119,7 @@ class Decoder {
   size_t out_len_ = 0;
   int64_t nbits_ = 0;
   Buf w_, cand_, res_, sym_, slot_, idx_, small_, blk_, bres_, stblk_, f_, p_, tlen_, toff_;
-  Buf l_, tt_, mtblk_, hist_, cb_, nodeblk_, nx_, ln_, pos_, t_, tf_, tent_, too_, treg_, out_;
+  Buf l_, tt_, mtblk_, hist_, cb_, nodeblk_, nx_, ln_, pos_, t_, tf_, tent_, too_, treg_, out_, stg_, rrow_;
   int32_t* h_small_ = nullptr;  // pinned
   std::vector<DecCand> cands_;
   std::vector<DecRes> res_h_;

@@ -27,12 +27,15 @@
 //   4. k_dec_hist / k_dec_colscan / k_dec_scatter: a stable counting sort of
 //      L by byte gives the FL map: entry q = (row of the q-th smallest
 //      L byte << 8) | that byte.
-//   5. k_dec_walk1 / k_dec_rank / k_dec_walk2 / k_dec_repl: the RLE1 bytes
-//      are read off the FL map's cycle through origPtr.  Rows that are
-//      multiples of 256 (and origPtr) are splitters; walkers measure the
-//      distance between consecutive splitters, the per-block ranking
+//   5. k_dec_walk1 / k_dec_rank / k_dec_copy / k_dec_walk2 / k_dec_repl: the
+//      RLE1 bytes are read off the FL map's cycle through origPtr.  Rows
+//      that are multiples of 256 (and origPtr) are splitters; a walker per
+//      splitter follows the map to the next splitter, keeping the bytes it
+//      passes (up to 1024) in its staging slot; the per-block ranking
 //      (pointer jumping in LDS) gives each splitter on origPtr's cycle its
-//      position, and the walkers write the bytes of their stretch.  A block
+//      position, the copy pass moves the staged bytes there, and the rare
+//      longer stretches are finished by a second walk from where the first
+//      stopped keeping bytes.  A block
 //      that is a string repeated k times has k cycles of m/k rows; like
 //      stock bzip2, which follows origPtr's cycle for m steps, the output is
 //      that cycle repeated (k_dec_repl).
@@ -848,6 +851,7 @@ __device__ inline uint32_t node_row(const DecBlk& B, int i) {
 __global__ void __launch_bounds__(256) k_dec_walk1(const DecBlk* __restrict__ blk, const int32_t* __restrict__ nodeblk,
                                                    int32_t nnodes, const uint32_t* __restrict__ tt,
                                                    int32_t* __restrict__ nx, int32_t* __restrict__ ln,
+                                                   uint8_t* __restrict__ stg, uint32_t* __restrict__ rrow,
                                                    DecBlkRes* __restrict__ bres) {
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g >= nnodes) return;
@@ -855,12 +859,26 @@ __global__ void __launch_bounds__(256) k_dec_walk1(const DecBlk* __restrict__ bl
   const DecBlk B = blk[b];
   const uint32_t* T = tt + B.m_off;
   const uint32_t orig = (uint32_t)B.orig_ptr;
+  uint64_t* S = reinterpret_cast<uint64_t*>(stg + (int64_t)g * kDecStage);
   uint32_t r = node_row(B, g - B.node0);
   int len = 0;
-  do {
-    r = T[r] >> 8;
+  uint64_t acc = 0;
+  for (;;) {
+    if (len == kDecStage) rrow[g] = r;
+    const uint32_t e = T[r];
+    if (len < kDecStage) {
+      acc |= (uint64_t)(e & 255u) << (8 * (len & 7));
+      if ((len & 7) == 7) {
+        S[len >> 3] = acc;
+        acc = 0;
+      }
+    }
+    r = e >> 8;
     len++;
-  } while (r % kDecSplit != 0 && r != orig && len <= B.m);
+    if (r % kDecSplit == 0 || r == orig || len > B.m) break;
+  }
+  const int kept = min(len, kDecStage);
+  if (kept & 7) S[kept >> 3] = acc;
   if (len > B.m) atomicOr(&bres[b].err, 16);
   nx[g] = r == orig ? origin_node(B) : (int32_t)(r / kDecSplit);
   ln[g] = len;
@@ -918,21 +936,50 @@ __global__ void __launch_bounds__(512) k_dec_rank(const DecBlk* __restrict__ blk
   }
 }
 
+// Staged bytes to their place: 16 bytes per thread, kDecStage / 16 threads
+// per walker.
+__global__ void __launch_bounds__(256) k_dec_copy(const DecBlk* __restrict__ blk, const int32_t* __restrict__ nodeblk,
+                                                  int32_t nnodes, const uint8_t* __restrict__ stg,
+                                                  const int32_t* __restrict__ pos, const int32_t* __restrict__ ln,
+                                                  uint8_t* __restrict__ T, const DecBlkRes* __restrict__ bres) {
+  constexpr int kChunks = kDecStage / 16;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int g = (int)(t / kChunks), ch = (int)(t % kChunks);
+  if (g >= nnodes) return;
+  const int p = pos[g], len = ln[g];
+  const int n = min(len, kDecStage);
+  if (p < 0 || ch * 16 >= n) return;
+  const int b = nodeblk[g];
+  if (bres[b].err || p + len > bres[b].cyc) return;
+  const uint4 v = reinterpret_cast<const uint4*>(stg + (int64_t)g * kDecStage)[ch];
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+  uint8_t* dst = T + blk[b].m_off + p + ch * 16;
+  const int c = min(16, n - ch * 16);
+#pragma unroll
+  for (int i = 0; i < 16; i++)
+    if (i < c) dst[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+}
+
+// Stretches longer than kDecStage: the bytes past the staged ones, walked
+// again from the row the first walk reached there.
 __global__ void __launch_bounds__(256) k_dec_walk2(const DecBlk* __restrict__ blk, const int32_t* __restrict__ nodeblk,
                                                    int32_t nnodes, const uint32_t* __restrict__ tt,
                                                    const int32_t* __restrict__ pos, const int32_t* __restrict__ ln,
-                                                   uint8_t* __restrict__ T, const DecBlkRes* __restrict__ bres) {
+                                                   const uint32_t* __restrict__ rrow, uint8_t* __restrict__ T,
+                                                   const DecBlkRes* __restrict__ bres) {
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g >= nnodes) return;
+  const int len = ln[g];
+  if (len <= kDecStage) return;
   const int b = nodeblk[g];
   if (bres[b].err) return;
   const DecBlk B = blk[b];
   const uint32_t* M = tt + B.m_off;
-  const int p = pos[g], len = ln[g];
-  if (p < 0 || len < 1 || p + len > bres[b].cyc) return;
+  const int p = pos[g];
+  if (p < 0 || p + len > bres[b].cyc) return;
   uint8_t* dst = T + B.m_off + p;
-  uint32_t r = node_row(B, g - B.node0);
-  for (int k = 0; k < len; k++) {
+  uint32_t r = rrow[g];
+  for (int k = kDecStage; k < len; k++) {
     const uint32_t e = M[r];
     dst[k] = (uint8_t)e;
     r = e >> 8;
@@ -1242,7 +1289,8 @@ Decoder::Decoder(int device) : device_(device) {
 Decoder::~Decoder() {
   (void)hipSetDevice(device_);
   Buf* bufs[] = {&w_, &cand_, &res_, &sym_, &slot_, &idx_, &small_, &blk_, &bres_, &stblk_, &f_, &p_, &tlen_, &toff_,
-                 &l_, &tt_, &mtblk_, &hist_, &cb_, &nodeblk_, &nx_, &ln_, &pos_, &t_, &tf_, &tent_, &too_, &treg_, &out_};
+                 &l_, &tt_, &mtblk_, &hist_, &cb_, &nodeblk_, &nx_, &ln_, &pos_, &t_, &tf_, &tent_, &too_, &treg_, &out_,
+                 &stg_, &rrow_};
   for (Buf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   if (h_small_) (void)hipHostFree(h_small_);
@@ -1504,6 +1552,8 @@ int Decoder::run(const uint8_t* d_in, size_t len, hipStream_t s, DecodeTimes* tm
   DEC_NEED(nx_, (size_t)nnodes * 4);
   DEC_NEED(ln_, (size_t)nnodes * 4);
   DEC_NEED(pos_, (size_t)nnodes * 4);
+  DEC_NEED(stg_, (size_t)nnodes * kDecStage);
+  DEC_NEED(rrow_, (size_t)nnodes * 4);
   DEC_NEED(tf_, (size_t)nmt * 5 * 4);
   DEC_NEED(tent_, (size_t)nmt * 4);
   DEC_NEED(too_, (size_t)nmt * 8);
@@ -1524,12 +1574,16 @@ int Decoder::run(const uint8_t* d_in, size_t len, hipStream_t s, DecodeTimes* tm
   DEC_CK(hipEventRecord(ev_[3], s));
   // stage 5: inverse BWT
   DEC_LAUNCH(k_dec_walk1, dim3(grid_for(nnodes, 256)), dim3(256), 0, s, dB, (const int32_t*)nodeblk_.p, nnodes,
-                     (const uint32_t*)tt_.p, (int32_t*)nx_.p, (int32_t*)ln_.p, (DecBlkRes*)bres_.p);
+                     (const uint32_t*)tt_.p, (int32_t*)nx_.p, (int32_t*)ln_.p, (uint8_t*)stg_.p, (uint32_t*)rrow_.p,
+                     (DecBlkRes*)bres_.p);
   DEC_LAUNCH(k_dec_rank, dim3(nb), dim3(512), 0, s, dB, (const int32_t*)nx_.p, (const int32_t*)ln_.p,
                      (int32_t*)pos_.p, (DecBlkRes*)bres_.p);
+  DEC_LAUNCH(k_dec_copy, dim3(grid_for((int64_t)nnodes * (kDecStage / 16), 256)), dim3(256), 0, s, dB,
+             (const int32_t*)nodeblk_.p, nnodes, (const uint8_t*)stg_.p, (const int32_t*)pos_.p,
+             (const int32_t*)ln_.p, (uint8_t*)t_.p, (const DecBlkRes*)bres_.p);
   DEC_LAUNCH(k_dec_walk2, dim3(grid_for(nnodes, 256)), dim3(256), 0, s, dB, (const int32_t*)nodeblk_.p, nnodes,
-                     (const uint32_t*)tt_.p, (const int32_t*)pos_.p, (const int32_t*)ln_.p, (uint8_t*)t_.p,
-                     (const DecBlkRes*)bres_.p);
+             (const uint32_t*)tt_.p, (const int32_t*)pos_.p, (const int32_t*)ln_.p, (const uint32_t*)rrow_.p,
+             (uint8_t*)t_.p, (const DecBlkRes*)bres_.p);
   DEC_LAUNCH(k_dec_repl, dim3(nmt), dim3(256), 0, s, dB, (const int32_t*)mtblk_.p, (uint8_t*)t_.p,
                      (const DecBlkRes*)bres_.p);
   DEC_CK(hipEventRecord(ev_[4], s));
