@@ -148,3 +148,21 @@ void bzip2_gpu_compress(BZFileInputStream &in_stream, int n, std::vector<uint8_t
   check(bzg_compress_buffer(all.data(), total, n, bb, out.data() + base, cap, &got), "bzip2_gpu_compress");
   out.resize(base + got);
 }
+
+int bzip2_gpu_decompress(const uint8_t *in, size_t len, std::vector<uint8_t> &out, std::string *err) {
+  size_t cap = len * 4 > (size_t)(1 << 20) ? len * 4 : (size_t)(1 << 20);
+  out.resize(cap);
+  size_t got = 0;
+  bzg_status st = bzg_decompress_buffer(in, len, out.data(), out.size(), &got);
+  if (st == BZG_ERR_CAPACITY) {
+    out.resize(got);
+    st = bzg_decompress_fetch(out.data(), out.size(), &got);
+  }
+  if (st != BZG_OK) {
+    if (err) *err = bzg_last_error();
+    out.clear();
+    return st == BZG_ERR_DATA ? 2 : 3;
+  }
+  out.resize(got);
+  return 0;
+}

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
 #include <vector>
 
 #include "io.h"
@@ -43,5 +44,12 @@ int huffman_encode(uint16_t *dev_data_in, int data_in_len, int alphabet_size,
                    uint32_t *&dev_encoded_data, uint8_t len[max_n_groups][max_alphabet_size],
                    int32_t code[max_n_groups][max_alphabet_size], uint8_t *dev_selectors,
                    int32_t num_selectors, int n_groups, hipStream_t stream);
+
+// Extension with no counterpart in the reference, whose decompression is not
+// built (TODO.md:28-34): decodes the concatenated .bz2 streams in[0, len) on
+// the device into `out` (replaced).  Returns 0; 2 for malformed or corrupt
+// input (bad header, CRC mismatch, truncation); 3 for a device failure.  The
+// message of a failure goes to *err when given.  Does not exit on errors.
+int bzip2_gpu_decompress(const uint8_t *in, size_t len, std::vector<uint8_t> &out, std::string *err = nullptr);
 
 #endif  // BZG_COMPRESSION_H
