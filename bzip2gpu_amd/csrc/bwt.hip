@@ -773,15 +773,36 @@ __global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
   // stage the tile's keys (with the row before and after) and SA values in
   // LDS; row e sits at e + e/16 (one word of skew per 16 rows), so the
   // 16-row runs the threads own below are read without bank conflicts
-  for (int e = tid - 1; e <= n; e += 256) {
-    const int p = pos + e;
-    const int ix = e + 1;
-    sk[ix + (ix >> 4)] = (p >= 0 && p < g.len) ? K[base + p] : 0u;
-  }
-  for (int e = tid; e < n; e += 256) {
-    const uint32_t v = S[base + pos + e];
-    ss[e + (e >> 4)] = v;
-    if (fin) d.SA[base + pos + e] = v;
+  // (all loads are issued before the LDS writes so that their latencies overlap)
+  {
+    uint32_t kv[17], sv[16];
+#pragma unroll
+    for (int q = 0; q < 17; q++) {
+      const int e = tid - 1 + 256 * q;
+      const int p = pos + e;
+      kv[q] = (e <= n && p >= 0 && p < g.len) ? K[base + p] : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int e = tid + 256 * q;
+      sv[q] = e < n ? S[base + pos + e] : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < 17; q++) {
+      const int e = tid - 1 + 256 * q;
+      if (e <= n) {
+        const int ix = e + 1;
+        sk[ix + (ix >> 4)] = kv[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int e = tid + 256 * q;
+      if (e < n) {
+        ss[e + (e >> 4)] = sv[q];
+        if (fin) d.SA[base + pos + e] = sv[q];
+      }
+    }
   }
   __syncthreads();
   // thread owns rows [16*tid, 16*tid+16) of the tile
