@@ -295,14 +295,56 @@ __global__ void __launch_bounds__(256) k_keys(Dev d, int cur, int round, int P, 
     const int total = L.total;
     for (int j = threadIdx.x; j < L.count; j += 256)
       if (j == 0 || L.blk[j] != L.blk[j - 1]) atomicMax(&d.st[L.blk[j]].rounds, round + 1);
-    for (int e = threadIdx.x; e < total; e += 256) {
-      int s = find_seg(L, e);
-      int b = L.blk[s];
-      const BlockDesc& D = d.desc[b];
-      int m = d.st[b].m;
-      int j = L.start[s] + (e - L.lofs[s]);
-      d.KEY[D.off + j] = make_key(d, D.off, m, j, round, P);
+    if (round == 0) {
+      for (int e = threadIdx.x; e < total; e += 256) {
+        int s = find_seg(L, e);
+        int b = L.blk[s];
+        const BlockDesc& D = d.desc[b];
+        int m = d.st[b].m;
+        int j = L.start[s] + (e - L.lofs[s]);
+        d.KEY[D.off + j] = make_key(d, D.off, m, j, round, P);
+      }
+      return;
     }
+    // rows' segment and block first (block data reloaded only when the block
+    // changes), then all SA loads, then all rank loads, then the stores
+    int jj[kPack / 256], mm[kPack / 256];
+    uint64_t oo[kPack / 256];
+    int cb = -1, cm = 0;
+    uint64_t coff = 0;
+#pragma unroll
+    for (int q = 0; q < kPack / 256; q++) {
+      const int e = threadIdx.x + 256 * q;
+      jj[q] = -1;
+      mm[q] = 1;
+      oo[q] = 0;
+      if (e < total) {
+        const int s = find_seg(L, e);
+        const int b = L.blk[s];
+        if (b != cb) {
+          cb = b;
+          coff = d.desc[b].off;
+          cm = d.st[b].m;
+        }
+        jj[q] = L.start[s] + (e - L.lofs[s]);
+        oo[q] = coff;
+        mm[q] = cm;
+      }
+    }
+    uint32_t v[kPack / 256];
+#pragma unroll
+    for (int q = 0; q < kPack / 256; q++) v[q] = jj[q] >= 0 ? d.SA[oo[q] + jj[q]] & kSAMask : 0u;
+#pragma unroll
+    for (int q = 0; q < kPack / 256; q++) {
+      if (jj[q] >= 0 && P < mm[q]) {
+        uint32_t p = v[q] + (uint32_t)P;
+        if (p >= (uint32_t)mm[q]) p -= (uint32_t)mm[q];
+        v[q] = d.RK[oo[q] + p];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kPack / 256; q++)
+      if (jj[q] >= 0) d.KEY[oo[q] + jj[q]] = v[q];
   } else {
     const int t = w - npacks;
     const int si = d.ltile_seg[t];
@@ -312,9 +354,36 @@ __global__ void __launch_bounds__(256) k_keys(Dev d, int cur, int round, int P, 
     const int m = d.st[g.blk].m;
     if (pos == 0 && threadIdx.x == 0) atomicMax(&d.st[g.blk].rounds, round + 1);
     const int n = min(kTile, g.len - pos);
-    for (int e = threadIdx.x; e < n; e += 256) {
-      int j = g.start + pos + e;
-      d.KEY[off + j] = make_key(d, off, m, j, round, P);
+    if (round == 0) {
+      for (int e = threadIdx.x; e < n; e += 256) {
+        int j = g.start + pos + e;
+        d.KEY[off + j] = make_key(d, off, m, j, round, P);
+      }
+      return;
+    }
+    // all SA loads, then all rank loads, then the stores
+    const int j0 = g.start + pos;
+    uint32_t v[kTile / 256];
+#pragma unroll
+    for (int q = 0; q < kTile / 256; q++) {
+      const int e = threadIdx.x + 256 * q;
+      v[q] = e < n ? d.SA[off + j0 + e] & kSAMask : 0u;
+    }
+    if (P < m) {
+#pragma unroll
+      for (int q = 0; q < kTile / 256; q++) {
+        const int e = threadIdx.x + 256 * q;
+        if (e < n) {
+          uint32_t p = v[q] + (uint32_t)P;
+          if (p >= (uint32_t)m) p -= (uint32_t)m;
+          v[q] = d.RK[off + p];
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kTile / 256; q++) {
+      const int e = threadIdx.x + 256 * q;
+      if (e < n) d.KEY[off + j0 + e] = v[q];
     }
   }
 }
@@ -395,13 +464,38 @@ __device__ inline void sort_small_body(const Dev& d, int cur, int keybits, int k
     }
   }
   __syncthreads();
-  // composite keys and SA values
-  for (int e = tid; e < total; e += 256) {
-    const int sg = sid[e];
-    const uint64_t off = d.desc[L.blk[sg]].off;
-    const int j = L.start[sg] + (e - L.lofs[sg]);
-    K0[e] = ((KT)sg << keybits) | (KT)d.KEY[off + j];
-    V0[e] = d.SA[off + j];
+  // composite keys and SA values (a pack's groups come from few blocks: the
+  // block offset is reloaded only when the block changes, and the rows'
+  // loads are all in flight before the LDS writes)
+  {
+    int cb = -1;
+    uint64_t coff = 0;
+    uint32_t kv[kPack / 256], sv[kPack / 256];
+#pragma unroll
+    for (int q = 0; q < kPack / 256; q++) {
+      const int e = tid + 256 * q;
+      kv[q] = 0;
+      sv[q] = 0;
+      if (e < total) {
+        const int sg = sid[e];
+        const int b = L.blk[sg];
+        if (b != cb) {
+          cb = b;
+          coff = d.desc[b].off;
+        }
+        const int j = L.start[sg] + (e - L.lofs[sg]);
+        kv[q] = d.KEY[coff + j];
+        sv[q] = d.SA[coff + j];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kPack / 256; q++) {
+      const int e = tid + 256 * q;
+      if (e < total) {
+        K0[e] = ((KT)sid[e] << keybits) | (KT)kv[q];
+        V0[e] = sv[q];
+      }
+    }
   }
   // largest group of the pack
   {
@@ -415,6 +509,8 @@ __device__ inline void sort_small_body(const Dev& d, int cur, int keybits, int k
     // Small groups only: every row's place in its group by counting (rows
     // with a smaller key, and equal keys before it); equal keys stay together
     // in input order and form the next round's subgroup.
+    int cb = -1;
+    uint64_t coff = 0;
     for (int e = tid; e < total; e += 256) {
       const int sg = sid[e];
       const int gs = L.lofs[sg];
@@ -429,11 +525,14 @@ __device__ inline void sort_small_body(const Dev& d, int cur, int keybits, int k
         eqb += (j < e) ? same : 0;
       }
       const int b = L.blk[sg];
-      const uint64_t off = d.desc[b].off;
+      if (b != cb) {
+        cb = b;
+        coff = d.desc[b].off;
+      }
       const int st0 = L.start[sg];
       const uint32_t v = V0[e];
-      d.SA[off + st0 + lt + eqb] = v;
-      d.RK[off + (v & kSAMask)] = (uint32_t)(st0 + lt);
+      d.SA[coff + st0 + lt + eqb] = v;
+      d.RK[coff + (v & kSAMask)] = (uint32_t)(st0 + lt);
       V1[e] = (eq >= 2 && eqb == 0) ? (((uint32_t)lt << 16) | (uint32_t)eq) : 0u;
     }
     __syncthreads();
@@ -534,13 +633,19 @@ __device__ inline void sort_small_body(const Dev& d, int cur, int keybits, int k
   }
   uint32_t endm = 0;
   int nout = 0;
+  int cb = -1;
+  uint64_t coff = 0;
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     int e = e0 + q;
     if (e < total) {
       int sg = (int)(kk[q] >> keybits);
       int b = L.blk[sg];
-      uint64_t off = d.desc[b].off;
+      if (b != cb) {
+        cb = b;
+        coff = d.desc[b].off;
+      }
+      const uint64_t off = coff;
       int j = L.start[sg] + (e - L.lofs[sg]);
       int hj = L.start[sg] + (head_of[q] - L.lofs[sg]);
       uint32_t v = Vs[e];
