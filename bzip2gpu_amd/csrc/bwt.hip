@@ -1024,12 +1024,12 @@ __global__ void __launch_bounds__(256) k_rk_apply(Dev d) {
   for (int i = threadIdx.x; i < kRkBucket; i += 256) touched[i] = 0;
   __syncthreads();
   const uint64_t* src = d.RKB + D.off + base;
-  for (int i0 = threadIdx.x; i0 < cnt; i0 += 1024) {
-    uint64_t p[4];
+  for (int i0 = threadIdx.x; i0 < cnt; i0 += 4096) {
+    uint64_t p[16];  // 16 loads in flight per thread
 #pragma unroll
-    for (int q = 0; q < 4; q++) p[q] = i0 + 256 * q < cnt ? src[i0 + 256 * q] : ~0ull;
+    for (int q = 0; q < 16; q++) p[q] = i0 + 256 * q < cnt ? src[i0 + 256 * q] : ~0ull;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < 16; q++) {
       if (p[q] != ~0ull) {
         const uint32_t o = (uint32_t)p[q] - base;
         win[o] = (uint32_t)(p[q] >> 32);

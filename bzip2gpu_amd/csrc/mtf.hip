@@ -40,13 +40,20 @@ __global__ void __launch_bounds__(256) k_mtf_gather(Dev d) {
   __syncthreads();
   const int n = min(kTile, m - start);
   const uint64_t above = ~((2ull << lane) - 1ull);
+  uint32_t sv[16];  // all 16 loads in flight before the first use
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int e = wv * 1024 + q * 64 + lane;
+    sv[q] = e < n ? d.SA[D.off + start + e] : 0u;
+  }
+#pragma unroll
   for (int q = 0; q < 16; q++) {
     const int e = wv * 1024 + q * 64 + lane;
     const bool valid = e < n;
     const int i = start + e;
     uint32_t c = 0;
     if (valid) {
-      const uint32_t v = d.SA[D.off + i];
+      const uint32_t v = sv[q];
       c = v >> 24;  // T[(start - 1) mod m], carried by the sort
       d.MTF[D.off + i] = (uint8_t)c;
       if ((v & kSAMask) == 0) d.st[b].orig_ptr = i;
