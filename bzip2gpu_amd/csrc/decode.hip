@@ -695,9 +695,19 @@ __global__ void __launch_bounds__(256) k_dec_mtf2(const DecBlk* __restrict__ blk
   const int i0 = (t - B.st0) * kDecSymTile;
   const int n = min(kDecSymTile, B.s - i0);
   const uint16_t* S = sym + B.sym_off;
-  for (int e = tid; e < n + 32; e += 256) {
-    const int j = i0 - 32 + e;
-    ss[e] = j >= 0 ? S[j] : (uint16_t)0xFFFF;
+  {
+    uint16_t v[17];  // all loads in flight before the LDS writes
+#pragma unroll
+    for (int q = 0; q < 17; q++) {
+      const int e = tid + 256 * q;
+      const int j = i0 - 32 + e;
+      v[q] = (e < n + 32 && j >= 0) ? S[j] : (uint16_t)0xFFFF;
+    }
+#pragma unroll
+    for (int q = 0; q < 17; q++) {
+      const int e = tid + 256 * q;
+      if (e < n + 32) ss[e] = v[q];
+    }
   }
   pl[tid] = P[(int64_t)t * 256 + tid];
   __syncthreads();
@@ -750,10 +760,17 @@ __global__ void __launch_bounds__(256) k_dec_hist(const DecBlk* __restrict__ blk
   const int n = min(kTile, B.m - p0);
   for (int i = tid; i < 1024; i += 256) (&h[0][0])[i] = 0;
   __syncthreads();
-  const uint8_t* src = L + B.m_off + p0;
+  const uint8_t* src = L + B.m_off + p0;  // 16-byte aligned
   const int w = tid >> 6;
-#pragma unroll 4
-  for (int e = tid; e < n; e += 256) atomicAdd(&h[w][src[e]], 1u);
+  const int q0 = tid * 16;
+  if (q0 + 16 <= n) {
+    const uint4 v = *reinterpret_cast<const uint4*>(src + q0);
+    const uint32_t x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 16; q++) atomicAdd(&h[w][(x[q >> 2] >> (8 * (q & 3))) & 255u], 1u);
+  } else {
+    for (int e = q0; e < n; e++) atomicAdd(&h[w][src[e]], 1u);
+  }
   __syncthreads();
   hist[(int64_t)t * 256 + tid] = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
 }
