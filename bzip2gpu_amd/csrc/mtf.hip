@@ -93,22 +93,80 @@ __global__ void __launch_bounds__(256) k_mtf_scan(Dev d) {
 }
 
 // Transpose of a 64x64 bit matrix held one row per lane (bit c of lane r is
-// element (r, c)): six stages that swap the off-diagonal s x s blocks.
+// element (r, c)), as two 32-bit words per lane.  Stage s (32, 16, ..., 1)
+// swaps the off-diagonal s x s blocks: lane r trades with lane r ^ s the bits
+// whose column index differs from r in bit s.  The lane exchanges use
+// gfx950's cross-lane VALU paths, no LDS: v_permlane32_swap (s = 32),
+// v_permlane16_swap (s = 16), DPP row rotate (s = 8), DPP row shifts (s = 4)
+// and DPP quad permutes (s = 2, 1).
+namespace {
+
+constexpr int kDppQuadXor1 = 0xB1;  // quad_perm [1,0,3,2]
+constexpr int kDppQuadXor2 = 0x4E;  // quad_perm [2,3,0,1]
+constexpr int kDppRowShl4 = 0x104;
+constexpr int kDppRowShr4 = 0x114;
+constexpr int kDppRowRor8 = 0x128;
+
+// word of lane ^ S (S < 16)
+template <int S>
+__device__ inline uint32_t xor_partner(uint32_t w) {
+  if constexpr (S == 1) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)w, kDppQuadXor1, 0xF, 0xF, false);
+  } else if constexpr (S == 2) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)w, kDppQuadXor2, 0xF, 0xF, false);
+  } else if constexpr (S == 4) {
+    // lanes 0-3 and 8-11 of each row read lane + 4, the others lane - 4
+    const int up = __builtin_amdgcn_mov_dpp((int)w, kDppRowShl4, 0xF, 0x5, false);
+    return (uint32_t)__builtin_amdgcn_update_dpp(up, (int)w, kDppRowShr4, 0xF, 0xA, false);
+  } else {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)w, kDppRowRor8, 0xF, 0xF, false);
+  }
+}
+
+// Stages s = 1, 2, 4 on one word.  A lane with bit s clear keeps the low s-bit
+// halves of each 2s-bit field and takes the partner's low halves into the
+// high ones; a lane with bit s set keeps the high halves and takes the
+// partner's high halves into the low ones.
+template <int S>
+__device__ inline uint32_t tstage_bits(uint32_t w, bool clear) {
+  constexpr uint32_t hi = S == 1 ? 0xAAAAAAAAu : S == 2 ? 0xCCCCCCCCu : 0xF0F0F0F0u;
+  const uint32_t p = xor_partner<S>(w);
+  const uint32_t t = __builtin_amdgcn_alignbit(p, p, clear ? 32 - S : S);  // rotate
+  const uint32_t keep = clear ? ~hi : hi;
+  return (w & keep) | (t & ~keep);
+}
+
 __device__ inline uint64_t bit_transpose64(uint64_t x) {
   const int lane = lane_id();
-  const uint64_t hi[6] = {0xFFFFFFFF00000000ull, 0xFFFF0000FFFF0000ull, 0xFF00FF00FF00FF00ull,
-                          0xF0F0F0F0F0F0F0F0ull, 0xCCCCCCCCCCCCCCCCull, 0xAAAAAAAAAAAAAAAAull};
-#pragma unroll
-  for (int k = 0; k < 6; k++) {
-    const int sft = 32 >> k;
-    uint32_t lo32 = __shfl_xor((uint32_t)x, sft, 64);
-    uint32_t hi32 = __shfl_xor((uint32_t)(x >> 32), sft, 64);
-    uint64_t p = ((uint64_t)hi32 << 32) | lo32;
-    if (lane & sft) x = (x & hi[k]) | ((p >> sft) & ~hi[k]);
-    else x = (x & ~hi[k]) | ((p << sft) & hi[k]);
+  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  {  // s = 32: lanes 0-31 take the low words of lanes 32-63 as high words
+    const auto r = __builtin_amdgcn_permlane32_swap(lo, hi, false, false);
+    lo = r[0];
+    hi = r[1];
   }
-  return x;
+  {  // s = 16: 16-bit fields; after the swap a[0] holds the own word in rows 0, 2
+     // and the partner's in rows 1, 3, a[1] the reverse
+    const uint32_t sel = (lane & 16) ? 0x07060302u : 0x05040100u;
+    const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    lo = __builtin_amdgcn_perm(a[1], a[0], sel);
+    hi = __builtin_amdgcn_perm(b[1], b[0], sel);
+  }
+  {  // s = 8: bytes
+    const uint32_t sel = (lane & 8) ? 0x03070105u : 0x06020400u;
+    lo = __builtin_amdgcn_perm(xor_partner<8>(lo), lo, sel);
+    hi = __builtin_amdgcn_perm(xor_partner<8>(hi), hi, sel);
+  }
+  lo = tstage_bits<4>(lo, !(lane & 4));
+  hi = tstage_bits<4>(hi, !(lane & 4));
+  lo = tstage_bits<2>(lo, !(lane & 2));
+  hi = tstage_bits<2>(hi, !(lane & 2));
+  lo = tstage_bits<1>(lo, !(lane & 1));
+  hi = tstage_bits<1>(hi, !(lane & 1));
+  return ((uint64_t)hi << 32) | lo;
 }
+
+}  // namespace
 
 // One wave per segment: the MTF proper.  The segment is staged in LDS and
 // the list is kept as s2p[] (list position of every present byte, one byte
@@ -123,7 +181,7 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
   __shared__ __align__(4) uint8_t pflag[256];  // old positions of this step's bytes
   __shared__ __align__(4) uint8_t sflag[256];  // this step's bytes
   __shared__ __align__(4) uint8_t rank[256];
-  __shared__ uint8_t newpos[256];
+  __shared__ __align__(4) uint8_t newpos[256];
   __shared__ unsigned long long pm[256];  // per-byte lane masks of the current step
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int b = d.tc_blk[t];
@@ -197,20 +255,26 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
     // behind this one, by a bit-serial compare of the positions (MSB first).
     const uint64_t fm = ballot(first);
     const int K = __popcll(fm);
-    uint64_t gt = 0, eq = fm;
+    // gt: earlier first lanes with a larger position, eq: those equal so far
+    // (MSB first); nm is all ones where this lane's bit is clear
+    uint32_t gl = 0, gh = 0;
+    uint32_t el = (uint32_t)(fm & below), eh = (uint32_t)((fm & below) >> 32);
 #pragma unroll
     for (int bb = 7; bb >= 0; bb--) {
-      const bool mb = (mypos >> bb) & 1;
-      const uint64_t B = ballot(first && mb);
-      gt = mb ? gt : (gt | (eq & B));
-      eq = mb ? (eq & B) : (eq & ~B);
+      const uint32_t nm = (uint32_t)__builtin_amdgcn_sbfe(~mypos, bb, 1);
+      const uint64_t B = ballot(nm == 0) & fm;
+      const uint32_t bl = (uint32_t)B, bh = (uint32_t)(B >> 32);
+      gl |= el & bl & nm;
+      gh |= eh & bh & nm;
+      el &= bl ^ nm;
+      eh &= bh ^ nm;
     }
     int out;
     if (has_prev) {
       const uint64_t between = below & ~((2ull << prev) - 1ull);
       out = __popcll(col & between);
     } else {
-      out = mypos + __popcll(gt & below);
+      out = mypos + __popc(gl) + __popc(gh);
     }
     if (valid) buf[i] = (uint8_t)out;
     const uint64_t nz = ballot(valid && out != 0);
@@ -226,9 +290,10 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
     if (valid) pm[c] = 0ull;  // every lane has read its mask (in program order)
     __syncthreads();
     {
+      // flags below this lane's four positions, from one ballot per byte slot
       const uint32_t f4 = reinterpret_cast<const uint32_t*>(pflag)[lane];
-      const int c4 = (int)(f4 & 1u) + (int)((f4 >> 8) & 1u) + (int)((f4 >> 16) & 1u) + (int)(f4 >> 24);
-      const int ex = wave_incl_sum(c4) - c4;
+      const int ex = mbcnt(ballot(f4 & 1u)) + mbcnt(ballot((f4 >> 8) & 1u)) + mbcnt(ballot((f4 >> 16) & 1u)) +
+                     mbcnt(ballot((f4 >> 24) != 0u));
       const int r1 = ex + (int)(f4 & 1u), r2 = r1 + (int)((f4 >> 8) & 1u), r3 = r2 + (int)((f4 >> 16) & 1u);
       reinterpret_cast<uint32_t*>(rank)[lane] =
           (uint32_t)ex | ((uint32_t)r1 << 8) | ((uint32_t)r2 << 16) | ((uint32_t)r3 << 24);
@@ -237,11 +302,15 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
     {
       const uint32_t sp4 = reinterpret_cast<const uint32_t*>(s2p)[lane];
       const uint32_t sf4 = reinterpret_cast<const uint32_t*>(sflag)[lane];
+      const uint32_t nw4 = reinterpret_cast<const uint32_t*>(newpos)[lane];
+      int rk[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) rk[r] = rank[(sp4 >> (8 * r)) & 255u];
       uint32_t np4 = 0;
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int op = (int)((sp4 >> (8 * r)) & 255u);
-        const int np = ((sf4 >> (8 * r)) & 1u) ? (int)newpos[4 * lane + r] : K + op - (int)rank[op];
+        const int np = ((sf4 >> (8 * r)) & 1u) ? (int)((nw4 >> (8 * r)) & 255u) : K + op - rk[r];
         np4 |= (uint32_t)(np & 255) << (8 * r);
       }
       __syncthreads();
