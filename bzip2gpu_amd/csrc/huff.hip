@@ -109,82 +109,75 @@ __device__ inline void wave_lds_sync() {
 }
 
 // Sift-down of `tmp` from the root (DOWNHEAP(1)); heap holds nHeap entries.
-// Lane l < 62 covers window level k = log2(l+2), index j = l+2-2^k below zz.
-// An entry lies on the min-child path when it and all its window ancestors
-// are the smaller of their sibling pair; the path stops above the first path
-// entry that tmp is strictly lighter than.
-// Window lanes of this lane's ancestors (levels 1..k-1 of its window).
-__device__ inline uint64_t window_ancestors(int lane) {
-  const int k = 31 - __clz(lane + 2);
-  const int j = lane + 2 - (1 << k);
-  uint64_t am = 0;
-  for (int kp = 1; kp < k; kp++) am |= 1ull << ((1 << kp) - 2 + (j >> (k - kp)));
-  return am;
+// Lane l < 62 covers window level k = log2(l+2), index j = l+2-2^k below zz;
+// siblings are lanes 2i, 2i+1.  An entry lies on the min-child path when it
+// and all its window ancestors are the chosen child of their pair; the path
+// stops above the first path entry that tmp is strictly lighter than.
+struct WinLane {
+  int k, j;
+  uint64_t am;  // window lanes of this lane's ancestors (levels 1..k-1)
+};
+
+__device__ inline WinLane window_lane(int lane) {
+  WinLane w;
+  w.k = 31 - __clz(lane + 2);
+  w.j = lane + 2 - (1 << w.k);
+  w.am = 0;
+  for (int kp = 1; kp < w.k; kp++) w.am |= 1ull << ((1 << kp) - 2 + (w.j >> (w.k - kp)));
+  return w;
 }
 
-__device__ inline void heap_down(int2* heap, int nHeap, int2 tmp, int lane, uint64_t am) {
-  const int k = 31 - __clz(lane + 2);
-  const int j = lane + 2 - (1 << k);
-  const bool act = lane < 62;
-  int zz = 1;
-  while (true) {
-    const int p = (zz << k) + j;
-    const int pl = p & ~1;  // left sibling of the pair
-    const bool exl = act && pl <= nHeap, exr = act && pl + 1 <= nHeap;
-    const int4 pr = exl ? *reinterpret_cast<const int4*>(heap + pl)
-                        : make_int4(0x7fffffff, 0, 0x7fffffff, 0);
-    const bool isr = j & 1;
-    const int2 e = isr ? make_int2(pr.z, pr.w) : make_int2(pr.x, pr.y);
-    const bool ex = isr ? exr : exl;
-    // the left child is taken unless the right one exists and is strictly lighter
-    const bool rlight = exr && pr.z < pr.x;
-    const bool win = isr ? rlight : (exl && !rlight);
-    const uint64_t W = __ballot(win);
-    const bool on = win && (W & am) == am;
-    const uint64_t P = __ballot(on);
-    const uint64_t S = P & __ballot(ex && tmp.x < e.x);
-    uint64_t moved;
-    bool done;
+constexpr int kHeapCap = 264;
+constexpr uint64_t kWinAct = (1ull << 62) - 1;
+constexpr uint64_t kWinOdd = 0xAAAAAAAAAAAAAAAAull & kWinAct;
+constexpr uint64_t kWinEven = 0x5555555555555555ull & kWinAct;
+
+__device__ inline void heap_down(int2* heap, int nHeap, int2 tmp, int lane, const WinLane& wl) {
+  int zz = 1, cur;
+  for (;;) {
+    const int p = (zz << wl.k) + wl.j;
+    const int2 e = heap[min(p, kHeapCap - 1)];  // entries past nHeap are masked below
+    const int sib = __builtin_amdgcn_mov_dpp(e.x, 0xB1, 0xF, 0xF, false);  // lane ^ 1
+    const uint64_t E = __ballot(p <= nHeap) & kWinAct;
+    // the right entry is taken when it exists and is strictly lighter
+    const uint64_t R = __ballot(e.x < sib) & E & kWinOdd;
+    const uint64_t W = R | (E & kWinEven & ~(R >> 1));
+    const uint64_t P = __ballot((W & wl.am) == wl.am) & W;
+    const uint64_t S = P & __ballot(tmp.x < e.x);
+    // path entries above the first stop move up; without a stop the sift
+    // goes on below the window's last level if the path reaches it
+    uint64_t moved = P;
+    uint32_t more = (uint32_t)(P >> 30);
     if (S) {
-      moved = P & ((S & (~S + 1)) - 1);  // path entries above the first stop
-      done = true;
-    } else {
-      moved = P;
-      done = (P >> 30) == 0;  // path ended inside the window
+      moved = P & ((S & (~S + 1)) - 1);
+      more = 0;
     }
-    int cur = zz;
-    if (moved) {
-      const int l = 63 - __builtin_clzll(moved);
-      const int kl = 31 - __clz(l + 2);
-      cur = (zz << kl) + (l + 2 - (1 << kl));
-    }
+    cur = moved ? __builtin_amdgcn_readlane(p, 63 - __builtin_clzll(moved)) : zz;
     // the next window reads only below cur, which these writes never touch
-    if ((moved >> lane) & 1ull) heap[p >> 1] = e;
-    if (done) {
-      if (lane == 0) heap[cur] = tmp;
-      wave_lds_sync();
-      return;
-    }
+    if (moved & (1ull << lane)) heap[p >> 1] = e;
+    if (!more) break;
     zz = cur;
   }
+  if (lane == 0) heap[cur] = tmp;
+  wave_lds_sync();
 }
 
 // Sift-up of `tmp` placed at position zz (UPHEAP(zz)): lane q holds the
 // ancestor zz >> (q+1); every ancestor heavier than tmp moves one level down.
 __device__ inline void heap_up(int2* heap, int zz, int2 tmp, int lane) {
+  uint64_t* h = reinterpret_cast<uint64_t*>(heap);  // whole entries, one load each
   const int a = lane < 9 ? (zz >> (lane + 1)) : 0;
-  const bool ex = a >= 1;
-  const int2 e = ex ? heap[a] : make_int2(0, 0);
-  const uint64_t M = __ballot(ex && tmp.x < e.x);
+  const uint64_t e = h[a];
+  const uint64_t M = __ballot(a >= 1 && tmp.x < (int)(uint32_t)e);
   const int c = __builtin_ctzll(~M);
-  if (lane < c) heap[zz >> lane] = e;
+  if (lane < c) h[zz >> lane] = e;
   if (lane == 0) heap[zz >> c] = tmp;
   wave_lds_sync();
 }
 
 // LDS of one table's code-length build.
 struct HeapLds {
-  __align__(16) int2 heap[264];
+  __align__(16) int2 heap[kHeapCap];
   int16_t anc[2][520];  // pointer-jumping ancestors (node ids <= 2*258-1)
   int16_t dep[2][520];
   int32_t leafw[260];
@@ -193,7 +186,7 @@ struct HeapLds {
 // Code lengths of one table from its symbol counts fr[0..alpha) (LDS), by
 // one wave; len[0..alpha) (LDS) receives them.
 __device__ void build_lengths(HeapLds& H, const int32_t* fr, uint8_t* len, int alpha, int lane) {
-  const uint64_t am = window_ancestors(lane);
+  const WinLane wl = window_lane(lane);
   for (int i = lane; i < alpha; i += 64) {
     const int f = fr[i];
     H.leafw[i + 1] = (f == 0 ? 1 : f) << 8;
@@ -210,12 +203,12 @@ __device__ void build_lengths(HeapLds& H, const int32_t* fr, uint8_t* len, int a
       const int2 t1 = H.heap[nHeap];
       wave_lds_sync();
       nHeap--;
-      heap_down(H.heap, nHeap, t1, lane, am);
+      heap_down(H.heap, nHeap, t1, lane, wl);
       const int2 n2 = H.heap[1];
       const int2 t2 = H.heap[nHeap];
       wave_lds_sync();
       nHeap--;
-      heap_down(H.heap, nHeap, t2, lane, am);
+      heap_down(H.heap, nHeap, t2, lane, wl);
       nNodes++;
       if (lane == 0) {
         H.anc[0][n1.y] = (int16_t)nNodes;
