@@ -285,15 +285,26 @@ __global__ void __launch_bounds__(384) k_huff_build(Dev d) {
     }
     for (int q = tid; q < 6 * 260; q += 384) (&rf[0][0])[q] = 0;
     __syncthreads();
+    // every group but a block's last holds kGSize symbols: that path runs
+    // without per-symbol guards
     for (int g = tid; g < nsel; g += 384) {
-      const int gs = g * kGSize, ge = min(gs + kGSize, s);
+      const int gs = g * kGSize;
+      const bool full = gs + kGSize <= s;
+      const int ge = full ? gs + kGSize : s;
       uint32_t w[kGSize / 2];
-#pragma unroll
-      for (int q = 0; q < kGSize / 2; q++) w[q] = gs + 2 * q < ge ? sym2[(gs >> 1) + q] : 0u;
       uint64_t acc = 0;
+      if (full) {
 #pragma unroll
-      for (int q = 0; q < kGSize; q++)
-        if (gs + q < ge) acc += PL[(w[q >> 1] >> (16 * (q & 1))) & 0xFFFFu];
+        for (int q = 0; q < kGSize / 2; q++) w[q] = sym2[(gs >> 1) + q];
+#pragma unroll
+        for (int q = 0; q < kGSize; q++) acc += PL[(w[q >> 1] >> (16 * (q & 1))) & 0xFFFFu];
+      } else {
+#pragma unroll
+        for (int q = 0; q < kGSize / 2; q++) w[q] = gs + 2 * q < ge ? sym2[(gs >> 1) + q] : 0u;
+#pragma unroll
+        for (int q = 0; q < kGSize; q++)
+          if (gs + q < ge) acc += PL[(w[q >> 1] >> (16 * (q & 1))) & 0xFFFFu];
+      }
       uint32_t bc = (uint32_t)(acc & 1023u);
       int bt = 0;
       for (int tt = 1; tt < ng; tt++) {
@@ -301,9 +312,15 @@ __global__ void __launch_bounds__(384) k_huff_build(Dev d) {
         if (c < bc) { bc = c; bt = tt; }
       }
       if (it == 3) d.sel[D.sel_off + g] = (uint8_t)bt;
+      int32_t* rt = rf[bt];
+      if (full) {
 #pragma unroll
-      for (int q = 0; q < kGSize; q++)
-        if (gs + q < ge) atomicAdd(&rf[bt][(w[q >> 1] >> (16 * (q & 1))) & 0xFFFFu], 1);
+        for (int q = 0; q < kGSize; q++) atomicAdd(&rt[(w[q >> 1] >> (16 * (q & 1))) & 0xFFFFu], 1);
+      } else {
+#pragma unroll
+        for (int q = 0; q < kGSize; q++)
+          if (gs + q < ge) atomicAdd(&rt[(w[q >> 1] >> (16 * (q & 1))) & 0xFFFFu], 1);
+      }
     }
     __syncthreads();
     if (wv < ng) build_lengths(H[wv], rf[wv], L[wv], alpha, lane);
