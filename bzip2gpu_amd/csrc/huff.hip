@@ -132,8 +132,22 @@ constexpr uint64_t kWinAct = (1ull << 62) - 1;
 constexpr uint64_t kWinOdd = 0xAAAAAAAAAAAAAAAAull & kWinAct;
 constexpr uint64_t kWinEven = 0x5555555555555555ull & kWinAct;
 
-__device__ inline void heap_down(int2* heap, int nHeap, int2 tmp, int lane, const WinLane& wl) {
+__device__ inline int2 readlane2(int2 v, int l) {
+  return make_int2(__builtin_amdgcn_readlane(v.x, l), __builtin_amdgcn_readlane(v.y, l));
+}
+__device__ inline int2 uniform2(int2 v) {
+  return make_int2(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y));
+}
+
+// Returns the new root; *pre receives the entry at position pre_pos after the
+// sift (pre_pos is the heap's last position: a leaf, so the sift changes it
+// only by ending there).
+__device__ inline int2 heap_down(int2* heap, int nHeap, int2 tmp, int lane, const WinLane& wl, int pre_pos,
+                                 int2* pre) {
+  const int2 pv = uniform2(heap[pre_pos]);  // read with the first window
   int zz = 1, cur;
+  int2 root = tmp;
+  bool first = true;
   for (;;) {
     const int p = (zz << wl.k) + wl.j;
     const int2 e = heap[min(p, kHeapCap - 1)];  // entries past nHeap are masked below
@@ -152,27 +166,40 @@ __device__ inline void heap_down(int2* heap, int nHeap, int2 tmp, int lane, cons
       moved = P & ((S & (~S + 1)) - 1);
       more = 0;
     }
+    if (first) {
+      if (moved & 3ull) root = readlane2(e, __builtin_ctzll(moved));
+      first = false;
+    }
     cur = moved ? __builtin_amdgcn_readlane(p, 63 - __builtin_clzll(moved)) : zz;
     // the next window reads only below cur, which these writes never touch
-    if (moved & (1ull << lane)) heap[p >> 1] = e;
+    if (__builtin_amdgcn_inverse_ballot_w64(moved)) heap[(zz << (wl.k - 1)) + (wl.j >> 1)] = e;
     if (!more) break;
     zz = cur;
   }
   if (lane == 0) heap[cur] = tmp;
   wave_lds_sync();
+  *pre = cur == pre_pos ? tmp : pv;
+  return root;
 }
 
 // Sift-up of `tmp` placed at position zz (UPHEAP(zz)): lane q holds the
 // ancestor zz >> (q+1); every ancestor heavier than tmp moves one level down.
-__device__ inline void heap_up(int2* heap, int zz, int2 tmp, int lane) {
+// With TRACK, returns the new root and *last receives the new entry at zz.
+template <bool TRACK>
+__device__ inline int2 heap_up(int2* heap, int zz, int2 tmp, int lane, int2* last) {
   uint64_t* h = reinterpret_cast<uint64_t*>(heap);  // whole entries, one load each
   const int a = lane < 9 ? (zz >> (lane + 1)) : 0;
-  const uint64_t e = h[a];
-  const uint64_t M = __ballot(a >= 1 && tmp.x < (int)(uint32_t)e);
+  const uint64_t ev = h[a];
+  const int2 e = make_int2((int)(uint32_t)ev, (int)(uint32_t)(ev >> 32));
+  const uint64_t M = __ballot(a >= 1 && tmp.x < e.x);
   const int c = __builtin_ctzll(~M);
-  if (lane < c) h[zz >> lane] = e;
+  if (lane < c) h[zz >> lane] = ev;
   if (lane == 0) heap[zz >> c] = tmp;
   wave_lds_sync();
+  if (!TRACK) return tmp;
+  const int L = 31 - __clz(zz);  // ancestors a_0 .. a_(L-1); a_(L-1) is the root
+  *last = c >= 1 ? readlane2(e, 0) : tmp;
+  return c == L ? tmp : readlane2(e, L - 1);
 }
 
 // LDS of one table's code-length build.
@@ -182,9 +209,86 @@ struct HeapLds {
   int16_t dep[2][520];
   int32_t leafw[260];
 };
+// code_too_long's two queues reuse heap/anc/dep: kQueueCap entries each
+constexpr int kQueueCap = 384;
+static_assert(offsetof(HeapLds, leafw) >= 2 * kQueueCap * 4, "queue scratch overlaps leafw");
+
+// Whether the tree built from H.leafw[1..alpha] is deeper than 17 levels.
+// Any min-priority queue pops the same weight values in the same order
+// (only which of two equal weights comes first depends on the queue), and
+// the depth byte of a merged weight is one more than its deeper input, so the
+// tree height is fixed by the values alone: the leaves are ranked by
+// (weight, id), and a two-queue merge over the sorted leaves and the merged
+// nodes (which come out in non-decreasing order) follows the heap's merges.
+// Each queue is read through a 64-entry register window (one entry per
+// lane) that moves on every 64 pops; merged nodes also enter the window of
+// their queue when it covers them.
+__device__ bool code_too_long(HeapLds& H, int alpha, int lane) {
+  int32_t* S = reinterpret_cast<int32_t*>(&H);  // sorted leaf weights
+  int32_t* C = S + kQueueCap;                   // merged weights, in order
+  uint32_t key[5];
+  int rk[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const int i = lane + 64 * q + 1;
+    key[q] = i <= alpha ? ((uint32_t)(H.leafw[i] >> 8) << 9) | (uint32_t)i : 0xFFFFFFFFu;
+    rk[q] = 0;
+  }
+  // rank of every leaf key among all keys (keys are distinct)
+#pragma unroll
+  for (int qj = 0; qj < 5; qj++) {
+    if (64 * qj >= alpha) break;
+    const int nj = min(64, alpha - 64 * qj);
+    for (int l = 0; l < nj; l++) {
+      const uint32_t kj = (uint32_t)__builtin_amdgcn_readlane((int)key[qj], l);
+#pragma unroll
+      for (int q = 0; q < 5; q++) rk[q] += kj < key[q] ? 1 : 0;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 5; q++)
+    if (lane + 64 * q + 1 <= alpha) S[rk[q]] = (int32_t)(key[q] >> 9) << 8;
+  wave_lds_sync();
+  int i1 = 0, i2 = 0, n2 = 0;
+  int winS = S[lane], winC = 0;
+  int sh = __builtin_amdgcn_readlane(winS, 0), ch = 0;  // queue heads
+  bool deep = false;
+  for (int step = 1; step < alpha; step++) {
+    int ab[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      if (i1 < alpha && (i2 >= n2 || sh <= ch)) {
+        ab[t] = sh;
+        i1++;
+        if ((i1 & 63) == 0) winS = S[i1 + lane];
+        sh = __builtin_amdgcn_readlane(winS, i1 & 63);
+      } else {
+        ab[t] = ch;
+        i2++;
+        if ((i2 & 63) == 0) winC = C[i2 + lane];
+        ch = __builtin_amdgcn_readlane(winC, i2 & 63);
+      }
+    }
+    const int da = ab[0] & 0xff, db = ab[1] & 0xff;
+    const int dn = 1 + (da > db ? da : db);
+    if (dn > 17) {
+      deep = true;
+      break;
+    }
+    const int c = ((ab[0] & 0xffffff00) + (ab[1] & 0xffffff00)) | dn;
+    if (lane == 0) C[n2] = c;
+    if ((n2 >> 6) == (i2 >> 6) && lane == (n2 & 63)) winC = c;
+    if (n2 == i2) ch = c;
+    n2++;
+  }
+  wave_lds_sync();
+  return deep;
+}
 
 // Code lengths of one table from its symbol counts fr[0..alpha) (LDS), by
-// one wave; len[0..alpha) (LDS) receives them.
+// one wave; len[0..alpha) (LDS) receives them.  The weight halving rounds of
+// libbz2 (huffman.cu:110-123) are decided by code_too_long before the one
+// exact build.
 __device__ void build_lengths(HeapLds& H, const int32_t* fr, uint8_t* len, int alpha, int lane) {
   const WinLane wl = window_lane(lane);
   for (int i = lane; i < alpha; i += 64) {
@@ -192,23 +296,33 @@ __device__ void build_lengths(HeapLds& H, const int32_t* fr, uint8_t* len, int a
     H.leafw[i + 1] = (f == 0 ? 1 : f) << 8;
   }
   wave_lds_sync();
+  while (code_too_long(H, alpha, lane)) {
+    for (int i = lane + 1; i <= alpha; i += 64) {
+      int j = H.leafw[i] >> 8;
+      j = 1 + (j / 2);
+      H.leafw[i] = j << 8;
+    }
+    wave_lds_sync();
+  }
   while (true) {
     int nNodes = alpha, nHeap = 0;
-    for (int i = 1; i <= alpha; i++) {
-      nHeap++;
-      heap_up(H.heap, nHeap, make_int2(H.leafw[i], i), lane);
+    for (int i0 = 1; i0 <= alpha; i0 += 64) {
+      const int lw = i0 + lane <= alpha ? H.leafw[i0 + lane] : 0;
+      const int n = min(64, alpha + 1 - i0);
+      for (int q = 0; q < n; q++) {
+        nHeap++;
+        heap_up<false>(H.heap, nHeap, make_int2(__builtin_amdgcn_readlane(lw, q), i0 + q), lane, nullptr);
+      }
     }
+    int2 root = uniform2(H.heap[1]), last = uniform2(H.heap[nHeap]);
     while (nHeap > 1) {
-      const int2 n1 = H.heap[1];
-      const int2 t1 = H.heap[nHeap];
-      wave_lds_sync();
+      const int2 n1 = root;
       nHeap--;
-      heap_down(H.heap, nHeap, t1, lane, wl);
-      const int2 n2 = H.heap[1];
-      const int2 t2 = H.heap[nHeap];
-      wave_lds_sync();
+      int2 t2;
+      const int2 n2 = heap_down(H.heap, nHeap, last, lane, wl, nHeap, &t2);
       nHeap--;
-      heap_down(H.heap, nHeap, t2, lane, wl);
+      int2 unused;
+      heap_down(H.heap, nHeap, t2, lane, wl, 0, &unused);
       nNodes++;
       if (lane == 0) {
         H.anc[0][n1.y] = (int16_t)nNodes;
@@ -218,7 +332,7 @@ __device__ void build_lengths(HeapLds& H, const int32_t* fr, uint8_t* len, int a
       const int d1 = w1 & 0xff, d2 = w2 & 0xff;
       const int2 nn = make_int2(((w1 & 0xffffff00) + (w2 & 0xffffff00)) | (1 + (d1 > d2 ? d1 : d2)), nNodes);
       nHeap++;
-      heap_up(H.heap, nHeap, nn, lane);
+      root = heap_up<true>(H.heap, nHeap, nn, lane, &last);
     }
     // depth of every node by pointer jumping (root: ancestor itself, depth 0)
     for (int v = lane + 1; v <= nNodes; v += 64) {
