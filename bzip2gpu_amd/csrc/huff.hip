@@ -655,9 +655,24 @@ __global__ void __launch_bounds__(256) k_group_scan(Dev d) {
   for (int64_t q = threadIdx.x; q < nw; q += 256) d.pay[D.w_off + q] = 0;
 }
 
+// Payload bits of one capacity tile's selector groups (huffman_encode.cu:
+// 15-71): each thread takes up to kPackRun consecutive symbols, a workgroup
+// scan of their code lengths gives every symbol's bit offset, the codes are
+// ORed into the tile's words staged in LDS, and the words go out coalesced
+// (plain stores inside the tile's bit range, atomicOr on the two edge words
+// it shares with the neighbouring tiles).
+constexpr int kPackGroups = kTile / kGSize + 2;  // selector groups a tile can start
+constexpr int kPackSyms = kPackGroups * kGSize;
+constexpr int kPackRun = (kPackSyms + 255) / 256;
+constexpr int kPackWords = kPackSyms * 17 / 32 + 2;
+
 __global__ void __launch_bounds__(256) k_pack(Dev d) {
   __shared__ uint8_t L[6][260];
   __shared__ uint32_t C[6][260];
+  __shared__ uint16_t S[kPackSyms];
+  __shared__ uint8_t T[kPackGroups];
+  __shared__ uint32_t W[kPackWords];
+  __shared__ int32_t tmp[8];
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int b = d.tc_blk[t];
   const BlockDesc D = d.desc[b];
@@ -667,29 +682,59 @@ __global__ void __launch_bounds__(256) k_pack(Dev d) {
   const int nsel = (s + kGSize - 1) / kGSize;
   const int g1 = min(((k + 1) * kTile + kGSize - 1) / kGSize, nsel);
   if (g0 >= g1) return;
-  for (int q = threadIdx.x; q < 6 * 260; q += 256) {
+  const int tid = threadIdx.x;
+  for (int q = tid; q < 6 * 260; q += 256) {
     (&L[0][0])[q] = d.len[(int64_t)b * 6 * 260 + q];
     (&C[0][0])[q] = d.code[(int64_t)b * 6 * 260 + q];
   }
+  const int sbase = g0 * kGSize, n = min(g1 * kGSize, s) - sbase;
+  for (int i = tid; i < n; i += 256) S[i] = d.SYM[D.off + sbase + i];
+  for (int g = tid; g < g1 - g0; g += 256) T[g] = d.sel[D.sel_off + g0 + g];
+  const int64_t a = d.gbits[D.sel_off + g0];  // the tile's bit range [a, e)
+  const int64_t e = g1 < nsel ? d.gbits[D.sel_off + g1] : d.st[b].bits;
+  const int64_t w0 = a >> 5;
+  const int nw = (int)(((e + 31) >> 5) - w0);
+  for (int q = tid; q < nw; q += 256) W[q] = 0u;
   __syncthreads();
-  __shared__ uint16_t S[(kTile / kGSize + 2) * kGSize];
-  const int sbase = g0 * kGSize, send = min(g1 * kGSize, s);
-  for (int i = sbase + (int)threadIdx.x; i < send; i += 256) S[i - sbase] = d.SYM[D.off + i];
-  __syncthreads();
-  const int g = g0 + (int)threadIdx.x;
-  if (g < g1) {
-    const int gs = g * kGSize, ge = min(gs + kGSize, s);
-    const int tt = d.sel[D.sel_off + g];
-    const uint16_t* sym = S - sbase;
-    const int64_t a = d.gbits[D.sel_off + g];
-    const int64_t e = g + 1 < nsel ? d.gbits[D.sel_off + g + 1] : d.st[b].bits;
-    BitOut o;
-    o.init(d.pay + D.w_off, a, e, false);
-    for (int i = gs; i < ge; i++) {
-      uint16_t v = sym[i];
-      o.put(C[tt][v], L[tt][v]);
+  const int i0 = tid * kPackRun, cnt = max(0, min(kPackRun, n - i0));
+  uint32_t code[kPackRun];
+  int len[kPackRun];
+  int sum = 0;
+#pragma unroll
+  for (int j = 0; j < kPackRun; j++) {
+    len[j] = 0;
+    code[j] = 0;
+    if (j < cnt) {
+      const int i = i0 + j;
+      const int tt = T[i / kGSize];
+      const int v = S[i];
+      len[j] = L[tt][v];
+      code[j] = C[tt][v];
+      sum += len[j];
     }
-    o.finish();
+  }
+  int off = wg_excl_sum<256, int>(sum, tmp, nullptr) + (int)(a & 31);
+#pragma unroll
+  for (int j = 0; j < kPackRun; j++) {
+    if (j < cnt) {
+      const int l = len[j], sh = off & 31, wi = off >> 5;
+      const uint32_t c = code[j];
+      if (sh + l <= 32) {
+        atomicOr(&W[wi], c << (32 - sh - l));
+      } else {
+        atomicOr(&W[wi], c >> (sh + l - 32));
+        atomicOr(&W[wi + 1], c << (64 - sh - l));
+      }
+      off += l;
+    }
+  }
+  __syncthreads();
+  uint32_t* pay = d.pay + D.w_off;
+  for (int q = tid; q < nw; q += 256) {
+    const int64_t wi = w0 + q;
+    const uint32_t v = W[q];
+    if (wi * 32 >= a && wi * 32 + 32 <= e) pay[wi] = v;
+    else if (v) atomicOr(&pay[wi], v);
   }
 }
 

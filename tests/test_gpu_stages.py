@@ -196,6 +196,30 @@ def test_huff_build_stability_and_parity(dev):
         np.testing.assert_array_equal(cd[:ng, :a + 2], rcd[:ng, :a + 2])
 
 
+def test_huff_build_halving_rounds(dev):
+    # skewed symbol counts whose trees exceed 17 levels, so libbz2 halves the
+    # weights (huffman.cu:110-123): lengths, codes and selectors must still
+    # equal the oracle's
+    import torch
+    import bzip2gpu_amd as bz
+    rng = np.random.default_rng(21)
+    for a in [24, 254]:
+        _case(f"huff build halving alphabet={a}")
+        # symbol k about 450000 / 2^k times: the main table's tree is 20 deep
+        counts = [max(1, 450000 >> k) for k in range(a + 1)] + [1]
+        data = np.repeat(np.arange(a + 2, dtype=np.uint16), counts)
+        data = data[data != a + 1]
+        rng.shuffle(data)
+        data = np.concatenate([data, np.array([a + 1], np.uint16)])
+        ln, cd, sel, ng = bz.huff_build(torch.from_numpy(data.astype(np.int16)).to(dev), a)
+        rln, rcd, rsel, rng_ = oracle.huff_build(data, a)
+        assert ng == rng_
+        np.testing.assert_array_equal(sel, rsel)
+        np.testing.assert_array_equal(ln[:ng, :a + 2], rln[:ng, :a + 2])
+        np.testing.assert_array_equal(cd[:ng, :a + 2], rcd[:ng, :a + 2])
+        assert rln[:ng, :a + 2].max() <= 17
+
+
 def test_huff_encoding_roundtrip(dev):
     import torch
     import bzip2gpu_amd as bz

@@ -135,6 +135,58 @@ def test_code_lengths_invariants():
             assert sum(2.0 ** -int(x) for x in ln) <= 1.0 + 1e-12
 
 
+def _tree_height_exceeds(freq, max_len=17):
+    """Height of the libbz2 tree from the weight values alone: leaves sorted
+    by (weight, id), then a two-queue merge (merged weights come out in
+    non-decreasing order), depth byte = 1 + the deeper input's."""
+    w = sorted(((max(int(f), 1) << 8), i) for i, f in enumerate(freq))
+    leaves = [x for x, _ in w]
+    merged, i1, i2 = [], 0, 0
+    for _ in range(len(leaves) - 1):
+        ab = []
+        for _t in range(2):
+            if i1 < len(leaves) and (i2 >= len(merged) or leaves[i1] <= merged[i2]):
+                ab.append(leaves[i1]); i1 += 1
+            else:
+                ab.append(merged[i2]); i2 += 1
+        d = 1 + max(ab[0] & 0xFF, ab[1] & 0xFF)
+        if d > max_len:
+            return True
+        merged.append(((ab[0] & ~0xFF) + (ab[1] & ~0xFF)) | d)
+    return False
+
+
+def test_halving_rounds_follow_from_weight_values():
+    # The device decides libbz2's weight-halving rounds (huffman.cu:110-123)
+    # from the tree height alone and then builds once: one build with the
+    # final weights must equal the retrying build.
+    rng = np.random.default_rng(11)
+    fib = [1, 1]
+    while len(fib) < 30:  # Fib(30) < 900000, the largest count a block can hold
+        fib.append(fib[-1] + fib[-2])
+    cases = []
+    for alpha in [3, 20, 30, 64, 130, 258]:
+        cases.append(np.array(fib[:alpha] if alpha <= 30 else fib + [1] * (alpha - 30)))
+        cases.append((rng.pareto(0.5, alpha) * 50).astype(np.int64) % 900000)
+        cases.append(np.where(rng.random(alpha) < 0.7, 0, rng.integers(1, 900000, alpha)))
+    rounds_seen = 0
+    for f in cases:
+        if f.sum() > 900000:  # a block holds at most 900000 + 1 symbols
+            f = np.maximum(f * 900000 // f.sum(), 0)
+        alpha = f.size
+        want = oracle.make_code_lengths(f.astype(np.int32), alpha, 17)
+        g = np.maximum(f, 1)
+        rounds = 0
+        while _tree_height_exceeds(g):
+            g = 1 + g // 2
+            rounds += 1
+        rounds_seen += rounds
+        got = oracle.make_code_lengths(g.astype(np.int32), alpha, 64)  # no further rounds
+        np.testing.assert_array_equal(got, want)
+        assert got.max() <= 17
+    assert rounds_seen > 0
+
+
 def test_huff_reference_encoding_roundtrip():
     data = refinputs.encoding_input()
     ln, cd, sel, ng = oracle.huff_build(data, 15)
