@@ -62,37 +62,37 @@ __global__ void __launch_bounds__(256) k_bwt_init(Dev d) {
   const int k = t - D.tile_c0;
   // SA entries carry the row's BWT byte T[(i-1) mod m] in their top 8 bits;
   // KEY gets the round-0 key, the rotation's first 4 bytes.  A thread
-  // handles 16 consecutive rows.
-  const int i0 = k * kTile + threadIdx.x * 16;
-  if (i0 < m) {
-    const uint8_t* T = d.T + D.off;
-    uint32_t* SA = d.SA + D.off;
-    uint32_t* KEY = d.KEY + D.off;
-    uint32_t prev = T[i0 == 0 ? m - 1 : i0 - 1];
-    if (i0 + 19 <= m) {
-      const uint4 v = *reinterpret_cast<const uint4*>(T + i0);
-      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-      uint32_t by[19];
+  // handles four runs of 4 rows, run g at row k*kTile + 1024*g + 4*tid, so a
+  // wave's 16-byte stores cover 1 KiB contiguously.
+  const uint8_t* T = d.T + D.off;
+  uint32_t* SA = d.SA + D.off;
+  uint32_t* KEY = d.KEY + D.off;
+  const int lane = lane_id();
 #pragma unroll
-      for (int q = 0; q < 16; q++) by[q] = (w[q >> 2] >> (8 * (q & 3))) & 255u;
+  for (int g = 0; g < 4; g++) {
+    const int r0 = k * kTile + 1024 * g + 4 * (int)threadIdx.x;
+    // rows r0..r0+3 need bytes r0-1 .. r0+6; the whole wave takes the fast
+    // path when its last lane's 8 bytes lie inside the block
+    const bool fast = k * kTile + 1024 * g + 4 * (int)(threadIdx.x | 63) + 8 <= m;
+    if (fast) {
+      const uint32_t w0 = *reinterpret_cast<const uint32_t*>(T + r0);
+      const uint32_t w1 = *reinterpret_cast<const uint32_t*>(T + r0 + 4);
+      // byte r0-1: the lane below's top byte; lane 0 reads it (row 0 wraps)
+      uint32_t below = (uint32_t)__shfl_up((int)w0, 1, 64);
+      uint32_t prev = below >> 24;
+      if (lane == 0) prev = T[r0 == 0 ? m - 1 : r0 - 1];
+      const uint64_t x = ((uint64_t)w1 << 32) | w0;  // bytes r0..r0+7, little-endian
+      uint32_t o[4], kk[4];
 #pragma unroll
-      for (int q = 16; q < 19; q++) by[q] = T[i0 + q];
-#pragma unroll
-      for (int q4 = 0; q4 < 4; q4++) {
-        uint32_t o[4], kk[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int q = q4 * 4 + r;
-          o[r] = (uint32_t)(i0 + q) | (prev << 24);
-          kk[r] = (by[q] << 24) | (by[q + 1] << 16) | (by[q + 2] << 8) | by[q + 3];
-          prev = by[q];
-        }
-        *reinterpret_cast<uint4*>(SA + i0 + q4 * 4) = make_uint4(o[0], o[1], o[2], o[3]);
-        *reinterpret_cast<uint4*>(KEY + i0 + q4 * 4) = make_uint4(kk[0], kk[1], kk[2], kk[3]);
+      for (int r = 0; r < 4; r++) {
+        o[r] = (uint32_t)(r0 + r) | ((r == 0 ? prev : (uint32_t)(x >> (8 * r - 8)) & 255u) << 24);
+        kk[r] = __builtin_bswap32((uint32_t)(x >> (8 * r)));
       }
+      *reinterpret_cast<uint4*>(SA + r0) = make_uint4(o[0], o[1], o[2], o[3]);
+      *reinterpret_cast<uint4*>(KEY + r0) = make_uint4(kk[0], kk[1], kk[2], kk[3]);
     } else {
-      for (int i = i0; i < min(i0 + 16, m); i++) {
-        SA[i] = (uint32_t)i | (prev << 24);
+      for (int i = r0; i < min(r0 + 4, m); i++) {
+        SA[i] = (uint32_t)i | ((uint32_t)T[i == 0 ? m - 1 : i - 1] << 24);
         uint32_t kk = 0;
         int p = i;
         for (int q = 0; q < 4; q++) {
@@ -100,7 +100,6 @@ __global__ void __launch_bounds__(256) k_bwt_init(Dev d) {
           p = p + 1 == m ? 0 : p + 1;
         }
         KEY[i] = kk;
-        prev = T[i];
       }
     }
   }
