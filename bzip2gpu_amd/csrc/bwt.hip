@@ -712,6 +712,10 @@ __global__ void __launch_bounds__(256) k_lhist(Dev d, int cur, int shift, int sr
 }
 
 // per large group: hist[t][digit] -> absolute destination base
+// Per large group and digit: exclusive prefix of the digit's count over the
+// group's tiles.  The first tile's prefix is always 0, so its entry holds
+// the digit's global start instead (group start + rows of smaller digits);
+// k_lscatter adds it back for the other tiles.
 __global__ void __launch_bounds__(256) k_lscan(Dev d, int cur) {
   __shared__ int32_t tmp[8];
   const int si = d.llist[blockIdx.x];
@@ -720,20 +724,19 @@ __global__ void __launch_bounds__(256) k_lscan(Dev d, int cur) {
   const int dg = threadIdx.x;
   uint32_t* hp = d.hist + (int64_t)t0 * 256 + dg;
   uint32_t run = 0;
-  for (int t = 0; t < nt; t += 8) {
-    uint32_t v[8];
+  for (int t = 0; t < nt; t += 16) {
+    uint32_t v[16];
 #pragma unroll
-    for (int q = 0; q < 8; q++) v[q] = t + q < nt ? hp[(int64_t)(t + q) * 256] : 0u;
+    for (int q = 0; q < 16; q++) v[q] = t + q < nt ? hp[(int64_t)(t + q) * 256] : 0u;
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < 16; q++) {
       if (t + q < nt) hp[(int64_t)(t + q) * 256] = run;
       run += v[q];
     }
   }
   int tot;
   const int base = wg_excl_sum<256, int>((int)run, tmp, &tot);
-  const uint32_t add = (uint32_t)(g.start + base);
-  for (int t = 0; t < nt; t++) hp[(int64_t)t * 256] += add;
+  hp[0] = (uint32_t)(g.start + base);
 }
 
 // One LSD pass over a large-group tile: stable rank of every row by digit,
@@ -787,7 +790,11 @@ __global__ void __launch_bounds__(NT) k_lscatter(Dev d, int cur, int shift, int 
     const int ls = wg_excl_sum<NT, int>(sum, tmp, &tot);
     if (tid < 256) {
       lstart[tid] = (uint32_t)ls;
-      gstart[tid] = d.hist[(int64_t)t * 256 + tid];
+      // a tile's prefix within its group plus the group's digit start,
+      // which the group's first tile entry holds (k_lscan)
+      const int t0 = t - pos / kTile;
+      const uint32_t hv = d.hist[(int64_t)t * 256 + tid];
+      gstart[tid] = t0 == t ? hv : hv + d.hist[(int64_t)t0 * 256 + tid];
       uint32_t r = (uint32_t)ls;
 #pragma unroll
       for (int w = 0; w < NW; w++) { cnt[w][tid] = r; r += c[w]; }

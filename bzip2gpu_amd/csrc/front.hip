@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(256) k_rle1_write(Dev d) {
   __shared__ int32_t tmp[8];
   __shared__ int32_t xa[256], xe[256];
   __shared__ uint8_t seen[256];
-  __shared__ uint8_t ob[2 * kTile];
+  __shared__ __align__(16) uint8_t ob[2 * kTile + 4];
   const int tid = threadIdx.x;
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int b = d.tn_blk[t];
@@ -299,8 +299,9 @@ __global__ void __launch_bounds__(256) k_rle1_write(Dev d) {
   for (int w = tid; w < nw; w += 256) {
     const int oo = 4 * w - lead;
     if (oo >= 0 && oo + 4 <= total) {
-      uint32_t v = (uint32_t)ob[oo] | ((uint32_t)ob[oo + 1] << 8) | ((uint32_t)ob[oo + 2] << 16) |
-                   ((uint32_t)ob[oo + 3] << 24);
+      // the 4 bytes from two aligned words (ob has 4 bytes of slack)
+      const uint32_t* o32 = reinterpret_cast<const uint32_t*>(ob) + (oo >> 2);
+      const uint32_t v = (uint32_t)((((uint64_t)o32[1] << 32) | o32[0]) >> (8 * (oo & 3)));
       *reinterpret_cast<uint32_t*>(A0 + 4 * (uintptr_t)w) = v;
     } else {
       for (int q = 0; q < 4; q++) {

@@ -23,9 +23,9 @@ void launch_tile_scan_c(const Dev& d, hipStream_t s);
 
 // L[i] = T[(SA[i]-1) mod m] (fmtf.cu:55) from the byte each SA entry carries,
 // orig_ptr, per-tile last occurrence.
-// Each wave walks 1024 consecutive rows 64 at a time; within a step the
-// last lane of each byte value (found from 8 ballots) records the position,
-// so later steps overwrite earlier ones without atomics.
+// Each wave walks 1024 consecutive rows 64 at a time; the last occurrence of
+// every byte value is an LDS atomic max over the rows' positions (one
+// counter set per wave, merged at the end).
 __global__ void __launch_bounds__(256) k_mtf_gather(Dev d) {
   __shared__ int32_t lo[4][256];
   const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
@@ -39,7 +39,6 @@ __global__ void __launch_bounds__(256) k_mtf_gather(Dev d) {
   lo[0][tid] = -1; lo[1][tid] = -1; lo[2][tid] = -1; lo[3][tid] = -1;
   __syncthreads();
   const int n = min(kTile, m - start);
-  const uint64_t above = ~((2ull << lane) - 1ull);
   uint32_t sv[16];  // all 16 loads in flight before the first use
 #pragma unroll
   for (int q = 0; q < 16; q++) {
@@ -58,14 +57,7 @@ __global__ void __launch_bounds__(256) k_mtf_gather(Dev d) {
       d.MTF[D.off + i] = (uint8_t)c;
       if ((v & kSAMask) == 0) d.st[b].orig_ptr = i;
     }
-    uint64_t peers = ballot(valid);
-#pragma unroll
-    for (int bb = 0; bb < 8; bb++) {
-      const bool bit = (c >> bb) & 1u;
-      const uint64_t mm = ballot(bit);
-      peers &= bit ? mm : ~mm;
-    }
-    if (valid && !(peers & above)) lo[wv][c] = i;
+    if (valid) atomicMax(&lo[wv][c], i);
   }
   __syncthreads();
   d.lo[(int64_t)t * 256 + tid] = max(max(lo[0][tid], lo[1][tid]), max(lo[2][tid], lo[3][tid]));
