@@ -830,9 +830,22 @@ __global__ void __launch_bounds__(256) k_lsum(Dev d, int cur, int fin) {
   const uint32_t* K = fin ? d.KEY2 : d.KEY;
   const int n = min(kTile, g.len - pos);
   int fb = 0x7fffffff, lb = -1;
-  for (int e = threadIdx.x; e < n; e += 256) {
-    int p = pos + e;
-    if (p == 0 || K[base + p] != K[base + p - 1]) { fb = min(fb, p); lb = max(lb, p); }
+  // each key is loaded once (all 16 loads in flight); the key before a
+  // lane's row comes from the lane below, lane 0 loads it
+  uint32_t kv[16], kp[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int e = (int)threadIdx.x + 256 * q;
+    kv[q] = e < n ? K[base + pos + e] : 0u;
+    kp[q] = lane_id() == 0 && e < n && pos + e > 0 ? K[base + pos + e - 1] : 0u;
+  }
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int e = (int)threadIdx.x + 256 * q;
+    const uint32_t below = (uint32_t)__shfl_up((int)kv[q], 1, 64);
+    const uint32_t prev = lane_id() == 0 ? kp[q] : below;
+    const int p = pos + e;
+    if (e < n && (p == 0 || kv[q] != prev)) { fb = min(fb, p); lb = max(lb, p); }
   }
   fb = wave_min(fb);
   lb = wave_max(lb);

@@ -608,8 +608,13 @@ __global__ void __launch_bounds__(256) k_huff_codes(Dev d) {
 
 // ------------------------------------------------------------ payload
 // bits per 50-symbol group
+// Payload bits of every capacity tile's selector groups (tile k holds the
+// groups that start in its 4096 symbols): the symbols staged in LDS, code
+// lengths summed per thread, then per workgroup.
 __global__ void __launch_bounds__(256) k_group_bits(Dev d) {
   __shared__ uint8_t L[6][260];
+  __shared__ uint8_t T[kTile / kGSize + 2];
+  __shared__ int32_t tmp[8];
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int b = d.tc_blk[t];
   const BlockDesc D = d.desc[b];
@@ -617,37 +622,39 @@ __global__ void __launch_bounds__(256) k_group_bits(Dev d) {
   const int k = t - D.tile_c0;
   const int g0 = (k * kTile + kGSize - 1) / kGSize;
   const int g1 = min(((k + 1) * kTile + kGSize - 1) / kGSize, (s + kGSize - 1) / kGSize);
-  if (g0 >= g1) return;
-  for (int q = threadIdx.x; q < 6 * 260; q += 256) (&L[0][0])[q] = d.len[(int64_t)b * 6 * 260 + q];
-  __syncthreads();
-  __shared__ uint16_t S[(kTile / kGSize + 2) * kGSize];
-  const int sbase = g0 * kGSize, send = min(g1 * kGSize, s);
-  for (int i = sbase + (int)threadIdx.x; i < send; i += 256) S[i - sbase] = d.SYM[D.off + i];
-  __syncthreads();
-  const int g = g0 + (int)threadIdx.x;
-  if (g < g1) {
-    const int gs = g * kGSize, ge = min(gs + kGSize, s);
-    const int tt = d.sel[D.sel_off + g];
-    const uint16_t* sym = S - sbase;
-    int nb = 0;
-    for (int i = gs; i < ge; i++) nb += L[tt][sym[i]];
-    d.gbits[D.sel_off + g] = nb;
+  const int tid = threadIdx.x;
+  if (g0 >= g1) {
+    if (tid == 0) d.gbits[D.sel_off + k] = 0;
+    return;
   }
+  constexpr int kSyms = (kTile / kGSize + 2) * kGSize;
+  __shared__ uint16_t S[kSyms];
+  for (int q = tid; q < 6 * 260; q += 256) (&L[0][0])[q] = d.len[(int64_t)b * 6 * 260 + q];
+  for (int g = tid; g < g1 - g0; g += 256) T[g] = d.sel[D.sel_off + g0 + g];
+  const int sbase = g0 * kGSize, n = min(g1 * kGSize, s) - sbase;
+  for (int i = tid; i < n; i += 256) S[i] = d.SYM[D.off + sbase + i];
+  __syncthreads();
+  int sum = 0;
+  for (int i = tid; i < n; i += 256) sum += L[T[i / kGSize]][S[i]];
+  int tot;
+  (void)wg_excl_sum<256, int>(sum, tmp, &tot);
+  if (tid == 0) d.gbits[D.sel_off + k] = tot;
 }
 
-// per block: exclusive scan of group bits -> offsets, W, zero payload words
+// per block: exclusive scan of the tiles' payload bits -> each tile's first
+// bit, the block's bit count, zeroed payload words
 __global__ void __launch_bounds__(256) k_group_scan(Dev d) {
   __shared__ int32_t tmp[8];
   const int b = blockIdx.x;
   const BlockDesc D = d.desc[b];
-  const int nsel = d.st[b].n_sel;
+  const int nt = tiles_for(D.cap);
   int run = 0;
-  for (int base = 0; base < nsel; base += 256) {
-    int g = base + threadIdx.x;
-    int v = g < nsel ? d.gbits[D.sel_off + g] : 0;
+  for (int base = 0; base < nt; base += 256) {
+    const int k = base + threadIdx.x;
+    const int v = k < nt ? d.gbits[D.sel_off + k] : 0;
     int tot;
-    int ex = wg_excl_sum<256, int>(v, tmp, &tot);
-    if (g < nsel) d.gbits[D.sel_off + g] = run + ex;
+    const int ex = wg_excl_sum<256, int>(v, tmp, &tot);
+    if (k < nt) d.gbits[D.sel_off + k] = run + ex;
     run += tot;
   }
   if (threadIdx.x == 0) d.st[b].bits = run;
@@ -690,8 +697,8 @@ __global__ void __launch_bounds__(256) k_pack(Dev d) {
   const int sbase = g0 * kGSize, n = min(g1 * kGSize, s) - sbase;
   for (int i = tid; i < n; i += 256) S[i] = d.SYM[D.off + sbase + i];
   for (int g = tid; g < g1 - g0; g += 256) T[g] = d.sel[D.sel_off + g0 + g];
-  const int64_t a = d.gbits[D.sel_off + g0];  // the tile's bit range [a, e)
-  const int64_t e = g1 < nsel ? d.gbits[D.sel_off + g1] : d.st[b].bits;
+  const int64_t a = d.gbits[D.sel_off + k];  // the tile's bit range [a, e)
+  const int64_t e = g1 < nsel ? d.gbits[D.sel_off + k + 1] : d.st[b].bits;
   const int64_t w0 = a >> 5;
   const int nw = (int)(((e + 31) >> 5) - w0);
   for (int q = tid; q < nw; q += 256) W[q] = 0u;

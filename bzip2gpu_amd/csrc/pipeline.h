@@ -70,7 +70,7 @@ struct Dev {
   uint32_t* code;      // [nblk][6][260]
   uint8_t* sel;        // sel_off-indexed
   uint8_t* selmtf;
-  int32_t* gbits;      // per 50-group bit count, then offsets (sel_off-indexed)
+  int32_t* gbits;      // per capacity tile of a block: payload bits, then first bit (sel_off-indexed)
   uint32_t* pay;       // payload words (w_off-indexed)
   uint32_t* hdr;       // header words (h_off = per-block stride hdr_stride)
   int64_t hdr_stride;  // words per block in hdr
