@@ -38,7 +38,22 @@ struct ShiftTables {
 };
 static_assert(kTile == 4096, "shift tables assume 4096-byte tiles");
 
+// Slicing-by-4 tables: s[k][b] is the register of byte b followed by k zero
+// bytes, so four bytes b0..b3 advance a register r in one step:
+// v = r ^ (b0<<24 | b1<<16 | b2<<8 | b3),
+// r' = s[3][v>>24] ^ s[2][(v>>16)&255] ^ s[1][(v>>8)&255] ^ s[0][v&255].
+struct CrcSlice4 {
+  uint32_t s[4][256];
+  constexpr CrcSlice4() : s() {
+    const CrcTable t;
+    for (int b = 0; b < 256; b++) s[0][b] = t.t[b];
+    for (int k = 1; k < 4; k++)
+      for (int b = 0; b < 256; b++) s[k][b] = (s[k - 1][b] << 8) ^ t.t[s[k - 1][b] >> 24];
+  }
+};
+
 static __constant__ CrcTable c_crc = CrcTable();
+static __constant__ CrcSlice4 c_crc4 = CrcSlice4();
 static __constant__ Pow8Table c_pow8 = Pow8Table();
 static __constant__ ShiftTables c_shift = ShiftTables();
 

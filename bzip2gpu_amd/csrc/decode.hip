@@ -1176,7 +1176,7 @@ __global__ void __launch_bounds__(256) k_dec_rle1b(const DecBlk* __restrict__ bl
   __shared__ uint32_t wf[4][5];
   __shared__ int32_t cum[kTile + 1];
   __shared__ uint8_t eb[kTile];
-  __shared__ uint32_t tab[256];
+  __shared__ uint32_t tab[4][256];
   __shared__ int32_t tmp[8];
   __shared__ uint32_t red[4];
   const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
@@ -1185,7 +1185,8 @@ __global__ void __launch_bounds__(256) k_dec_rle1b(const DecBlk* __restrict__ bl
   const DecBlk B = blk[b];
   const int p0 = (t - B.mt0) * kTile;
   const int n = min(kTile, B.m - p0);
-  tab[tid] = c_crc.t[tid];
+#pragma unroll
+  for (int k = 0; k < 4; k++) tab[k][tid] = c_crc4.s[k][tid];
   uint8_t x[16];
   uint32_t pb;
   const int c = load_t16(T + B.m_off, p0, n, tid, x, &pb);
@@ -1256,11 +1257,28 @@ __global__ void __launch_bounds__(256) k_dec_rle1b(const DecBlk* __restrict__ bl
     int i = lo;
     const int oe = min(o0 + 16, tot);
     uint32_t reg = 0;
-    for (int o = o0; o < oe; o++) {
-      while (cum[i + 1] <= o) i++;
-      const uint32_t v = eb[i];
-      dst[o] = (uint8_t)v;
-      reg = (reg << 8) ^ tab[(reg >> 24) ^ v];
+    if (oe - o0 == 16) {
+      // the 16 bytes in registers, then the register four bytes per step
+      uint32_t wd[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int k = 0; k < 16; k++) {
+        while (cum[i + 1] <= o0 + k) i++;
+        const uint32_t v = eb[i];
+        dst[o0 + k] = (uint8_t)v;
+        wd[k >> 2] |= v << (24 - 8 * (k & 3));
+      }
+#pragma unroll
+      for (int w4 = 0; w4 < 4; w4++) {
+        const uint32_t v = reg ^ wd[w4];
+        reg = tab[3][v >> 24] ^ tab[2][(v >> 16) & 255u] ^ tab[1][(v >> 8) & 255u] ^ tab[0][v & 255u];
+      }
+    } else {
+      for (int o = o0; o < oe; o++) {
+        while (cum[i + 1] <= o) i++;
+        const uint32_t v = eb[i];
+        dst[o] = (uint8_t)v;
+        reg = (reg << 8) ^ tab[0][(reg >> 24) ^ v];
+      }
     }
     acc ^= crc_shift(reg, (uint32_t)(tot - oe));
   }

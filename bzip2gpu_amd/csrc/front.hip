@@ -61,7 +61,7 @@ __device__ inline int load16(const uint8_t* src, int len, int tid, uint8_t x[16]
 // (block-relative) and the RLE1 output count of the positions whose run
 // starts inside the tile.
 __global__ void __launch_bounds__(256) k_front1(Dev d) {
-  __shared__ uint32_t tab[256];
+  __shared__ uint32_t tab[4][256];
   __shared__ uint8_t lastb[256];
   __shared__ int32_t xs[256];
   __shared__ int32_t tmp[8];
@@ -74,16 +74,27 @@ __global__ void __launch_bounds__(256) k_front1(Dev d) {
   const int start = k * kTile;
   const int len = min(kTile, D.n - start);
   const uint8_t* src = d.in + D.in_off + start;
-  tab[tid] = c_crc.t[tid];
+#pragma unroll
+  for (int k = 0; k < 4; k++) tab[k][tid] = c_crc4.s[k][tid];
   uint8_t x[16];
   const int c = load16(src, len, tid, x);
   lastb[tid] = c > 0 ? x[c - 1] : 0;
   __syncthreads();
   // (fixed-count loops over x[] keep the register indexing static)
   uint32_t reg = 0;
+  if (c == 16) {
+    // four bytes per step, the four table reads of a step independent
 #pragma unroll
-  for (int q = 0; q < 16; q++)
-    if (q < c) reg = (reg << 8) ^ tab[(reg >> 24) ^ x[q]];
+    for (int w = 0; w < 4; w++) {
+      const uint32_t v = reg ^ (((uint32_t)x[4 * w] << 24) | ((uint32_t)x[4 * w + 1] << 16) |
+                                ((uint32_t)x[4 * w + 2] << 8) | (uint32_t)x[4 * w + 3]);
+      reg = tab[3][v >> 24] ^ tab[2][(v >> 16) & 255u] ^ tab[1][(v >> 8) & 255u] ^ tab[0][v & 255u];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+      if (q < c) reg = (reg << 8) ^ tab[0][(reg >> 24) ^ x[q]];
+  }
   if (c > 0) reg = crc_shift(reg, (uint32_t)(len - tid * 16 - c));
   // run starts
   const int p0 = start + tid * 16;
