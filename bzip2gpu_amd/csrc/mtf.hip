@@ -160,6 +160,14 @@ __device__ inline uint64_t bit_transpose64(uint64_t x) {
 
 }  // namespace
 
+// The MTF workgroup is one wave, whose LDS accesses execute in program
+// order: a barrier is only needed to keep the compiler from moving them.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One wave per segment: the MTF proper.  The segment is staged in LDS and
 // the list is kept as s2p[] (list position of every present byte, one byte
 // each).  After each 64-symbol step the K distinct bytes of the step take
@@ -198,7 +206,7 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
     sflag[q] = 0;
     pm[q] = 0ull;
   }
-  __syncthreads();
+  wave_sync();
   // list position of every present byte: # present bytes more recent
   {
     int e4[4], p4[4] = {0, 0, 0, 0};
@@ -213,7 +221,7 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
         (uint32_t)(p4[0] & 255) | ((uint32_t)(p4[1] & 255) << 8) | ((uint32_t)(p4[2] & 255) << 16) |
         ((uint32_t)(p4[3] & 255) << 24);
   }
-  __syncthreads();
+  wave_sync();
   int fnz = 0x7fffffff;
   const uint64_t below = lanemask_lt();
   for (int base = 0; base < n; base += 64) {
@@ -223,7 +231,7 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
     // lanes holding the same byte: each lane ORs its bit into its byte's
     // LDS mask and reads the mask back (cleared again below)
     if (valid) atomicOr(&pm[c], 1ull << lane);
-    __syncthreads();
+    wave_sync();
     const uint64_t peers = valid ? pm[c] : 0ull;
     const uint64_t pb = peers & below;
     const bool has_prev = valid && pb != 0;
@@ -280,7 +288,7 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
     }
     if (last) newpos[c] = (uint8_t)(__popcll(lastm >> lane) - 1);
     if (valid) pm[c] = 0ull;  // every lane has read its mask (in program order)
-    __syncthreads();
+    wave_sync();
     {
       // flags below this lane's four positions, from one ballot per byte slot
       const uint32_t f4 = reinterpret_cast<const uint32_t*>(pflag)[lane];
@@ -290,7 +298,7 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
       reinterpret_cast<uint32_t*>(rank)[lane] =
           (uint32_t)ex | ((uint32_t)r1 << 8) | ((uint32_t)r2 << 16) | ((uint32_t)r3 << 24);
     }
-    __syncthreads();
+    wave_sync();
     {
       const uint32_t sp4 = reinterpret_cast<const uint32_t*>(s2p)[lane];
       const uint32_t sf4 = reinterpret_cast<const uint32_t*>(sflag)[lane];
@@ -305,14 +313,14 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
         const int np = ((sf4 >> (8 * r)) & 1u) ? (int)((nw4 >> (8 * r)) & 255u) : K + op - rk[r];
         np4 |= (uint32_t)(np & 255) << (8 * r);
       }
-      __syncthreads();
+      wave_sync();
       reinterpret_cast<uint32_t*>(s2p)[lane] = np4;
       if (first) {
         pflag[mypos] = 0;
         sflag[c] = 0;
       }
     }
-    __syncthreads();
+    wave_sync();
   }
   for (int o = lane * 16; o < n; o += 1024) {
     if (o + 16 <= n) {
