@@ -746,9 +746,51 @@ __global__ void __launch_bounds__(256) k_pack(Dev d) {
 }
 
 // ------------------------------------------------------------ assembly
-// One workgroup: per stream block bit offsets, combined CRC, sizes, byte
-// offsets; zeroes the words the placement kernels share and writes the
-// stream header word.
+// One workgroup per stream: bit offset of every block inside its stream (an
+// exclusive scan of header + payload bits after the 32-bit stream header),
+// the stream's bit length and its combined CRC.  The combined CRC folds
+// c = rotl(c, 1) ^ crc_q over the blocks, i.e. the XOR of every block CRC
+// rotated by the number of blocks after it.
+__global__ void __launch_bounds__(256) k_layout_blocks(Dev d) {
+  __shared__ int64_t tmp64[4];
+  __shared__ uint32_t xr[4];
+  const int si = blockIdx.x;
+  const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+  StreamState S = d.ss[si];
+  int64_t run = 32;
+  uint32_t crc = 0;
+  for (int q0 = 0; q0 < S.n_blk; q0 += 256) {
+    const int q = q0 + tid;
+    int64_t nb = 0;
+    uint32_t c = 0;
+    BlockState* st = d.st + S.first_blk + q;
+    if (q < S.n_blk) {
+      nb = st->hdr_bits + st->bits;
+      const int r = (S.n_blk - 1 - q) & 31;
+      c = r ? (st->crc << r) | (st->crc >> (32 - r)) : st->crc;
+    }
+    const int64_t inc = wave_incl_sum(nb);
+    if (lane == 63) tmp64[wv] = inc;
+    for (int o = 32; o >= 1; o >>= 1) c ^= (uint32_t)__shfl_xor((int)c, o, 64);
+    if (lane == 0) xr[wv] = c;
+    __syncthreads();
+    int64_t pre = 0;
+    for (int w = 0; w < wv; w++) pre += tmp64[w];
+    if (q < S.n_blk) st->bit_off = run + pre + inc - nb;
+    run += tmp64[0] + tmp64[1] + tmp64[2] + tmp64[3];
+    crc ^= xr[0] ^ xr[1] ^ xr[2] ^ xr[3];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    S.bits = run + 48 + 32;
+    S.crc = crc;
+    S.bytes = (S.bits + 7) / 8;
+    d.ss[si] = S;
+  }
+}
+
+// One workgroup: the streams' 4-byte aligned places in the output (a scan
+// of their word counts) and their header words.
 __global__ void __launch_bounds__(256) k_layout(Dev d) {
   __shared__ int64_t tmp64[8];
   uint32_t* out = reinterpret_cast<uint32_t*>(d.out);
@@ -759,19 +801,7 @@ __global__ void __launch_bounds__(256) k_layout(Dev d) {
     StreamState S{};
     if (si < d.nstream) {
       S = d.ss[si];
-      int64_t off = 32;
-      uint32_t crc = 0;
-      for (int q = 0; q < S.n_blk; q++) {
-        BlockState* st = d.st + S.first_blk + q;
-        st->bit_off = off;
-        off += st->hdr_bits + st->bits;
-        crc = ((crc << 1) | (crc >> 31)) ^ st->crc;
-      }
-      off += 48 + 32;
-      S.bits = off;
-      S.crc = crc;
-      S.bytes = (off + 7) / 8;
-      words = (off + 31) / 32;
+      words = (S.bits + 31) / 32;
     }
     int64_t inc = wave_incl_sum(words);
     if (lane_id() == 63) tmp64[threadIdx.x >> 6] = inc;
@@ -785,21 +815,27 @@ __global__ void __launch_bounds__(256) k_layout(Dev d) {
       d.ss[si] = S;
       uint32_t* w0 = out + S.byte_off / 4;
       w0[0] = __builtin_bswap32(0x425A6830u + (uint32_t)S.level);
-      for (int q = 0; q < S.n_blk; q++) {
-        const BlockState* st = d.st + S.first_blk + q;
-        int64_t a = st->bit_off, e = a + st->hdr_bits + st->bits;
-        w0[a >> 5] = 0;
-        w0[(e - 1) >> 5] = 0;
-        // the header/payload seam inside a block is shared too
-        int64_t mid = a + st->hdr_bits;
-        w0[mid >> 5] = 0;
-      }
-      int64_t ta = S.bits - 80;
+      // the words shared at the end-of-stream marker
+      const int64_t ta = S.bits - 80;
       w0[ta >> 5] = 0;
       w0[(S.bits - 1) >> 5] = 0;
     }
     run += tot;
   }
+}
+
+// Zeroes the words a block shares with its neighbours (first and last word,
+// and the one at its header/payload seam), which the placement ORs into.
+__global__ void __launch_bounds__(256) k_seams(Dev d) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= d.nblk) return;
+  const BlockState* st = d.st + b;
+  const StreamState S = d.ss[d.desc[b].stream];
+  uint32_t* w0 = reinterpret_cast<uint32_t*>(d.out) + S.byte_off / 4;
+  const int64_t a = st->bit_off, e = a + st->hdr_bits + st->bits;
+  w0[a >> 5] = 0;
+  w0[(e - 1) >> 5] = 0;
+  w0[(a + st->hdr_bits) >> 5] = 0;
 }
 
 // Copies `nbits` bits from aligned source words into the stream at bit
@@ -870,8 +906,12 @@ void launch_pack(const Dev& d, hipStream_t s) {
 
 void launch_assemble(const Dev& d, hipStream_t s) {
   if (d.nstream == 0) return;
+  k_layout_blocks<<<d.nstream, 256, 0, s>>>(d);
   k_layout<<<1, 256, 0, s>>>(d);
-  if (d.nblk > 0) k_place<<<d.ntile_c, 256, 0, s>>>(d);
+  if (d.nblk > 0) {
+    k_seams<<<(d.nblk + 255) / 256, 256, 0, s>>>(d);
+    k_place<<<d.ntile_c, 256, 0, s>>>(d);
+  }
   k_trailer<<<(d.nstream + 63) / 64, 64, 0, s>>>(d);
 }
 
