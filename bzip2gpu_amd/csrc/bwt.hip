@@ -68,8 +68,11 @@ __global__ void __launch_bounds__(256) k_bwt_init(Dev d) {
   uint32_t* SA = d.SA + D.off;
   uint32_t* KEY = d.KEY + D.off;
   const int lane = lane_id();
+  // a block of more than kHalf rows is one large group in round 0, whose
+  // first radix pass makes these entries from the text itself
+  const bool gen = m > kHalf;
 #pragma unroll
-  for (int g = 0; g < 4; g++) {
+  for (int g = 0; g < 4 && !gen; g++) {
     const int r0 = k * kTile + 1024 * g + 4 * (int)threadIdx.x;
     // rows r0..r0+3 need bytes r0-1 .. r0+6; the whole wave takes the fast
     // path when its last lane's 8 bytes lie inside the block
@@ -687,6 +690,9 @@ __global__ void __launch_bounds__(256) k_sort_small64(Dev d, int cur, int keybit
 // ----------------------------------------------------------- large groups
 // Digit histogram of a large-group tile (per-wave counters, so that skewed
 // digits collide only within a wave).
+// With GEN (round 0's first pass, whose groups are whole blocks) the digit
+// is read from the text: the key's low byte is T[(i+3) mod m].
+template <bool GEN>
 __global__ void __launch_bounds__(256) k_lhist(Dev d, int cur, int shift, int src) {
   __shared__ uint32_t h[4][256];
   const int tid = threadIdx.x, wv = tid >> 6;
@@ -699,10 +705,48 @@ __global__ void __launch_bounds__(256) k_lhist(Dev d, int cur, int shift, int sr
   __syncthreads();
   const int n = min(kTile, g.len - pos);
   uint32_t k[16];
+  if (GEN) {
+    // digits of rows pos .. pos+n-1 are the bytes at pos+3 .. pos+n+2: each
+    // thread takes 16 aligned bytes at pos + 16*tid (the rows 3 before
+    // them), thread 255 also the 3 bytes after the tile (mod m)
+    const uint8_t* T = d.T + off;
+    const int m = g.len;
+    const int q0 = 16 * tid;
+    uint32_t w4[4] = {0u, 0u, 0u, 0u};
+    if (q0 + 16 <= n) {
+      const uint4 v = *reinterpret_cast<const uint4*>(T + pos + q0);
+      w4[0] = v.x; w4[1] = v.y; w4[2] = v.z; w4[3] = v.w;
+    } else {
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
-    const int e = tid + 256 * q;
-    k[q] = e < n ? K[e] : 0u;
+      for (int j = 0; j < 16; j++)
+        if (q0 + j < n) w4[j >> 2] |= (uint32_t)T[pos + q0 + j] << (8 * (j & 3));
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const int e = q0 + j - 3;  // the row whose digit this byte is
+      const uint32_t dg = (w4[j >> 2] >> (8 * (j & 3))) & 255u;
+      k[j] = (q0 + j < n && e >= 0) ? dg : 0xFFFFFFFFu;
+    }
+    if (tid == 255) {
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        int i = pos + n + j;
+        i = i >= m ? i - m : i;  // m > kHalf > 3
+        if (n - 3 + j >= 0) atomicAdd(&h[wv][T[i]], 1u);  // row n-3+j
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+      if (k[j] != 0xFFFFFFFFu) atomicAdd(&h[wv][k[j]], 1u);
+    __syncthreads();
+    d.hist[(int64_t)t * 256 + tid] = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
+    return;
+  } else {
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int e = tid + 256 * q;
+      k[q] = e < n ? K[e] : 0u;
+    }
   }
 #pragma unroll
   for (int q = 0; q < 16; q++)
@@ -744,7 +788,7 @@ __global__ void __launch_bounds__(256) k_lscan(Dev d, int cur) {
 // one digit go to consecutive global addresses (coalesced runs).  NT threads
 // (NT/64 waves), 4096/NT rows each; the shape is an autotuned choice and
 // every shape gives the same result.
-template <int NT>
+template <int NT, bool GEN>
 __global__ void __launch_bounds__(NT) k_lscatter(Dev d, int cur, int shift, int src) {
   constexpr int NW = NT / 64, PER = kTile / NT;
   __shared__ uint32_t cnt[NW][256];
@@ -767,11 +811,58 @@ __global__ void __launch_bounds__(NT) k_lscatter(Dev d, int cur, int shift, int 
   for (int q = tid; q < NW * 256; q += NT) (&cnt[0][0])[q] = 0;
   __syncthreads();
   uint32_t key[PER], val[PER], rk[PER];
+  if (GEN) {
+    // Round 0's first pass (groups are whole blocks, rows in text order):
+    // row i's key is the rotation's first 4 bytes, big-endian, and its SA
+    // entry is i with the BWT byte T[(i-1) mod m] on top (as k_bwt_init
+    // writes them for the small blocks).  The tile's text, from 4 bytes
+    // before it, is staged as words in the (not yet used) skey area.
+    const uint8_t* T = d.T + off;
+    const int m = g.len;
+    uint32_t* tw = skey;
+    constexpr int kW = kTile / 4 + 2, kWI = (kW + NT - 1) / NT;
+    if (pos >= 4 && pos + 4 * kW - 4 <= m) {
+      // inside the block: aligned words (off and pos are multiples of 256),
+      // all loads in flight at once
+      const uint32_t* T32 = reinterpret_cast<const uint32_t*>(T + pos - 4);
+      uint32_t v[kWI];
 #pragma unroll
-  for (int q = 0; q < PER; q++) {
-    const int e = wv * (64 * PER) + q * 64 + lane;
-    key[q] = e < n ? Ks[base + e] : 0u;
-    val[q] = e < n ? Ss[base + e] : 0u;
+      for (int j = 0; j < kWI; j++) v[j] = tid + NT * j < kW ? T32[tid + NT * j] : 0u;
+#pragma unroll
+      for (int j = 0; j < kWI; j++)
+        if (tid + NT * j < kW) tw[tid + NT * j] = v[j];
+    } else {
+      // the block's first and last tiles: byte by byte, positions mod m
+      for (int w = tid; w < kW; w += NT) {
+        const int a = pos + 4 * w - 4;
+        uint32_t v = 0;
+        for (int j = 0; j < 4; j++) {
+          int i = (a + j) % m;
+          i = i < 0 ? i + m : i;
+          v |= (uint32_t)T[i] << (8 * j);
+        }
+        tw[w] = v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = wv * (64 * PER) + q * 64 + lane;
+      const int e4 = e + 4;  // byte index of row e in the staged words
+      const uint32_t lo = tw[e4 >> 2], hi = tw[(e4 >> 2) + 1];
+      const uint32_t x = __builtin_amdgcn_alignbyte(hi, lo, (uint32_t)(e4 & 3));
+      const uint32_t pv = (tw[(e4 - 1) >> 2] >> (8 * ((e4 - 1) & 3))) & 255u;
+      key[q] = e < n ? __builtin_bswap32(x) : 0u;
+      val[q] = e < n ? (uint32_t)(pos + e) | (pv << 24) : 0u;
+    }
+    __syncthreads();  // (the staged words are overwritten below)
+  } else {
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = wv * (64 * PER) + q * 64 + lane;
+      key[q] = e < n ? Ks[base + e] : 0u;
+      val[q] = e < n ? Ss[base + e] : 0u;
+    }
   }
 #pragma unroll
   for (int q = 0; q < PER; q++) {
@@ -1087,12 +1178,22 @@ void launch_bwt_round(const Dev& d, int cur, int round, int32_t P, int key_bits,
     const int npass = (key_bits + 7) / 8;
     for (int p = 0; p < npass; p++) {
       if (pass_ev) (void)hipEventRecord(pass_ev[p], s);
-      k_lhist<<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
+      // round 0's first pass reads the text (k_bwt_init leaves KEY / SA of
+      // the large blocks unwritten); it is not one of the probed passes
+      const bool gen = round == 0 && p == 0;
+      if (gen) k_lhist<true><<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
+      else k_lhist<false><<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
       k_lscan<<<nlarge, 256, 0, s>>>(d, cur);
+      if (gen) {
+        if (scatter_nt[p] == 256) k_lscatter<256, true><<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
+        else if (scatter_nt[p] == 1024) k_lscatter<1024, true><<<ntiles, 1024, 0, s>>>(d, cur, 8 * p, p & 1);
+        else k_lscatter<512, true><<<ntiles, 512, 0, s>>>(d, cur, 8 * p, p & 1);
+        continue;
+      }
       probe_begin("k_lscatter", s);
-      if (scatter_nt[p] == 256) k_lscatter<256><<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
-      else if (scatter_nt[p] == 1024) k_lscatter<1024><<<ntiles, 1024, 0, s>>>(d, cur, 8 * p, p & 1);
-      else k_lscatter<512><<<ntiles, 512, 0, s>>>(d, cur, 8 * p, p & 1);
+      if (scatter_nt[p] == 256) k_lscatter<256, false><<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
+      else if (scatter_nt[p] == 1024) k_lscatter<1024, false><<<ntiles, 1024, 0, s>>>(d, cur, 8 * p, p & 1);
+      else k_lscatter<512, false><<<ntiles, 512, 0, s>>>(d, cur, 8 * p, p & 1);
       probe_end("k_lscatter", s);
     }
     if (pass_ev) (void)hipEventRecord(pass_ev[npass], s);

@@ -364,7 +364,8 @@ int Engine::run_bwt(hipStream_t s, int* rounds_out) {
     active_elems_ += (int64_t)small + (int64_t)h_ctr_[CTR_LELEMS];
     small_rows_ += small;
     large_rows_ += h_ctr_[CTR_LELEMS];
-    large_pass_rows_ += (int64_t)h_ctr_[CTR_LELEMS] * ((keybits + 7) / 8);
+    // radix passes that read KEY / SA (round 0's first pass reads the text)
+    large_pass_rows_ += (int64_t)h_ctr_[CTR_LELEMS] * ((keybits + 7) / 8 - (round == 0 ? 1 : 0));
     trace(s, "bwt round start", round, nseg, ntiles);
     if (rev_.size() < (size_t)round + 1) {
       hipEvent_t a = nullptr, b = nullptr;
