@@ -69,6 +69,23 @@ def test_levels_and_block_sizes(dev):
                 assert bz2.decompress(got) == data
 
 
+def test_round0_tile_edges(dev):
+    """Round 0's first radix pass makes keys from the text: blocks just above
+    the small-group limit (1024 rows) and blocks whose last 4096-row tile has
+    1-3 rows (the key and the digit wrap around the block end).  Random bytes
+    keep RLE1 from changing the lengths."""
+    import bzip2gpu_amd as bz
+    rng = np.random.default_rng(21)
+    for n in [1025, 1026, 1027, 1030, 4095, 4096, 4097, 4098, 4099, 8193, 12290, 40963]:
+        data = rng.integers(0, 256, n, dtype=np.uint8).tobytes()
+        _case(f"round-0 tile edges n={n}")
+        _same(bz.compress(data, 9), oracle.compress(data, 9), f"n={n}")
+    # a text-like block with long equal-key groups across tile edges
+    data = (b"abcabcabd" * 1000)[:8195]
+    _case("round-0 tile edges periodic n=8195")
+    _same(bz.compress(data, 9), oracle.compress(data, 9), "periodic 8195")
+
+
 def test_stock_equal_blocks(dev):
     import bzip2gpu_amd as bz
     rng = np.random.default_rng(2)
