@@ -674,9 +674,8 @@ constexpr int kPackRun = (kPackSyms + 255) / 256;
 constexpr int kPackWords = kPackSyms * 17 / 32 + 2;
 
 __global__ void __launch_bounds__(256) k_pack(Dev d) {
-  __shared__ uint8_t L[6][260];
-  __shared__ uint32_t C[6][260];
-  __shared__ uint16_t S[kPackSyms];
+  __shared__ uint32_t CL[6][260];  // code << 5 | length
+  __shared__ __align__(4) uint16_t S[kPackSyms + 2];
   __shared__ uint8_t T[kPackGroups];
   __shared__ uint32_t W[kPackWords];
   __shared__ int32_t tmp[8];
@@ -690,13 +689,42 @@ __global__ void __launch_bounds__(256) k_pack(Dev d) {
   const int g1 = min(((k + 1) * kTile + kGSize - 1) / kGSize, nsel);
   if (g0 >= g1) return;
   const int tid = threadIdx.x;
-  for (int q = tid; q < 6 * 260; q += 256) {
-    (&L[0][0])[q] = d.len[(int64_t)b * 6 * 260 + q];
-    (&C[0][0])[q] = d.code[(int64_t)b * 6 * 260 + q];
-  }
   const int sbase = g0 * kGSize, n = min(g1 * kGSize, s) - sbase;
-  for (int i = tid; i < n; i += 256) S[i] = d.SYM[D.off + sbase + i];
-  for (int g = tid; g < g1 - g0; g += 256) T[g] = d.sel[D.sel_off + g0 + g];
+  // all loads of the tables, symbols (two per word: sbase is even and the
+  // block's symbols start 4-byte aligned) and selectors in flight at once
+  {
+    constexpr int kTab = (6 * 260 + 255) / 256, kSw = (kPackSyms / 2 + 255) / 256;
+    const uint32_t* code = d.code + (int64_t)b * 6 * 260;
+    const uint8_t* len = d.len + (int64_t)b * 6 * 260;
+    const uint32_t* sym2 = reinterpret_cast<const uint32_t*>(d.SYM + D.off + sbase);
+    const int nw2 = (n + 1) >> 1;
+    uint32_t cv[kTab], sv[kSw];
+    uint8_t lv[kTab];
+#pragma unroll
+    for (int j = 0; j < kTab; j++) {
+      const int q = tid + 256 * j;
+      cv[j] = q < 6 * 260 ? code[q] : 0u;
+      lv[j] = q < 6 * 260 ? len[q] : (uint8_t)0;
+    }
+#pragma unroll
+    for (int j = 0; j < kSw; j++) {
+      const int q = tid + 256 * j;
+      sv[j] = q < nw2 ? sym2[q] : 0u;
+    }
+    const uint8_t tv = tid < g1 - g0 ? d.sel[D.sel_off + g0 + tid] : (uint8_t)0;
+#pragma unroll
+    for (int j = 0; j < kTab; j++) {
+      const int q = tid + 256 * j;
+      if (q < 6 * 260) (&CL[0][0])[q] = (cv[j] << 5) | lv[j];
+    }
+#pragma unroll
+    for (int j = 0; j < kSw; j++) {
+      const int q = tid + 256 * j;
+      if (q < nw2) reinterpret_cast<uint32_t*>(S)[q] = sv[j];
+    }
+    if (tid < g1 - g0) T[tid] = tv;
+  }
+  static_assert(kPackGroups <= 256, "one selector per thread");
   const int64_t a = d.gbits[D.sel_off + k];  // the tile's bit range [a, e)
   const int64_t e = g1 < nsel ? d.gbits[D.sel_off + k + 1] : d.st[b].bits;
   const int64_t w0 = a >> 5;
@@ -704,36 +732,41 @@ __global__ void __launch_bounds__(256) k_pack(Dev d) {
   for (int q = tid; q < nw; q += 256) W[q] = 0u;
   __syncthreads();
   const int i0 = tid * kPackRun, cnt = max(0, min(kPackRun, n - i0));
-  uint32_t code[kPackRun];
-  int len[kPackRun];
+  uint32_t cl[kPackRun];
   int sum = 0;
 #pragma unroll
   for (int j = 0; j < kPackRun; j++) {
-    len[j] = 0;
-    code[j] = 0;
+    cl[j] = 0;
     if (j < cnt) {
       const int i = i0 + j;
-      const int tt = T[i / kGSize];
-      const int v = S[i];
-      len[j] = L[tt][v];
-      code[j] = C[tt][v];
-      sum += len[j];
+      cl[j] = CL[T[i / kGSize]][S[i]];
+      sum += (int)(cl[j] & 31u);
     }
   }
-  int off = wg_excl_sum<256, int>(sum, tmp, nullptr) + (int)(a & 31);
+  const int off = wg_excl_sum<256, int>(sum, tmp, nullptr) + (int)(a & 31);
+  // The thread's bits [off, off + sum) gather in a 64-bit register; each
+  // completed word is stored, by atomicOr only where a neighbour's bits share
+  // it (the first word when off is not word-aligned, and the last, partial
+  // word).
+  {
+    const int wfirst = off >> 5;
+    int wi = wfirst, nb = off & 31;
+    uint64_t buf = 0;
 #pragma unroll
-  for (int j = 0; j < kPackRun; j++) {
-    if (j < cnt) {
-      const int l = len[j], sh = off & 31, wi = off >> 5;
-      const uint32_t c = code[j];
-      if (sh + l <= 32) {
-        atomicOr(&W[wi], c << (32 - sh - l));
-      } else {
-        atomicOr(&W[wi], c >> (sh + l - 32));
-        atomicOr(&W[wi + 1], c << (64 - sh - l));
+    for (int j = 0; j < kPackRun; j++) {
+      const int l = (int)(cl[j] & 31u);
+      buf = (buf << l) | (uint64_t)(cl[j] >> 5);
+      nb += l;
+      if (nb >= 32) {
+        const uint32_t word = (uint32_t)(buf >> (nb - 32));
+        if (wi == wfirst && (off & 31)) atomicOr(&W[wi], word);
+        else W[wi] = word;
+        wi++;
+        nb -= 32;
+        buf &= (1ull << nb) - 1ull;
       }
-      off += l;
     }
+    if (nb > 0 && sum > 0) atomicOr(&W[wi], (uint32_t)(buf << (32 - nb)));
   }
   __syncthreads();
   uint32_t* pay = d.pay + D.w_off;
