@@ -628,14 +628,45 @@ __global__ void __launch_bounds__(256) k_group_bits(Dev d) {
     return;
   }
   constexpr int kSyms = (kTile / kGSize + 2) * kGSize;
-  __shared__ uint16_t S[kSyms];
-  for (int q = tid; q < 6 * 260; q += 256) (&L[0][0])[q] = d.len[(int64_t)b * 6 * 260 + q];
-  for (int g = tid; g < g1 - g0; g += 256) T[g] = d.sel[D.sel_off + g0 + g];
+  constexpr int kTab = (6 * 260 + 255) / 256, kSw = (kSyms / 2 + 255) / 256;
+  static_assert(kTile / kGSize + 2 <= 256, "one selector per thread");
+  static_assert(kGSize % 2 == 0, "a symbol pair lies in one group");
   const int sbase = g0 * kGSize, n = min(g1 * kGSize, s) - sbase;
-  for (int i = tid; i < n; i += 256) S[i] = d.SYM[D.off + sbase + i];
+  // all loads in flight at once: lengths and selectors to LDS, symbol pairs
+  // (sbase is even, the block's symbols start 4-byte aligned) to registers
+  const uint8_t* len = d.len + (int64_t)b * 6 * 260;
+  const uint32_t* sym2 = reinterpret_cast<const uint32_t*>(d.SYM + D.off + sbase);
+  const int nw2 = (n + 1) >> 1;
+  uint8_t lv[kTab];
+  uint32_t sv[kSw];
+#pragma unroll
+  for (int j = 0; j < kTab; j++) {
+    const int q = tid + 256 * j;
+    lv[j] = q < 6 * 260 ? len[q] : (uint8_t)0;
+  }
+#pragma unroll
+  for (int j = 0; j < kSw; j++) {
+    const int q = tid + 256 * j;
+    sv[j] = q < nw2 ? sym2[q] : 0u;
+  }
+  const uint8_t tv = tid < g1 - g0 ? d.sel[D.sel_off + g0 + tid] : (uint8_t)0;
+#pragma unroll
+  for (int j = 0; j < kTab; j++) {
+    const int q = tid + 256 * j;
+    if (q < 6 * 260) (&L[0][0])[q] = lv[j];
+  }
+  if (tid < g1 - g0) T[tid] = tv;
   __syncthreads();
   int sum = 0;
-  for (int i = tid; i < n; i += 256) sum += L[T[i / kGSize]][S[i]];
+#pragma unroll
+  for (int j = 0; j < kSw; j++) {
+    const int q = tid + 256 * j;
+    if (q < nw2) {
+      const int tt = T[(2 * q) / kGSize];
+      sum += L[tt][sv[j] & 0xFFFFu];
+      if (2 * q + 1 < n) sum += L[tt][sv[j] >> 16];
+    }
+  }
   int tot;
   (void)wg_excl_sum<256, int>(sum, tmp, &tot);
   if (tid == 0) d.gbits[D.sel_off + k] = tot;
