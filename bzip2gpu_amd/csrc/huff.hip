@@ -449,7 +449,9 @@ __global__ void __launch_bounds__(384) k_huff_build(Dev d) {
 // (table, length) hands out consecutive codes to that length's symbols in
 // symbol order.  The selector MTF runs over 256 chunks whose entering list
 // state comes from a max-scan of per-chunk last occurrences.
+constexpr int kSelLds = 24576;
 __global__ void __launch_bounds__(256) k_huff_codes(Dev d) {
+  __shared__ uint8_t SL[kSelLds];
   __shared__ uint8_t L[6][260];
   __shared__ int32_t lcount[6][32];
   __shared__ uint32_t lbase[6][32];
@@ -497,15 +499,20 @@ __global__ void __launch_bounds__(256) k_huff_codes(Dev d) {
     }
   }
   // selector MTF
+  // the selectors, in LDS when they fit (blocks of up to ~1.2M symbols),
+  // else in the block's global scratch; their MTF indexes replace them
   const uint8_t* sel = d.sel + D.sel_off;
-  uint8_t* smtf = d.selmtf + D.sel_off;
+  uint8_t* sl = nsel <= kSelLds ? SL : d.selmtf + D.sel_off;
+#pragma unroll 16
+  for (int i = tid; i < nsel; i += 256) sl[i] = sel[i];
+  __syncthreads();
   const int per = (nsel + 255) / 256;
   const int c0 = min(nsel, tid * per), c1 = min(nsel, c0 + per);
   int lastocc[6];
 #pragma unroll
   for (int q = 0; q < 6; q++) lastocc[q] = -1;
   for (int i = c0; i < c1; i++) {
-    const int v = sel[i];
+    const int v = sl[i];
 #pragma unroll
     for (int q = 0; q < 6; q++) lastocc[q] = v == q ? i : lastocc[q];
   }
@@ -521,22 +528,25 @@ __global__ void __launch_bounds__(256) k_huff_codes(Dev d) {
     // never-seen ids rank after seen ones, in ascending order (initial list)
     if (ent[q] < 0) ent[q] = -1000000000 - q;
   }
-  uint8_t lst[6];
+  // the list as 6 nibbles (nibble q = the table id at list position q)
+  uint32_t lst = 0;
   {
     int ids[6] = {0, 1, 2, 3, 4, 5};
     for (int a2 = 0; a2 < ng; a2++)
       for (int c = a2 + 1; c < ng; c++)
         if (ent[ids[c]] > ent[ids[a2]]) { int t2 = ids[a2]; ids[a2] = ids[c]; ids[c] = t2; }
-    for (int a2 = 0; a2 < 6; a2++) lst[a2] = (uint8_t)ids[a2];
+#pragma unroll
+    for (int a2 = 0; a2 < 6; a2++) lst |= (uint32_t)ids[a2] << (4 * a2);
   }
   int my_bits = 0;
   for (int i = c0; i < c1; i++) {
-    const uint8_t v = sel[i];
-    int j = 0;
-    while (lst[j] != v) j++;
-    for (int q = j; q > 0; q--) lst[q] = lst[q - 1];
-    lst[0] = v;
-    smtf[i] = (uint8_t)j;
+    const uint32_t v = sl[i];
+    // position of v: the lowest zero nibble of lst ^ v-in-every-nibble
+    const uint32_t x = lst ^ (v * 0x111111u);
+    const int j = __builtin_ctz((x - 0x111111u) & ~x & 0x888888u) >> 2;
+    const uint32_t low = (1u << (4 * j)) - 1u;
+    lst = (lst & ~((low << 4) | 15u)) | ((lst & low) << 4) | v;
+    sl[i] = (uint8_t)j;
     my_bits += j + 1;
   }
   // header layout
@@ -582,7 +592,7 @@ __global__ void __launch_bounds__(256) k_huff_codes(Dev d) {
     const int64_t a = fixed + sb_ex;
     o.init(hw, a, a + my_bits, false);
     for (int i = c0; i < c1; i++) {
-      const int j = smtf[i];
+      const int j = sl[i];
       o.put(((1u << j) - 1) << 1, j + 1);  // j ones then a zero
     }
     o.finish();
