@@ -463,6 +463,9 @@ __global__ void __launch_bounds__(256) k_huff_codes(Dev d) {
   const BlockDesc D = d.desc[b];
   const int ng = st->n_groups, nsel = st->n_sel;
   const int alpha = st->n_in_use + 2;
+  uint32_t pres[8];  // the block's byte-presence bits, loaded once
+#pragma unroll
+  for (int q = 0; q < 8; q++) pres[q] = st->present[q];
   for (int q = tid; q < 6 * 32; q += 256) (&lcount[0][0])[q] = 0;
   __syncthreads();
   for (int q = tid; q < 6 * 260; q += 256) {
@@ -481,6 +484,7 @@ __global__ void __launch_bounds__(256) k_huff_codes(Dev d) {
     }
     // coding-table bits: 5 + sum(1 + 2*|delta|)
     int cur = L[tid][0], nb = 5;
+#pragma unroll 8
     for (int v = 0; v < alpha; v++) {
       const int tg = L[tid][v];
       nb += 1 + 2 * (tg > cur ? tg - cur : cur - tg);
@@ -494,6 +498,7 @@ __global__ void __launch_bounds__(256) k_huff_codes(Dev d) {
     if (tb < ng && l > 0 && lcount[tb][l] > 0) {
       uint32_t* code = d.code + ((int64_t)b * 6 + tb) * 260;
       uint32_t c = lbase[tb][l];
+#pragma unroll 8
       for (int v = 0; v < alpha; v++)
         if (L[tb][v] == l) code[v] = c++;
     }
@@ -529,14 +534,18 @@ __global__ void __launch_bounds__(256) k_huff_codes(Dev d) {
     if (ent[q] < 0) ent[q] = -1000000000 - q;
   }
   // the list as 6 nibbles (nibble q = the table id at list position q)
+  // (tables in use ordered by descending entering position, which are
+  // distinct; the others keep their places)
   uint32_t lst = 0;
-  {
-    int ids[6] = {0, 1, 2, 3, 4, 5};
-    for (int a2 = 0; a2 < ng; a2++)
-      for (int c = a2 + 1; c < ng; c++)
-        if (ent[ids[c]] > ent[ids[a2]]) { int t2 = ids[a2]; ids[a2] = ids[c]; ids[c] = t2; }
 #pragma unroll
-    for (int a2 = 0; a2 < 6; a2++) lst |= (uint32_t)ids[a2] << (4 * a2);
+  for (int q = 0; q < 6; q++) {
+    int pos = q;
+    if (q < ng) {
+      pos = 0;
+#pragma unroll
+      for (int r = 0; r < 6; r++) pos += (r < ng && ent[r] > ent[q]) ? 1 : 0;
+    }
+    lst |= (uint32_t)q << (4 * pos);
   }
   int my_bits = 0;
   for (int i = c0; i < c1; i++) {
@@ -552,8 +561,9 @@ __global__ void __launch_bounds__(256) k_huff_codes(Dev d) {
   // header layout
   int nused = 0;
   uint32_t l1 = 0;
+#pragma unroll
   for (int r = 0; r < 16; r++) {
-    uint32_t wv = st->present[r >> 1];
+    uint32_t wv = pres[r >> 1];
     uint32_t half = (r & 1) ? (wv >> 16) : (wv & 0xFFFF);
     if (half) { l1 |= 1u << (15 - r); nused++; }
   }
@@ -573,9 +583,10 @@ __global__ void __launch_bounds__(256) k_huff_codes(Dev d) {
     o.put(0, 1);
     o.put((uint32_t)st->orig_ptr, 24);
     o.put(l1, 16);
+#pragma unroll
     for (int r = 0; r < 16; r++) {
       if (!((l1 >> (15 - r)) & 1u)) continue;
-      uint32_t wv = st->present[r >> 1];
+      uint32_t wv = pres[r >> 1];
       uint32_t half = (r & 1) ? (wv >> 16) : (wv & 0xFFFF);
       uint32_t l2 = 0;
       for (int q = 0; q < 16; q++) if ((half >> q) & 1u) l2 |= 1u << (15 - q);
@@ -606,11 +617,20 @@ __global__ void __launch_bounds__(256) k_huff_codes(Dev d) {
     o.init(hw, a, a + tbits[tid], false);
     int cur = L[tid][0];
     o.put((uint32_t)cur, 5);
+#pragma unroll 4
     for (int v = 0; v < alpha; v++) {
       const int tg = L[tid][v];
-      while (cur < tg) { o.put(2, 2); cur++; }
-      while (cur > tg) { o.put(3, 2); cur--; }
-      o.put(0, 1);
+      const int dd = tg - cur, ad = dd < 0 ? -dd : dd;
+      if (ad <= 15) {
+        // |dd| steps "10" (up) or "11" (down), then "0", in one put
+        const uint32_t m2 = (1u << (2 * ad)) - 1u;
+        o.put((dd >= 0 ? 0xAAAAAAAAu & m2 : m2) << 1, 2 * ad + 1);
+      } else {
+        while (cur < tg) { o.put(2, 2); cur++; }
+        while (cur > tg) { o.put(3, 2); cur--; }
+        o.put(0, 1);
+      }
+      cur = tg;
     }
     o.finish();
   }
