@@ -130,6 +130,7 @@ __global__ void __launch_bounds__(256) k_plan_scan1(Dev d, int cur) {
   const int per = (n + kScanWG - 1) / kScanWG;
   const int i0 = blockIdx.x * per, i1 = min(n, i0 + per);
   int sa = 0, sb = 0, sc = 0, se = 0;
+#pragma unroll 8
   for (int i = i0 + threadIdx.x; i < i1; i += 256) {
     PlanVals v = plan_vals(d.seg[cur][i]);
     sa += v.a; sb += v.b; sc += v.c; se += v.e;
@@ -159,24 +160,59 @@ __global__ void __launch_bounds__(kScanWG) k_plan_scan2(Dev d, int cur) {
   if (threadIdx.x == 0) d.ctr[CTR_NSEG] = d.ctr[cur];
 }
 
+// Per group: exclusive prefixes of the small-group rows, large-group tiles
+// and large-group count.  Each step takes 4096 groups, group base + 256q +
+// tid in thread tid's slot q (coalesced, all loads in flight); the 64 (slot,
+// wave) runs are in group order, so wave scans, a scan over the runs' totals
+// by one wave, and the carried totals give every prefix.
 __global__ void __launch_bounds__(256) k_plan_scan3(Dev d, int cur) {
-  __shared__ int32_t tmp[8];
+  constexpr int kPer = 16;
+  __shared__ int32_t tw[3][64];
+  __shared__ int32_t tt[3];
   const int n = d.ctr[cur];
   const int per = (n + kScanWG - 1) / kScanWG;
   const int i0 = blockIdx.x * per, i1 = min(n, i0 + per);
+  const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
   int ra = d.scan_part[0 * kScanWG + blockIdx.x];
   int rb = d.scan_part[1 * kScanWG + blockIdx.x];
   int rc = d.scan_part[2 * kScanWG + blockIdx.x];
-  for (int base = i0; base < i1; base += 256) {
-    int i = base + threadIdx.x;
-    PlanVals v{0, 0, 0};
-    if (i < i1) v = plan_vals(d.seg[cur][i]);
-    int ta, tb, tc;
-    int ea = wg_excl_sum<256, int>(v.a, tmp, &ta);
-    int eb = wg_excl_sum<256, int>(v.b, tmp, &tb);
-    int ec = wg_excl_sum<256, int>(v.c, tmp, &tc);
-    if (i < i1) { d.pa[i] = ra + ea; d.pb[i] = rb + eb; d.pc[i] = rc + ec; }
-    ra += ta; rb += tb; rc += tc;
+  for (int base = i0; base < i1; base += 256 * kPer) {
+    PlanVals v[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      const int i = base + 256 * q + tid;
+      v[q] = i < i1 ? plan_vals(d.seg[cur][i]) : PlanVals{0, 0, 0, 0};
+    }
+    int xa[kPer], xb[kPer], xc[kPer];  // exclusive within the run
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      const int ia = wave_incl_sum(v[q].a), ib = wave_incl_sum(v[q].b), ic = wave_incl_sum(v[q].c);
+      xa[q] = ia - v[q].a; xb[q] = ib - v[q].b; xc[q] = ic - v[q].c;
+      if (lane == 63) { tw[0][4 * q + wv] = ia; tw[1][4 * q + wv] = ib; tw[2][4 * q + wv] = ic; }
+    }
+    __syncthreads();
+    if (wv == 0) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const int x = tw[k][lane];
+        const int inc = wave_incl_sum(x);
+        tw[k][lane] = inc - x;
+        if (lane == 63) tt[k] = inc;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      const int i = base + 256 * q + tid;
+      if (i < i1) {
+        const int j = 4 * q + wv;
+        d.pa[i] = ra + tw[0][j] + xa[q];
+        d.pb[i] = rb + tw[1][j] + xb[q];
+        d.pc[i] = rc + tw[2][j] + xc[q];
+      }
+    }
+    ra += tt[0]; rb += tt[1]; rc += tt[2];
+    __syncthreads();
   }
 }
 

@@ -76,11 +76,17 @@ __global__ void __launch_bounds__(256) k_mtf_scan(Dev d) {
   if (s == 0) d.st[b].n_in_use = tot;
   int E = present ? -1 - rank : (int)0x80000000;
   const int nt = tiles_for(m);
-  for (int k = 0; k < nt; k++) {
-    int64_t idx = (int64_t)(D.tile_c0 + k) * 256 + s;
-    int v = d.lo[idx];  // -1: not in this segment
-    d.lo[idx] = E;
-    if (present && v >= 0) E = v;  // later segments hold later positions
+  // 16 tiles' loads in flight per step (the scan itself is cheap)
+  for (int k0 = 0; k0 < nt; k0 += 16) {
+    int v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+      v[q] = k0 + q < nt ? d.lo[(int64_t)(D.tile_c0 + k0 + q) * 256 + s] : -1;  // -1: not in this segment
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      if (k0 + q < nt) d.lo[(int64_t)(D.tile_c0 + k0 + q) * 256 + s] = E;
+      if (present && v[q] >= 0) E = v[q];  // later segments hold later positions
+    }
   }
 }
 
