@@ -64,30 +64,48 @@ struct BitOut {
 
 // ------------------------------------------------------------ table build
 __global__ void __launch_bounds__(64) k_huff_init(Dev d) {
+  __shared__ int32_t fq[264];
+  __shared__ int32_t pgs[6], pge[6];
   const int b = blockIdx.x;
+  const int tid = threadIdx.x;
   BlockState* st = d.st + b;
   const int s = st->s;
   const int alpha = st->n_in_use + 2;
-  uint8_t* len = d.len + (int64_t)b * 6 * 260;
-  for (int q = threadIdx.x; q < 6 * 260; q += 64) len[q] = 15;
+  const int ng = s < 200 ? 2 : s < 600 ? 3 : s < 1200 ? 4 : s < 2400 ? 5 : 6;
+  const int32_t* freq = d.freq + (int64_t)b * 260;
+  {
+    int f[5];  // the counts staged in LDS, all loads in flight
+#pragma unroll
+    for (int r = 0; r < 5; r++) f[r] = tid + 64 * r < 260 ? freq[tid + 64 * r] : 0;
+#pragma unroll
+    for (int r = 0; r < 5; r++)
+      if (tid + 64 * r < 264) fq[tid + 64 * r] = f[r];
+  }
   // zero this block's header words (header writers use atomicOr at edges)
   uint32_t* hw = d.hdr + (int64_t)b * d.hdr_stride;
-  for (int64_t q = threadIdx.x; q < d.hdr_stride; q += 64) hw[q] = 0;
+  for (int64_t q = tid; q < d.hdr_stride; q += 64) hw[q] = 0;
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  const int ng = s < 200 ? 2 : s < 600 ? 3 : s < 1200 ? 4 : s < 2400 ? 5 : 6;
-  st->n_groups = ng;
-  st->n_sel = (s + kGSize - 1) / kGSize;
-  const int32_t* freq = d.freq + (int64_t)b * 260;
-  // initial split, huffman.cu:148-185 (bound `ge < alphabet_size`, :160)
-  int gs = 0, rem = s;
-  for (int part = ng; part > 0; part--) {
-    int tf = rem / part, ge = gs - 1, a = 0;
-    while (a < tf && ge < alpha) { ge++; a += freq[ge]; }
-    if (ge > gs && part != ng && part != 1 && ((ng - part) % 2 == 1)) { a -= freq[ge]; ge--; }
-    for (int v = 0; v < alpha; v++) len[(part - 1) * 260 + v] = (v >= gs && v <= ge) ? 0 : 15;
-    gs = ge + 1;
-    rem -= a;
+  if (tid == 0) {
+    st->n_groups = ng;
+    st->n_sel = (s + kGSize - 1) / kGSize;
+    // initial split, huffman.cu:148-185 (bound `ge < alphabet_size`, :160);
+    // table part-1 covers symbols [gs, ge]
+    int gs = 0, rem = s;
+    for (int part = ng; part > 0; part--) {
+      int tf = rem / part, ge = gs - 1, a = 0;
+      while (a < tf && ge < alpha) { ge++; a += fq[ge]; }
+      if (ge > gs && part != ng && part != 1 && ((ng - part) % 2 == 1)) { a -= fq[ge]; ge--; }
+      pgs[part - 1] = gs;
+      pge[part - 1] = ge;
+      gs = ge + 1;
+      rem -= a;
+    }
+  }
+  __syncthreads();
+  uint8_t* len = d.len + (int64_t)b * 6 * 260;
+  for (int q = tid; q < 6 * 260; q += 64) {
+    const int tb = q / 260, v = q - 260 * tb;
+    len[q] = (tb < ng && v < alpha && v >= pgs[tb] && v <= pge[tb]) ? 0 : 15;
   }
 }
 
