@@ -362,6 +362,23 @@ __device__ inline void rle2_tile(const Dev& d) {
   const int nt = tiles_for(m);
   if (WRITE)
     for (int q = tid; q < 4 * 260; q += 256) (&hist[0][0])[q] = 0;
+  // the tile's bytes (and the byte before each thread's) are loaded before
+  // the carry lookup so that the latencies overlap
+  const int n = min(kTile, m - start);
+  const uint8_t* src = d.MTF + D.off + start;  // 256-byte aligned
+  const int q0 = tid * 16;
+  const int p0 = start + q0;
+  uint8_t x[16];
+  if (q0 + 16 <= n) {
+    const uint4 v = *reinterpret_cast<const uint4*>(src + q0);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 16; q++) x[q] = (uint8_t)(w[q >> 2] >> (8 * (q & 3)));
+  } else {
+#pragma unroll
+    for (int q = 0; q < 16; q++) x[q] = (q0 + q < n) ? src[q0 + q] : 1;
+  }
+  const bool prev_nz = q0 > 0 ? src[q0 - 1] != 0 : (start > 0 ? src[-1] != 0 : true);
   {
     int f = 0x7fffffff;
     for (int j = k + 1 + tid; j < nt; j += 256) {
@@ -377,20 +394,6 @@ __device__ inline void rle2_tile(const Dev& d) {
     }
     __syncthreads();
   }
-  const int n = min(kTile, m - start);
-  const uint8_t* src = d.MTF + D.off + start;  // 256-byte aligned
-  const int q0 = tid * 16;
-  const int p0 = start + q0;
-  uint8_t x[16];
-  if (q0 + 16 <= n) {
-    const uint4 v = *reinterpret_cast<const uint4*>(src + q0);
-    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-    for (int q = 0; q < 16; q++) x[q] = (uint8_t)(w[q >> 2] >> (8 * (q & 3)));
-  } else {
-#pragma unroll
-    for (int q = 0; q < 16; q++) x[q] = (q0 + q < n) ? src[q0 + q] : 1;
-  }
   // nonzero positions (positions past the tile count as nonzero)
   uint32_t nzm = 0;
 #pragma unroll
@@ -401,7 +404,6 @@ __device__ inline void rle2_tile(const Dev& d) {
   xe[tid] = wg_rev_incl_min<256, int>(firstNZ, tmp, carry);
   __syncthreads();
   const int es = tid < 255 ? xe[tid + 1] : carry;
-  const bool prev_nz = q0 > 0 ? src[q0 - 1] != 0 : (start > 0 ? src[-1] != 0 : true);
   // symbols per position: nonzero v -> 1 (v+1); the first zero of a run of
   // length L -> the bits of L+1 below its leading one
   int e_of[16];
