@@ -272,6 +272,7 @@ int Engine::setup(const uint8_t* d_in, hipStream_t s) {
     d.pb = c.take<int32_t>(seg_cap);
     d.pc = c.take<int32_t>(seg_cap);
     d.packs = c.take<int32_t>(pack_cap + 2);
+    d.horder = c.take<int32_t>(B + 2);
     d.ltile_seg = c.take<int32_t>(ltile_cap);
     d.ltile_pos = c.take<int32_t>(ltile_cap);
     d.llist = c.take<int32_t>(ltile_cap);

@@ -393,12 +393,33 @@ __device__ void build_lengths(HeapLds& H, const int32_t* fr, uint8_t* len, int a
 // 64-bit packed lengths, first strict minimum, counts in LDS), then wave t
 // rebuilds table t's lengths.  A block's tables never wait for another
 // block's.
+// Launch order of the code-length builds: blocks by descending alphabet
+// size (the heap work grows with it).  More blocks than CUs means some CUs
+// run two builds at once; the builds launched first are paired with those
+// launched last, so the largest alphabets share a CU with the smallest,
+// which finish early.  A counting sort in one workgroup; the order within
+// an alphabet size does not matter.
+__global__ void __launch_bounds__(256) k_huff_order(Dev d) {
+  __shared__ int32_t cnt[260];
+  const int tid = threadIdx.x;
+  for (int q = tid; q < 260; q += 256) cnt[q] = 0;
+  __syncthreads();
+  for (int b = tid; b < d.nblk; b += 256) atomicAdd(&cnt[259 - min(d.st[b].n_in_use + 2, 259)], 1);
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int q = 0; q < 260; q++) { const int c = cnt[q]; cnt[q] = run; run += c; }
+  }
+  __syncthreads();
+  for (int b = tid; b < d.nblk; b += 256) d.horder[atomicAdd(&cnt[259 - min(d.st[b].n_in_use + 2, 259)], 1)] = b;
+}
+
 __global__ void __launch_bounds__(384) k_huff_build(Dev d) {
   __shared__ uint64_t PL[260];
   __shared__ int32_t rf[6][260];
   __shared__ uint8_t L[6][260];
   __shared__ HeapLds H[6];
-  const int b = blockIdx.x;
+  const int b = d.horder[blockIdx.x];
   const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
   const BlockDesc D = d.desc[b];
   const BlockState* st = d.st + b;
@@ -1002,6 +1023,7 @@ __global__ void __launch_bounds__(64) k_trailer(Dev d) {
 
 void launch_huffman(const Dev& d, hipStream_t s) {
   k_huff_init<<<d.nblk, 64, 0, s>>>(d);
+  k_huff_order<<<1, 256, 0, s>>>(d);
   probe_begin("k_huff_build", s);
   k_huff_build<<<d.nblk, 384, 0, s>>>(d);
   probe_end("k_huff_build", s);

@@ -81,6 +81,7 @@ struct Dev {
   int32_t* pb;         // per-seg exclusive prefix of large-group tile counts
   int32_t* pc;         // per-seg exclusive prefix of large-group count
   int32_t* packs;      // pack -> first seg index
+  int32_t* horder;     // blocks in code-length build order (largest alphabet first)
   int32_t* ltile_seg;  // large tile -> seg index
   int32_t* ltile_pos;  // large tile -> first element within its seg
   int32_t* llist;      // compact list of large seg indices
