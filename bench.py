@@ -51,10 +51,11 @@ def _args():
 
 
 # Algorithmic HBM bytes of one unit of work of each probed kernel, and the
-# name of the unit (per-unit figures restated in DESIGN.md).
+# name of the unit.
 ALG_BYTES = {
-    # one LSD radix pass of the large-group suffix sort: read key 4 + SA 4,
-    # write key 4 + SA 4 per row and pass
+    # one LSD radix pass of the large-group suffix sort that reads keys: read
+    # key 4 + SA 4, write key 4 + SA 4 per row and pass (round 0's first pass
+    # reads the text instead and is not probed)
     "k_lscatter": (16, "radix_pass_rows"),
     # LDS sort of a pack: read key 4 + SA 4, write SA 4 + scatter rank 4
     "k_sort_small": (16, "bwt_rows_small"),
@@ -353,7 +354,7 @@ def _cpu_baseline(data, level, block_bytes):
 def main():
     a = _args()
     if a.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (asked for {a.gpus}); multi-GPU runs are not part of this build",
+        print(f"bench.py runs on exactly one GPU (asked for {a.gpus}); runs over several GPUs are not supported",
               file=sys.stderr)
         sys.exit(2)
     if a.workload == "batch4096":
