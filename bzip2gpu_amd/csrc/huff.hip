@@ -389,7 +389,7 @@ __device__ void build_lengths(HeapLds& H, const int32_t* fr, uint8_t* len, int a
 }  // namespace
 
 // The four selection / code-length iterations of one block in one
-// workgroup of six waves: all threads assign 50-symbol groups (costs from
+// workgroup of eight waves: all threads assign 50-symbol groups (costs from
 // 64-bit packed lengths, first strict minimum, counts in LDS), then wave t
 // rebuilds table t's lengths.  A block's tables never wait for another
 // block's.
@@ -414,7 +414,8 @@ __global__ void __launch_bounds__(256) k_huff_order(Dev d) {
   for (int b = tid; b < d.nblk; b += 256) d.horder[atomicAdd(&cnt[259 - min(d.st[b].n_in_use + 2, 259)], 1)] = b;
 }
 
-__global__ void __launch_bounds__(384) k_huff_build(Dev d) {
+constexpr int kHuffThreads = 512;  // 8 waves: all select, waves 0..5 build
+__global__ void __launch_bounds__(kHuffThreads) k_huff_build(Dev d) {
   __shared__ uint64_t PL[260];
   __shared__ int32_t rf[6][260];
   __shared__ uint8_t L[6][260];
@@ -426,21 +427,21 @@ __global__ void __launch_bounds__(384) k_huff_build(Dev d) {
   const int ng = st->n_groups, nsel = st->n_sel, s = st->s;
   const int alpha = st->n_in_use + 2;
   uint8_t* glen = d.len + (int64_t)b * 6 * 260;
-  for (int q = tid; q < 6 * 260; q += 384) (&L[0][0])[q] = glen[q];
+  for (int q = tid; q < 6 * 260; q += kHuffThreads) (&L[0][0])[q] = glen[q];
   __syncthreads();
   const uint32_t* sym2 = reinterpret_cast<const uint32_t*>(d.SYM + D.off);  // 2 symbols per word
   for (int it = 0; it < 4; it++) {
-    for (int v = tid; v < 260; v += 384) {
+    for (int v = tid; v < 260; v += kHuffThreads) {
       uint64_t w = 0;
 #pragma unroll
       for (int tt = 0; tt < 6; tt++) w |= (uint64_t)L[tt][v] << (10 * tt);
       PL[v] = w;
     }
-    for (int q = tid; q < 6 * 260; q += 384) (&rf[0][0])[q] = 0;
+    for (int q = tid; q < 6 * 260; q += kHuffThreads) (&rf[0][0])[q] = 0;
     __syncthreads();
     // every group but a block's last holds kGSize symbols: that path runs
     // without per-symbol guards
-    for (int g = tid; g < nsel; g += 384) {
+    for (int g = tid; g < nsel; g += kHuffThreads) {
       const int gs = g * kGSize;
       const bool full = gs + kGSize <= s;
       const int ge = full ? gs + kGSize : s;
@@ -479,7 +480,7 @@ __global__ void __launch_bounds__(384) k_huff_build(Dev d) {
     if (wv < ng) build_lengths(H[wv], rf[wv], L[wv], alpha, lane);
     __syncthreads();
   }
-  for (int q = tid; q < 6 * 260; q += 384) glen[q] = (&L[0][0])[q];
+  for (int q = tid; q < 6 * 260; q += kHuffThreads) glen[q] = (&L[0][0])[q];
 }
 
 // Codes (huffman.cu:135-146, :352-362), selector MTF (compress.cu:119-137)
@@ -1025,7 +1026,7 @@ void launch_huffman(const Dev& d, hipStream_t s) {
   k_huff_init<<<d.nblk, 64, 0, s>>>(d);
   k_huff_order<<<1, 256, 0, s>>>(d);
   probe_begin("k_huff_build", s);
-  k_huff_build<<<d.nblk, 384, 0, s>>>(d);
+  k_huff_build<<<d.nblk, kHuffThreads, 0, s>>>(d);
   probe_end("k_huff_build", s);
   k_huff_codes<<<d.nblk, 256, 0, s>>>(d);
 }
