@@ -95,19 +95,19 @@ int ensure_input(Ctx* c, size_t bytes) {
   return 0;
 }
 
+// Grow-only result buffer of the multi-sub-batch path; its contents need
+// not survive a growth (a call sizes it before any sub-batch runs).
 int ensure_result(Ctx* c, size_t bytes) {
   if (bytes <= c->d_res_cap) return 0;
-  uint8_t* nb = nullptr;
-  size_t want = bytes + bytes / 4 + 4096;
-  hipError_t e = hipMalloc(&nb, want);
-  if (e != hipSuccess) return (int)e;
   if (c->d_res) {
-    // keep what earlier sub-batches of this call already placed
-    e = hipMemcpy(nb, c->d_res, c->d_res_cap, hipMemcpyDeviceToDevice);
-    if (e == hipSuccess) e = hipFree(c->d_res);
+    hipError_t e = hipFree(c->d_res);
     if (e != hipSuccess) return (int)e;
   }
-  c->d_res = nb;
+  c->d_res = nullptr;
+  c->d_res_cap = 0;
+  const size_t want = bytes + bytes / 8 + 4096;
+  hipError_t e = hipMalloc(&c->d_res, want);
+  if (e != hipSuccess) return (int)e;
   c->d_res_cap = want;
   return 0;
 }
@@ -232,27 +232,45 @@ bzg_status bzg_compress_batch_device(const uint8_t* d_in, int32_t n_inputs, cons
   if (hipSetDevice(c->device) != hipSuccess) return fail(BZG_ERR_HIP, "hipSetDevice failed");
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   // Inputs are taken in order in sub-batches of at most max_batch_bytes raw
-  // bytes (bounded workspace); every sub-batch's streams are copied into
-  // the library-owned result buffer, 4-byte aligned, in input order.
+  // bytes (bounded workspace).  With more than one sub-batch, each one
+  // assembles its streams directly into its own region of the
+  // library-owned result buffer (sized from every sub-batch's plan first),
+  // so no stream is copied afterwards.
   const uint64_t max_batch = c->max_batch_bytes;
-  std::vector<StreamSpec> ss;
-  std::vector<StreamState> lay;
-  int64_t out_pos = 0;
-  int e;
-  StageTimes sum{};
-  int i0 = 0;
-  while (i0 < n_inputs) {
-    ss.clear();
+  std::vector<std::pair<int, int>> parts;  // [i0, i1) per sub-batch
+  for (int i0 = 0; i0 < n_inputs;) {
     uint64_t tot = 0;
     int i1 = i0;
-    while (i1 < n_inputs && (i1 == i0 || tot + lens[i1] <= max_batch)) {
-      int lv = clamp_level(levels[i1]);
-      ss.push_back(StreamSpec{in_off[i1], lens[i1], lv, block_bytes > 0 ? block_bytes : 100000 * lv});
-      tot += lens[i1];
-      i1++;
+    while (i1 < n_inputs && (i1 == i0 || tot + lens[i1] <= max_batch)) tot += lens[i1++];
+    parts.emplace_back(i0, i1);
+    i0 = i1;
+  }
+  auto specs = [&](int i0, int i1, std::vector<StreamSpec>& ss) {
+    ss.clear();
+    for (int i = i0; i < i1; i++) {
+      const int lv = clamp_level(levels[i]);
+      ss.push_back(StreamSpec{in_off[i], lens[i], lv, block_bytes > 0 ? block_bytes : 100000 * lv});
     }
+  };
+  std::vector<StreamSpec> ss;
+  std::vector<StreamState> lay;
+  std::vector<int64_t> region(parts.size() + 1, 0);  // result-buffer offset of each sub-batch
+  int e;
+  if (parts.size() > 1) {
+    for (size_t p = 0; p < parts.size(); p++) {
+      specs(parts[p].first, parts[p].second, ss);
+      if ((e = c->eng.plan(ss))) return hip_fail(e, "plan");
+      region[p + 1] = region[p] + ((c->eng.planned_out_bytes() + 255) & ~255ll);
+    }
+    if ((e = ensure_result(c, (size_t)region.back()))) return hip_fail(e, "result buffer");
+  }
+  StageTimes sum{};
+  for (size_t p = 0; p < parts.size(); p++) {
+    const int i0 = parts[p].first, i1 = parts[p].second;
+    specs(i0, i1, ss);
+    uint8_t* dst = parts.size() > 1 ? c->d_res + region[p] : nullptr;
     if ((e = c->eng.plan(ss))) return hip_fail(e, "plan");
-    if ((e = c->eng.setup(d_in, s))) return hip_fail(e, "setup");
+    if ((e = c->eng.setup(d_in, s, dst))) return hip_fail(e, "setup");
     StageTimes t{};
     if ((e = c->eng.run(s, true, &t))) return hip_fail(e, "run");
     sum.front += t.front; sum.bwt += t.bwt; sum.mtf += t.mtf; sum.huff += t.huff;
@@ -262,34 +280,50 @@ bzg_status bzg_compress_batch_device(const uint8_t* d_in, int32_t n_inputs, cons
     sum.large_rows += t.large_rows;
     sum.large_pass_rows += t.large_pass_rows;
     if ((e = c->eng.stream_layout(lay, s))) return hip_fail(e, "layout");
-    const bool single = (i0 == 0 && i1 == n_inputs);
-    if (single) {
-      for (int i = i0; i < i1; i++) {
-        h_byte_off[i] = lay[i - i0].byte_off;
-        h_bytes[i] = lay[i - i0].bytes;
-      }
-      *d_out = c->eng.out_buffer();
-    } else {
-      int64_t need = 0;
-      for (auto& L : lay) need += (L.bytes + 3) & ~3ll;
-      if ((e = ensure_result(c, (size_t)(out_pos + need)))) return hip_fail(e, "result buffer");
-      for (int i = i0; i < i1; i++) {
-        const StreamState& L = lay[i - i0];
-        hipError_t he = hipMemcpyAsync(c->d_res + out_pos, c->eng.out_buffer() + L.byte_off, (size_t)L.bytes,
-                                       hipMemcpyDeviceToDevice, s);
-        if (he != hipSuccess) return hip_fail((int)he, "result copy");
-        h_byte_off[i] = out_pos;
-        h_bytes[i] = L.bytes;
-        out_pos += (L.bytes + 3) & ~3ll;
-      }
-      *d_out = c->d_res;
+    for (int i = i0; i < i1; i++) {
+      h_byte_off[i] = region[p] + lay[i - i0].byte_off;
+      h_bytes[i] = lay[i - i0].bytes;
     }
-    i0 = i1;
   }
-  if (n_inputs == 0) *d_out = c->eng.out_buffer();
+  *d_out = parts.size() > 1 ? c->d_res : c->eng.out_buffer();
   hipError_t he = hipStreamSynchronize(s);
   if (he != hipSuccess) return hip_fail((int)he, "batch");
   c->last = sum;
+  return BZG_OK;
+}
+
+bzg_status bzg_compress_segment(const uint8_t* d_in, size_t len, int32_t block_bytes, int32_t lead_bits,
+                                uint32_t* h_crc, uint8_t* h_out, size_t out_cap, size_t* out_len, int64_t* h_bits,
+                                void* stream) {
+  if ((!d_in && len) || !h_crc || !h_out || !out_len || !h_bits) return fail(BZG_ERR_INVALID, "null pointer");
+  if (block_bytes <= 0 || block_bytes > BZG_REF_BLOCK_BYTES) return fail(BZG_ERR_INVALID, "block_bytes out of range");
+  if (lead_bits < 0 || lead_bits > 7) return fail(BZG_ERR_INVALID, "lead_bits must be 0..7");
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(BZG_ERR_HIP, "hipSetDevice failed");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  StreamSpec spec{0, len, 9, block_bytes};
+  spec.lead = lead_bits;
+  spec.crc_in = *h_crc;
+  std::vector<StreamSpec> ss{spec};
+  std::vector<StreamState> lay;
+  int e;
+  if ((e = c->eng.plan(ss))) return hip_fail(e, "plan");
+  if ((e = c->eng.setup(d_in, s))) return hip_fail(e, "setup");
+  if ((e = c->eng.run(s, true, &c->last))) return hip_fail(e, "run");
+  if ((e = c->eng.stream_layout(lay, s))) return hip_fail(e, "layout");
+  const size_t n = (size_t)lay[0].bytes;
+  *out_len = n;
+  if (n > out_cap) return fail(BZG_ERR_CAPACITY, "output buffer too small");
+  if (n) {
+    hipError_t he = hipMemcpyAsync(h_out, c->eng.out_buffer() + lay[0].byte_off, n, hipMemcpyDeviceToHost, s);
+    if (he == hipSuccess) he = hipStreamSynchronize(s);
+    if (he != hipSuccess) return hip_fail((int)he, "D2H");
+  }
+  *h_crc = lay[0].crc;
+  *h_bits = lay[0].bits;
   return BZG_OK;
 }
 
@@ -345,6 +379,7 @@ bzg_status bzg_server_wait(bzg_server* srv, int64_t ticket, int64_t* h_out_bytes
   if (!srv) return fail(BZG_ERR_INVALID, "bad argument");
   bzg::Server::Req r;
   int e = srv->s->wait(ticket, &r);
+  if (e == bzg::Server::kUnknownTicket) return fail(BZG_ERR_INVALID, "unknown or already waited-for ticket");
   if (e) return hip_fail(e, "server request");
   if (h_out_bytes) *h_out_bytes = r.out_bytes;
   if (h_out_off) *h_out_off = r.out_off;
@@ -380,6 +415,7 @@ bzg_status bzg_server_replay(bzg_server* srv, const uint8_t* d_pool, int32_t n, 
   for (int32_t i = 0; i < n; i++) {
     bzg::Server::Req r;
     int e = S.wait(ids[i], &r);
+    if (e == bzg::Server::kUnknownTicket) return fail(BZG_ERR_INTERNAL, "replay request record lost");
     if (e) return hip_fail(e, "server request");
     h_done_ns[i] = r.done_ns;
     if (h_out_bytes) h_out_bytes[i] = r.out_bytes;
@@ -821,9 +857,11 @@ bzg_status bzg_huff_encode(const uint16_t* d_sym, int32_t s, const uint8_t* h_le
   if (s <= 0 || !d_sym || !h_len || !h_code || !d_sel || !d_words || !h_bits) return fail(BZG_ERR_INVALID, "bad argument");
   if (s > kStageMax || n_groups < 1 || n_groups > 6 || n_sel != (s + 49) / 50)
     return fail(BZG_ERR_INVALID, "size out of range");
+  // the encoder's longest code is 17 bits (huffman.cu:321); the payload
+  // capacity of a block is sized for that
   for (int g = 0; g < n_groups; g++)
     for (int v = 0; v < 258; v++)
-      if (h_len[g * 258 + v] > 20) return fail(BZG_ERR_INVALID, "code length out of range");
+      if (h_len[g * 258 + v] > 17) return fail(BZG_ERR_INVALID, "code length above 17");
   StageLock L;
   bzg_status st = stage_begin(L, stream, s);
   if (st) return st;
@@ -844,6 +882,11 @@ bzg_status bzg_huff_encode(const uint16_t* d_sym, int32_t s, const uint8_t* h_le
   ck((hipError_t)set_field<int32_t>(d, offsetof(BlockState, s), s, L.s));
   ck((hipError_t)set_field<int32_t>(d, offsetof(BlockState, n_sel), n_sel, L.s));
   ck((hipError_t)set_field<int32_t>(d, offsetof(BlockState, n_groups), n_groups, L.s));
+  launch_check_codes(d, n_groups, L.s);
+  int32_t bad = 0;
+  ck(hipMemcpyAsync(&bad, &d.st[0].err, 4, hipMemcpyDeviceToHost, L.s));
+  if ((e = sync(L.s))) return hip_fail(e, "huff_encode");
+  if (bad) return fail(BZG_ERR_INVALID, "symbol >= 258 or selector >= n_groups");
   launch_pack(d, L.s);
   if ((e = (int)hipGetLastError())) return hip_fail(e, "huff_encode");
   int64_t bits;

@@ -64,15 +64,22 @@ struct BlockState {
   int32_t pad_;
 };
 
-// Per-output-stream layout (device-written by the layout kernel).
+// Per-output-stream layout (device-written by the layout kernel).  A whole
+// stream (lead < 0) has the 4-byte header, its blocks and the end-of-stream
+// trailer.  A segment of a stream written over several pipeline runs
+// (lead = 0..7) has only its blocks' bits, starting `lead` bits into its
+// first byte (the bits the earlier segments left in their last byte), and
+// carries the combined CRC on from crc_in.
 struct StreamState {
-  int64_t bits;        // total bits incl. stream header/trailer, before pad
+  int64_t bits;        // total bits incl. stream header/trailer (or lead), before pad
   int64_t byte_off;    // 4-byte aligned offset into the output buffer
   int64_t bytes;       // stream length in bytes
-  uint32_t crc;        // combined stream CRC
+  uint32_t crc;        // combined stream CRC (after this segment's blocks)
   int32_t first_blk;   // first block index (blocks of a stream are consecutive)
   int32_t n_blk;
   int32_t level;
+  int32_t lead;        // -1: whole stream; 0..7: segment starting at this bit
+  uint32_t crc_in;     // combined CRC of the blocks before a segment
 };
 
 // ------------------------------------------------------------- lane helpers

@@ -3,10 +3,12 @@
 // the reference (see compression.h).
 #include "compression.h"
 
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <map>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "bzg.h"
@@ -113,40 +115,144 @@ int huffman_encode(uint16_t *dev_data_in, int data_in_len, int /*alphabet_size*/
   return (int)bits;
 }
 
-// compress.cu:184-375 - fetch every chunk in chunk order, then one batched
-// pipeline call with the chunk size as block size (the chunk cut IS the
-// block cut in the reference, io.cpp:17-18).  Appends to `out`.
-void bzip2_gpu_compress(BZFileInputStream &in_stream, int n, std::vector<uint8_t> &out) {
+// compress.cu:184-375 - appends one .bz2 stream to `out`.  The chunk cut IS
+// the block cut (io.cpp:17-18): chunks are fetched in chunk order and copied
+// into one of two device staging buffers as they arrive; every filled
+// buffer is compressed as one segment of the stream (bzg_compress_segment)
+// on a worker thread while the other buffer fills, so reading, host-to-
+// device copies and compression overlap and host memory holds only the
+// ring's chunks and the output.
+namespace {
+
+constexpr size_t kDefaultSegmentBytes = size_t(256) << 20;
+
+struct Staging {
+  uint8_t *d = nullptr;
+  size_t len = 0;
+  bool ready = false;  // filled, waiting for the worker
+};
+
+// MSB-first bits appended to a byte vector whose last byte already holds
+// `used` bits (0 = byte-aligned).
+struct TailBits {
+  std::vector<uint8_t> &out;
+  int used;
+  void put(uint32_t v, int k) {
+    for (int i = k - 1; i >= 0; i--) {
+      if (used == 0) out.push_back(0);
+      if ((v >> i) & 1u) out.back() |= (uint8_t)(0x80u >> used);
+      used = (used + 1) & 7;
+    }
+  }
+};
+
+}  // namespace
+
+void bzip2_gpu_compress_segmented(BZFileInputStream &in_stream, int n, std::vector<uint8_t> &out,
+                                  size_t segment_bytes) {
   if (n < 1) n = 1;
   if (n > 9) n = 9;
-  std::map<int, std::pair<uint8_t *, int>> chunks;
-  uint8_t *buf;
-  int idx;
-  int len;
-  while ((len = in_stream.fetch(buf, idx)) >= 0) chunks[idx] = {buf, len};
-  if (in_stream.failed) {
-    fprintf(stderr, "bzip2_gpu_compress: input could not be read\n");
-    exit(EXIT_FAILURE);
-  }
-  std::vector<uint8_t> all;
-  size_t total = 0;
-  for (auto &kv : chunks) total += (size_t)kv.second.second;
-  all.reserve(total);
-  for (auto &kv : chunks) {
-    all.insert(all.end(), kv.second.first, kv.second.first + kv.second.second);
-    (void)hipHostFree(kv.second.first);
-  }
-  int32_t bb = in_stream.buffer_size;
+  const int32_t bb = in_stream.buffer_size;
   if (bb <= 0 || bb > BZG_REF_BLOCK_BYTES) {
     fprintf(stderr, "bzip2_gpu_compress: chunk size must be 1..900000\n");
     exit(EXIT_FAILURE);
   }
-  size_t cap = bzg_compress_bound(total, bb);
-  size_t base = out.size();
-  out.resize(base + cap);
-  size_t got = 0;
-  check(bzg_compress_buffer(all.data(), total, n, bb, out.data() + base, cap, &got), "bzip2_gpu_compress");
-  out.resize(base + got);
+  const size_t per = segment_bytes / (size_t)bb > 0 ? segment_bytes / (size_t)bb : 1;  // chunks per segment
+  const size_t cap = per * (size_t)bb;
+  out.push_back('B');
+  out.push_back('Z');
+  out.push_back('h');
+  out.push_back((uint8_t)('0' + n));
+  hipStream_t copy_s = nullptr, comp_s = nullptr;
+  check_hip(hipStreamCreateWithFlags(&copy_s, hipStreamNonBlocking), "hipStreamCreate");
+  check_hip(hipStreamCreateWithFlags(&comp_s, hipStreamNonBlocking), "hipStreamCreate");
+  Staging buf[2];
+  std::mutex mu;
+  std::condition_variable cv;
+  bool done = false;
+  uint32_t crc = 0;
+  int used = 0;  // bits of out.back() the stream already holds
+  std::thread worker([&] {
+    std::vector<uint8_t> seg;
+    for (int k = 0;; k ^= 1) {
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return buf[k].ready || done; });
+        if (!buf[k].ready) return;  // done, nothing left
+      }
+      seg.resize(bzg_compress_bound(buf[k].len, bb));
+      size_t got = 0;
+      int64_t bits = 0;
+      check(bzg_compress_segment(buf[k].d, buf[k].len, bb, used, &crc, seg.data(), seg.size(), &got, &bits, comp_s),
+            "bzip2_gpu_compress");
+      // the segment's first `used` bits are zero: they merge into out.back()
+      size_t from = 0;
+      if (used > 0 && got > 0) {
+        out.back() |= seg[0];
+        from = 1;
+      }
+      out.insert(out.end(), seg.begin() + (std::ptrdiff_t)from, seg.begin() + (std::ptrdiff_t)got);
+      used = (int)(bits & 7);
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        buf[k].ready = false;
+      }
+      cv.notify_all();
+    }
+  });
+  bool eof = false;
+  for (int k = 0; !eof; k ^= 1) {
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait(lk, [&] { return !buf[k].ready; });
+    }
+    size_t len = 0;
+    for (size_t c = 0; c < per; c++) {
+      uint8_t *chunk;
+      int idx;
+      const int l = in_stream.fetch(chunk, idx);
+      if (l < 0) {
+        eof = true;
+        break;
+      }
+      if (!buf[k].d) check_hip(hipMalloc((void **)&buf[k].d, cap), "hipMalloc");
+      check_hip(hipMemcpyAsync(buf[k].d + len, chunk, (size_t)l, hipMemcpyHostToDevice, copy_s), "hipMemcpyAsync");
+      check_hip(hipStreamSynchronize(copy_s), "hipStreamSynchronize");
+      (void)hipHostFree(chunk);
+      len += (size_t)l;
+    }
+    if (len > 0) {
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        buf[k].len = len;
+        buf[k].ready = true;
+      }
+      cv.notify_all();
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    done = true;
+  }
+  cv.notify_all();
+  worker.join();
+  for (auto &b : buf)
+    if (b.d) (void)hipFree(b.d);
+  (void)hipStreamDestroy(copy_s);
+  (void)hipStreamDestroy(comp_s);
+  if (in_stream.failed) {
+    fprintf(stderr, "bzip2_gpu_compress: input could not be read\n");
+    exit(EXIT_FAILURE);
+  }
+  // end-of-stream magic, combined CRC, zero padding (compress.cu:367-375)
+  TailBits tb{out, used};
+  tb.put(0x177245u, 24);
+  tb.put(0x385090u, 24);
+  tb.put(crc, 32);
+}
+
+void bzip2_gpu_compress(BZFileInputStream &in_stream, int n, std::vector<uint8_t> &out) {
+  bzip2_gpu_compress_segmented(in_stream, n, out, kDefaultSegmentBytes);
 }
 
 int bzip2_gpu_decompress(const uint8_t *in, size_t len, std::vector<uint8_t> &out, std::string *err) {

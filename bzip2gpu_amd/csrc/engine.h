@@ -14,11 +14,15 @@
 namespace bzg {
 
 // One output stream: raw bytes [in_off, in_off+len) of the device input.
+// lead >= 0 makes it a segment of a longer stream instead (blocks only, see
+// StreamState), with crc_in the combined CRC of the blocks before it.
 struct StreamSpec {
   uint64_t in_off;
   uint64_t len;
   int level;        // header digit 1..9
   int block_bytes;  // raw bytes per block (900000 = the reference's cut)
+  int lead = -1;
+  uint32_t crc_in = 0;
 };
 
 // Per-block result copied back to the host (the reference's BlockData,
@@ -69,7 +73,11 @@ class Engine {
   // allow_big lets one block exceed 900000 raw bytes (stage entry points).
   int plan(const std::vector<StreamSpec>& streams, bool allow_big = false);
   // Allocates (grow-only) and uploads the plan; d_in holds the raw bytes.
-  int setup(const uint8_t* d_in, hipStream_t s);
+  // With `out` the streams are assembled there (planned_out_bytes() bytes,
+  // 4-byte aligned) instead of in the workspace's own output area.
+  int setup(const uint8_t* d_in, hipStream_t s, uint8_t* out = nullptr);
+  // Device bytes the planned streams may take (worst case).
+  int64_t planned_out_bytes() const { return out_cap_ + 64; }
   // Full pipeline on the planned batch.  With assemble = true the .bz2
   // streams are written to the device output buffer.
   int run(hipStream_t s, bool assemble, StageTimes* times = nullptr);
@@ -110,11 +118,16 @@ class Engine {
   int64_t graph_replays(int i) const { return sgraph_[i].replays; }
 
  private:
-  int grow(size_t bytes);
+  int grow(size_t bytes, hipStream_t s);
   int run_stage(int idx, hipStream_t s, int (Engine::*fn)(hipStream_t));
+  // Captured graphs of a stage group, one per recent device-side plan (a
+  // batch split into sub-batches cycles through several plans per call).
+  static constexpr int kGraphCache = 8;
   struct StageGraph {
-    hipGraphExec_t exec = nullptr;
-    Dev key{};
+    hipGraphExec_t exec[kGraphCache] = {};
+    Dev key[kGraphCache] = {};
+    int64_t used[kGraphCache] = {};  // last use, for eviction
+    int64_t clock = 0;
     int64_t captures = 0, replays = 0;
   };
   StageGraph sgraph_[5];

@@ -130,7 +130,10 @@ Engine::Engine(int device) : device_(device) {
 
 Engine::~Engine() {
   (void)hipSetDevice(device_);
-  if (ws_) (void)hipFree(ws_);
+  if (ws_) {  // allocated stream-ordered (grow)
+    (void)hipFreeAsync(ws_, nullptr);
+    (void)hipStreamSynchronize(nullptr);
+  }
   if (h_ctr_) (void)hipHostFree(h_ctr_);
   for (auto& ev : ev_)
     if (ev) (void)hipEventDestroy(ev);
@@ -141,7 +144,8 @@ Engine::~Engine() {
   for (auto& ev : tev_)
     for (auto e : ev) (void)hipEventDestroy(e);
   for (auto& g : sgraph_)
-    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    for (auto x : g.exec)
+      if (x) (void)hipGraphExecDestroy(x);
 }
 
 int Engine::plan(const std::vector<StreamSpec>& streams, bool allow_big) {
@@ -161,6 +165,9 @@ int Engine::plan(const std::vector<StreamSpec>& streams, bool allow_big) {
     StreamState st{};
     st.first_blk = (int32_t)desc_.size();
     st.level = level;
+    if (S.lead > 7) return (int)hipErrorInvalidValue;
+    st.lead = S.lead < 0 ? -1 : S.lead;
+    st.crc_in = S.crc_in;
     int64_t sbytes = 16;
     for (uint64_t off = 0; off < S.len; off += (uint64_t)bb) {
       BlockDesc D{};
@@ -193,16 +200,19 @@ int Engine::plan(const std::vector<StreamSpec>& streams, bool allow_big) {
   return 0;
 }
 
-int Engine::grow(size_t bytes) {
+// Stream-ordered growth: the old workspace is released and the new one
+// allocated in order on `s` (every earlier use of the workspace was on a
+// stream the caller synchronised before this call), so that one engine's
+// growth does not wait for other engines' work on the device.
+int Engine::grow(size_t bytes, hipStream_t s) {
   if (bytes <= ws_bytes_) return 0;
   if (ws_) {
-    BZG_TRY(hipDeviceSynchronize());
-    BZG_TRY(hipFree(ws_));
+    BZG_TRY(hipFreeAsync(ws_, s));
     ws_ = nullptr;
     ws_bytes_ = 0;
   }
   size_t want = bytes + bytes / 8;
-  BZG_TRY(hipMalloc(&ws_, want));
+  BZG_TRY(hipMallocAsync(&ws_, want, s));
   ws_bytes_ = want;
   return 0;
 }
@@ -222,7 +232,7 @@ struct Carver {
 }  // namespace
 
 
-int Engine::setup(const uint8_t* d_in, hipStream_t s) {
+int Engine::setup(const uint8_t* d_in, hipStream_t s, uint8_t* out) {
   BZG_TRY(hipSetDevice(device_));
   const int64_t B = (int64_t)desc_.size();
   const int64_t S = (int64_t)ss_.size();
@@ -280,15 +290,16 @@ int Engine::setup(const uint8_t* d_in, hipStream_t s) {
     d.lsum = c.take<int32_t>(ltile_cap * 2);
     d.ctr = c.take<int32_t>(CTR_N);
     d.scan_part = c.take<int32_t>(3 * kScanWG);
-    d.out = c.take<uint8_t>(out_cap_ + 64);
+    if (!out) d.out = c.take<uint8_t>(out_cap_ + 64);
     return d;
   };
   Carver probe;
   carve(probe);
-  BZG_TRY((hipError_t)grow(probe.off + 4096));
+  BZG_TRY((hipError_t)grow(probe.off + 4096, s));
   Carver real;
   real.base = static_cast<char*>(ws_);
   Dev d = carve(real);
+  if (out) d.out = out;
   d.in = d_in;
   d.nblk = (int32_t)B;
   d.nstream = (int32_t)S;
@@ -428,18 +439,28 @@ int Engine::run_assemble(hipStream_t s) {
   return (int)hipGetLastError();
 }
 
-// Runs a stage group as a captured HIP graph.  The graph is instantiated
-// once and replayed while the device-side plan (the Dev struct: workspace
-// pointers, input pointer, block/tile/stream counts, hence every grid size)
-// is unchanged; the stage groups have no host-sized launches, copies or
-// memsets, so a replay does exactly what the eager launches do.
+// Runs a stage group as a captured HIP graph.  A graph is instantiated once
+// per device-side plan (the Dev struct: workspace, input and output
+// pointers, block/tile/stream counts, hence every grid size) and replayed
+// while that plan recurs; the stage groups have no host-sized launches,
+// copies or memsets, so a replay does exactly what the eager launches do.
 int Engine::run_stage(int idx, hipStream_t s, int (Engine::*fn)(hipStream_t)) {
   // eager on the legacy stream, inside a caller's own capture, or when off
   if (!use_graphs_ || !stage_graphs() || s == nullptr || capturing(s)) return (this->*fn)(s);
   StageGraph& g = sgraph_[idx];
-  if (!g.exec || std::memcmp(&g.key, &d_, sizeof(Dev)) != 0) {
-    if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    g.exec = nullptr;
+  g.clock++;
+  int slot = -1, victim = 0;
+  for (int i = 0; i < kGraphCache; i++) {
+    if (g.exec[i] && std::memcmp(&g.key[i], &d_, sizeof(Dev)) == 0) {
+      slot = i;
+      break;
+    }
+    if (!g.exec[i] || (g.exec[victim] && g.used[i] < g.used[victim])) victim = i;
+  }
+  if (slot < 0) {
+    slot = victim;
+    if (g.exec[slot]) (void)hipGraphExecDestroy(g.exec[slot]);
+    g.exec[slot] = nullptr;
     hipGraph_t graph = nullptr;
     BZG_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const int rc = (this->*fn)(s);
@@ -448,17 +469,18 @@ int Engine::run_stage(int idx, hipStream_t s, int (Engine::*fn)(hipStream_t)) {
       if (graph) (void)hipGraphDestroy(graph);
       return rc ? rc : (int)e;
     }
-    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    const hipError_t ei = hipGraphInstantiate(&g.exec[slot], graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ei != hipSuccess) {
-      g.exec = nullptr;
+      g.exec[slot] = nullptr;
       return (int)ei;
     }
-    std::memcpy(&g.key, &d_, sizeof(Dev));
+    std::memcpy(&g.key[slot], &d_, sizeof(Dev));
     g.captures++;
   }
+  g.used[slot] = g.clock;
   g.replays++;
-  return (int)hipGraphLaunch(g.exec, s);
+  return (int)hipGraphLaunch(g.exec[slot], s);
 }
 
 int Engine::run(hipStream_t s, bool assemble, StageTimes* times) {

@@ -891,8 +891,12 @@ __global__ void __launch_bounds__(256) k_layout_blocks(Dev d) {
   const int si = blockIdx.x;
   const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
   StreamState S = d.ss[si];
-  int64_t run = 32;
-  uint32_t crc = 0;
+  const bool seg = S.lead >= 0;
+  int64_t run = seg ? S.lead : 32;
+  // blocks before a segment: c = rotl(c, 1) ^ crc_q continues from crc_in,
+  // which ends up rotated by this segment's block count
+  const int rin = S.n_blk & 31;
+  uint32_t crc = seg ? (rin ? (S.crc_in << rin) | (S.crc_in >> (32 - rin)) : S.crc_in) : 0u;
   for (int q0 = 0; q0 < S.n_blk; q0 += 256) {
     const int q = q0 + tid;
     int64_t nb = 0;
@@ -916,7 +920,7 @@ __global__ void __launch_bounds__(256) k_layout_blocks(Dev d) {
     __syncthreads();
   }
   if (tid == 0) {
-    S.bits = run + 48 + 32;
+    S.bits = seg ? run : run + 48 + 32;
     S.crc = crc;
     S.bytes = (S.bits + 7) / 8;
     d.ss[si] = S;
@@ -948,11 +952,15 @@ __global__ void __launch_bounds__(256) k_layout(Dev d) {
       S.byte_off = (run + pre + inc - words) * 4;
       d.ss[si] = S;
       uint32_t* w0 = out + S.byte_off / 4;
-      w0[0] = __builtin_bswap32(0x425A6830u + (uint32_t)S.level);
-      // the words shared at the end-of-stream marker
-      const int64_t ta = S.bits - 80;
-      w0[ta >> 5] = 0;
-      w0[(S.bits - 1) >> 5] = 0;
+      if (S.lead < 0) {
+        w0[0] = __builtin_bswap32(0x425A6830u + (uint32_t)S.level);
+        // the words shared at the end-of-stream marker
+        const int64_t ta = S.bits - 80;
+        w0[ta >> 5] = 0;
+        w0[(S.bits - 1) >> 5] = 0;
+      } else if (S.bits > 0) {
+        w0[0] = 0;  // the lead bits stay zero (an empty segment has no words)
+      }
     }
     run += tot;
   }
@@ -1017,6 +1025,7 @@ __global__ void __launch_bounds__(64) k_trailer(Dev d) {
   const int si = blockIdx.x * 64 + threadIdx.x;
   if (si >= d.nstream) return;
   const StreamState S = d.ss[si];
+  if (S.lead >= 0) return;  // segments end without the trailer
   uint32_t* w0 = reinterpret_cast<uint32_t*>(d.out) + S.byte_off / 4;
   uint32_t src[3] = {0x17724538u, (0x5090u << 16) | (S.crc >> 16), (S.crc & 0xFFFFu) << 16};
   place_bits(w0, src, S.bits - 80, 80, 0, 1);

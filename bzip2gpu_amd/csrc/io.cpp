@@ -17,6 +17,11 @@ BZFileInputStream::BZFileInputStream(int _n_buffers, int _buffer_size, std::stri
   setup_buffers(_n_buffers, _buffer_size);
 }
 
+BZFileInputStream::BZFileInputStream(int _n_buffers, int _buffer_size, std::istream *stream) {
+  in_stream = stream;
+  setup_buffers(_n_buffers, _buffer_size);
+}
+
 BZFileInputStream::~BZFileInputStream() {
   {
     std::lock_guard<std::mutex> lk(mtx);
@@ -41,8 +46,8 @@ void BZFileInputStream::setup_buffers(int _n_buffers, int _buffer_size) {
     buffers[i] = nullptr;
     buffer_lengths[i] = -1;
   }
-  if (!*in_stream || buffer_size <= 0) {
-    failed = !*in_stream;
+  if (!in_stream || !*in_stream || buffer_size <= 0) {
+    failed = !in_stream || !*in_stream;
     is_closed = true;
     return;
   }
@@ -65,8 +70,12 @@ void BZFileInputStream::reader_thread_task() {
     }
     in_stream->read(reinterpret_cast<char *>(chunk), buffer_size);
     int got = (int)in_stream->gcount();
+    // a short read ends the input: end of file (failbit + eofbit) or a read
+    // error (badbit), which is reported through `failed`
     bool eof = !*in_stream;
+    bool bad = in_stream->bad();
     lock.lock();
+    if (bad) failed = true;
     if (got > 0) {
       tail_idx = (tail_idx + 1) % n_buffers;
       buffers[tail_idx] = chunk;

@@ -135,6 +135,7 @@ void launch_sa_attach(const Dev& d, const int32_t* sa, hipStream_t s);  // calle
 void launch_sa_strip(const Dev& d, int32_t* sa, hipStream_t s);         // SA entries -> caller SA
 void launch_fnz(const Dev& d, hipStream_t s);        // tc_fnz from MTF[0..m)
 void launch_sym_hist(const Dev& d, hipStream_t s);   // freq from SYM[0..s)
+void launch_check_codes(const Dev& d, int n_groups, hipStream_t s);  // st.err if SYM / sel out of range
 void launch_huffman(const Dev& d, hipStream_t s);    // tables, selectors, codes, headers
 void launch_pack(const Dev& d, hipStream_t s);       // payload words per block
 void launch_assemble(const Dev& d, hipStream_t s);   // stream layout + placement into out

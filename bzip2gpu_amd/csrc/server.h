@@ -9,6 +9,7 @@
 #include <deque>
 #include <mutex>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "engine.h"
@@ -42,13 +43,20 @@ class Server {
   Server(const Server&) = delete;
   Server& operator=(const Server&) = delete;
 
-  // Queues a request (device buffer); arrival_ns < 0 stamps "now".
+  // Queues a request (device buffer); arrival_ns < 0 stamps "now".  Ticket
+  // ids increase monotonically over the server's life.
   int64_t submit(const uint8_t* d_in, uint64_t len, int level, int64_t arrival_ns = -1);
-  // Blocks until request id is done; copies its record.
+  // Blocks until request id is done, copies its record and forgets it: each
+  // ticket is waited for once (a second wait, or an unknown id, returns
+  // kUnknownTicket).  Otherwise returns the request's error (0 or a HIP code).
+  static constexpr int kUnknownTicket = -1;
   int wait(int64_t id, Req* out);
   // Keep every output in a caller-provided device arena (results stay in HBM).
+  // Waits until no request is queued or running first.
   void set_keep(bool keep, uint8_t* arena, uint64_t arena_bytes);
-  // Clears the request table and restarts the clock (between replays).
+  // Waits until no request is queued or running, then restarts the clock
+  // (between replays).  Records of finished requests nobody has waited for
+  // yet stay, with their times on the old clock.
   void reset_clock();
   int64_t now_ns() const;
   uint64_t max_batch() const { return max_batch_; }
@@ -56,8 +64,10 @@ class Server {
 
  private:
   void worker(int k);
-  int64_t take_arena(int64_t bytes);
-  uint8_t* arena_ptr() const;
+  // arena offset for `bytes` of output, -1 when it does not fit (mu_ held)
+  int64_t take_arena_locked(int64_t bytes);
+  // waits until nothing is queued or running (mu_ held through lk)
+  void quiesce(std::unique_lock<std::mutex>& lk);
 
   int device_;
   int mode_;
@@ -67,17 +77,18 @@ class Server {
   std::chrono::steady_clock::time_point t0_;
   std::mutex mu_;
   std::condition_variable cv_work_, cv_done_;
-  std::deque<Req> reqs_;
+  // every field below is guarded by mu_
+  std::unordered_map<int64_t, Req> reqs_;  // requests not yet waited for
+  int64_t next_id_ = 0;
   std::deque<int64_t> pending_;
-  uint64_t inflight_ = 0;
+  uint64_t inflight_ = 0;   // device bytes of the running batches
+  int running_ = 0;         // batches taken by workers and not yet done
   int64_t batches_ = 0;
   bool stop_ = false;
   bool keep_ = false;
-  std::mutex arena_mu_;
   uint8_t* ext_arena_ = nullptr;
   uint64_t ext_arena_bytes_ = 0;
   int64_t arena_pos_ = 0;
-  uint8_t* arena_ = nullptr;
   std::vector<std::thread> threads_;
 };
 

@@ -40,17 +40,14 @@ Server::~Server() {
   }
   cv_work_.notify_all();
   for (auto& t : threads_) t.join();
-  (void)hipSetDevice(device_);
-  if (arena_) (void)hipFree(arena_);
 }
 
 int64_t Server::now_ns() const { return ns_since(t0_); }
 
 int64_t Server::submit(const uint8_t* d_in, uint64_t len, int level, int64_t arrival_ns) {
   std::lock_guard<std::mutex> lk(mu_);
-  int64_t id = (int64_t)reqs_.size();
-  reqs_.emplace_back();
-  Req& r = reqs_.back();
+  const int64_t id = next_id_++;
+  Req& r = reqs_[id];
   r.d = d_in;
   r.len = len;
   r.level = std::min(9, std::max(1, level));
@@ -62,14 +59,24 @@ int64_t Server::submit(const uint8_t* d_in, uint64_t len, int level, int64_t arr
 
 int Server::wait(int64_t id, Req* out) {
   std::unique_lock<std::mutex> lk(mu_);
-  if (id < 0 || id >= (int64_t)reqs_.size()) return (int)hipErrorInvalidValue;
-  cv_done_.wait(lk, [&] { return reqs_[id].state == 2; });
-  if (out) *out = reqs_[id];
-  return reqs_[id].err;
+  auto it = reqs_.find(id);
+  if (it == reqs_.end()) return kUnknownTicket;
+  // element references of an unordered_map survive inserts and rehashes
+  Req& r = it->second;
+  cv_done_.wait(lk, [&] { return r.state == 2; });
+  if (out) *out = r;
+  const int err = r.err;
+  reqs_.erase(id);
+  return err;
+}
+
+void Server::quiesce(std::unique_lock<std::mutex>& lk) {
+  cv_done_.wait(lk, [&] { return pending_.empty() && running_ == 0; });
 }
 
 void Server::set_keep(bool keep, uint8_t* arena, uint64_t arena_bytes) {
-  std::lock_guard<std::mutex> lk(mu_);
+  std::unique_lock<std::mutex> lk(mu_);
+  quiesce(lk);
   keep_ = keep;
   ext_arena_ = arena;
   ext_arena_bytes_ = arena_bytes;
@@ -77,11 +84,9 @@ void Server::set_keep(bool keep, uint8_t* arena, uint64_t arena_bytes) {
 }
 
 void Server::reset_clock() {
-  std::lock_guard<std::mutex> lk(mu_);
+  std::unique_lock<std::mutex> lk(mu_);
+  quiesce(lk);
   t0_ = Clock::now();
-  reqs_.clear();
-  pending_.clear();
-  arena_pos_ = 0;
 }
 
 void Server::worker(int k) {
@@ -98,6 +103,8 @@ void Server::worker(int k) {
   std::vector<uint8_t> hout;
   for (;;) {
     uint64_t est = 0;
+    bool keep = false;        // output placement of this batch, fixed at admission
+    uint8_t* arena = nullptr;
     {
       std::unique_lock<std::mutex> lk(mu_);
       cv_work_.wait(lk, [&] { return stop_ || !pending_.empty(); });
@@ -106,7 +113,7 @@ void Server::worker(int k) {
       batch.clear();
       uint64_t tot = 0;
       while (!pending_.empty()) {
-        const Req& r = reqs_[pending_.front()];
+        const Req& r = reqs_.at(pending_.front());
         uint64_t need = (tot + r.len) * kBytesPerInputByte;
         bool fits_batch = batch.empty() || tot + r.len <= max_batch_;
         bool fits_budget = inflight_ + need <= budget_ || (batch.empty() && inflight_ == 0);
@@ -122,7 +129,10 @@ void Server::worker(int k) {
       }
       est = tot * kBytesPerInputByte;
       inflight_ += est;
-      for (int64_t id : batch) reqs_[id].state = 1;
+      running_++;
+      for (int64_t id : batch) reqs_.at(id).state = 1;
+      keep = keep_;
+      arena = ext_arena_;
     }
     int err = 0;
     std::vector<std::pair<int64_t, int64_t>> res(batch.size(), {0, -1});  // bytes, arena offset
@@ -130,15 +140,19 @@ void Server::worker(int k) {
       Req r;
       {
         std::lock_guard<std::mutex> lk(mu_);
-        r = reqs_[batch[0]];
+        r = reqs_.at(batch[0]);
       }
       err = compress_plain_device(eng, r.d, r.len, r.level, 100000 * r.level, hout, s);
       if (!err) {
         res[0].first = (int64_t)hout.size();
-        if (keep_) {
-          int64_t off = take_arena(res[0].first);
+        if (keep) {
+          int64_t off;
+          {
+            std::lock_guard<std::mutex> lk(mu_);
+            off = take_arena_locked(res[0].first);
+          }
           if (off >= 0) {
-            err = (int)hipMemcpyAsync(arena_ptr() + off, hout.data(), hout.size(), hipMemcpyHostToDevice, s);
+            err = (int)hipMemcpyAsync(arena + off, hout.data(), hout.size(), hipMemcpyHostToDevice, s);
             if (!err) err = (int)hipStreamSynchronize(s);
             res[0].second = off;
           }
@@ -150,10 +164,10 @@ void Server::worker(int k) {
       {
         std::lock_guard<std::mutex> lk(mu_);
         // one device address space: offsets relative to the lowest buffer
-        base = reqs_[batch[0]].d;
-        for (int64_t id : batch) base = std::min(base, reqs_[id].d);
+        base = reqs_.at(batch[0]).d;
+        for (int64_t id : batch) base = std::min(base, reqs_.at(id).d);
         for (int64_t id : batch) {
-          const Req& r = reqs_[id];
+          const Req& r = reqs_.at(id);
           ss.push_back(StreamSpec{(uint64_t)(r.d - base), r.len, r.level, 100000 * r.level});
         }
       }
@@ -164,24 +178,28 @@ void Server::worker(int k) {
       if (!err) {
         for (size_t q = 0; q < batch.size(); q++) {
           res[q].first = lay[q].bytes;
-          if (keep_) {
-            int64_t off = take_arena(lay[q].bytes);
+          if (keep) {
+            int64_t off;
+            {
+              std::lock_guard<std::mutex> lk(mu_);
+              off = take_arena_locked(lay[q].bytes);
+            }
             if (off >= 0) {
-              hipError_t he = hipMemcpyAsync(arena_ptr() + off, eng.out_buffer() + lay[q].byte_off,
+              hipError_t he = hipMemcpyAsync(arena + off, eng.out_buffer() + lay[q].byte_off,
                                              (size_t)lay[q].bytes, hipMemcpyDeviceToDevice, s);
               if (he != hipSuccess) err = (int)he;
               res[q].second = off;
             }
           }
         }
-        if (!err && keep_) err = (int)hipStreamSynchronize(s);
+        if (!err && keep) err = (int)hipStreamSynchronize(s);
       }
     }
     {
       std::lock_guard<std::mutex> lk(mu_);
       int64_t t = ns_since(t0_);
       for (size_t q = 0; q < batch.size(); q++) {
-        Req& r = reqs_[batch[q]];
+        Req& r = reqs_.at(batch[q]);
         r.done_ns = t;
         r.out_bytes = res[q].first;
         r.out_off = res[q].second;
@@ -190,6 +208,7 @@ void Server::worker(int k) {
         r.state = 2;
       }
       inflight_ -= est;
+      running_--;
       batches_++;
     }
     cv_done_.notify_all();
@@ -198,8 +217,7 @@ void Server::worker(int k) {
   (void)hipStreamDestroy(s);
 }
 
-int64_t Server::take_arena(int64_t bytes) {
-  std::lock_guard<std::mutex> lk(arena_mu_);
+int64_t Server::take_arena_locked(int64_t bytes) {
   int64_t b = (bytes + 3) & ~3ll;
   uint64_t cap = ext_arena_ ? ext_arena_bytes_ : 0;
   if ((uint64_t)(arena_pos_ + b) > cap) return -1;
@@ -207,7 +225,5 @@ int64_t Server::take_arena(int64_t bytes) {
   arena_pos_ += b;
   return off;
 }
-
-uint8_t* Server::arena_ptr() const { return ext_arena_; }
 
 }  // namespace bzg

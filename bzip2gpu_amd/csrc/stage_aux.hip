@@ -56,6 +56,24 @@ __global__ void __launch_bounds__(256) k_sym_hist(Dev d) {
     if (h[q]) atomicAdd(&d.freq[(int64_t)b * 260 + q], h[q]);
 }
 
+// Range check of a caller's packing input (per-stage encode entry point):
+// every symbol must index a row of kMaxAlpha code entries and every selector
+// one of the n_groups tables; a violation sets the block's err flag.  Tile t
+// checks symbols and selectors [start, start + kTile) of its block.
+__global__ void __launch_bounds__(256) k_check_codes(Dev d, int n_groups) {
+  const int t = blockIdx.x;
+  const int b = d.tc_blk[t];
+  const BlockDesc D = d.desc[b];
+  const int s = d.st[b].s, nsel = d.st[b].n_sel;
+  const int start = (t - D.tile_c0) * kTile;
+  int bad = 0;
+  for (int e = threadIdx.x; e < kTile && start + e < s; e += 256)
+    bad |= d.SYM[D.off + start + e] >= kMaxAlpha ? 1 : 0;
+  for (int e = threadIdx.x; e < kTile && start + e < nsel; e += 256)
+    bad |= d.sel[D.sel_off + start + e] >= n_groups ? 1 : 0;
+  if (__syncthreads_or(bad) && threadIdx.x == 0) d.st[b].err = 1;
+}
+
 // A caller's suffix array -> internal entries (start | preceding byte << 24).
 __global__ void __launch_bounds__(256) k_sa_attach(Dev d, const int32_t* sa) {
   const int t = blockIdx.x;
@@ -86,5 +104,6 @@ void launch_sa_strip(const Dev& d, int32_t* sa, hipStream_t s) { k_sa_strip<<<d.
 void launch_presence(const Dev& d, hipStream_t s) { k_presence<<<d.ntile_c, 256, 0, s>>>(d); }
 void launch_fnz(const Dev& d, hipStream_t s) { k_fnz<<<d.ntile_c, 256, 0, s>>>(d); }
 void launch_sym_hist(const Dev& d, hipStream_t s) { k_sym_hist<<<d.ntile_c, 256, 0, s>>>(d); }
+void launch_check_codes(const Dev& d, int n_groups, hipStream_t s) { k_check_codes<<<d.ntile_c, 256, 0, s>>>(d, n_groups); }
 
 }  // namespace bzg

@@ -87,6 +87,27 @@ bzg_status bzg_compress_batch_device(const uint8_t *d_in, int32_t n_inputs,
                                      int64_t *h_byte_off, int64_t *h_bytes,
                                      void *stream);
 
+/* One segment of a stream compressed piece by piece, so that ingestion
+ * overlaps compression (bzip2_gpu_compress's chunk loop over the reading
+ * ring, src/compression/compress.cu:225-278 with src/utils/io.cpp:34-55,
+ * and the BitWriter's block serialisation, compress.cu:284-365): the blocks
+ * of d_in[0, len) (device memory), cut into block_bytes blocks, written as
+ * bare block bits with no stream header and no end-of-stream trailer,
+ * starting lead_bits (0..7) bits into the first output byte; those lead
+ * bits are zero, for the caller to OR into the last, partial byte of what
+ * it already holds.  *h_crc is the combined stream CRC: on entry its value
+ * after the earlier segments (0 before the first), on return after this
+ * one.  *h_bits receives lead_bits + this segment's block bits, so the
+ * next segment's lead_bits is *h_bits mod 8.  The caller writes the stream
+ * header ("BZh" + level) before the first segment and the 48-bit
+ * end-of-stream magic, the final combined CRC and the zero padding after
+ * the last.  Output size is bounded by bzg_compress_bound(len, block_bytes). */
+bzg_status bzg_compress_segment(const uint8_t *d_in, size_t len,
+                                int32_t block_bytes, int32_t lead_bits,
+                                uint32_t *h_crc, uint8_t *h_out,
+                                size_t out_cap, size_t *out_len,
+                                int64_t *h_bits, void *stream);
+
 /* Largest raw byte count one device pipeline run takes in the batched
  * entry points (default 1 GiB); bigger input sets run as consecutive
  * sub-batches with bounded device memory (about 70 bytes per input byte). */
@@ -190,20 +211,28 @@ typedef struct bzg_server bzg_server;
 bzg_status bzg_server_create(int32_t mode, int32_t workers, uint64_t max_batch_bytes,
                              uint64_t hbm_budget_bytes, bzg_server **out);
 void bzg_server_destroy(bzg_server *srv);
-/* Queues one request; *h_ticket identifies it. */
+/* Queues one request; *h_ticket identifies it (tickets increase
+ * monotonically over the server's life and are never reused). */
 bzg_status bzg_server_submit(bzg_server *srv, const uint8_t *d_in, uint64_t len, int32_t level,
                              int64_t *h_ticket);
 /* Waits for a request; its stream is d_arena[*h_out_off .. +*h_out_bytes)
- * when outputs are kept (bzg_server_keep), else only the size is set. */
+ * when outputs are kept (bzg_server_keep), else only the size is set.  The
+ * request's record is released by the wait: each ticket is waited for once,
+ * and a ticket that is unknown or already waited for gives BZG_ERR_INVALID. */
 bzg_status bzg_server_wait(bzg_server *srv, int64_t ticket, int64_t *h_out_bytes, int64_t *h_out_off,
                            int64_t *h_latency_ns);
-/* Keep outputs in the caller's device arena d_arena (arena_bytes). */
+/* Keep outputs in the caller's device arena d_arena (arena_bytes), filled
+ * from its start in completion order; outputs that no longer fit are not
+ * kept (offset -1).  Waits until no request is queued or running first. */
 bzg_status bzg_server_keep(bzg_server *srv, uint8_t *d_arena, uint64_t arena_bytes);
 /* Open-loop replay: request i = d_pool[in_off[i] .. +lens[i]) at level
  * levels[i] is submitted at arrival_ns[i] (nanoseconds from the start,
  * non-decreasing); returns when all are done.  Fills completion times,
  * output sizes, arena offsets (-1 when not kept) and the size of the
- * pipeline run that served each request. */
+ * pipeline run that served each request.  The replay first waits until
+ * every request submitted earlier has finished (their tickets stay valid),
+ * then restarts the server clock.  Calls of bzg_server_submit from other
+ * threads during a replay join its mix. */
 bzg_status bzg_server_replay(bzg_server *srv, const uint8_t *d_pool, int32_t n, const uint64_t *in_off,
                              const uint64_t *lens, const int32_t *levels, const int64_t *arrival_ns,
                              int64_t *h_done_ns, int64_t *h_out_bytes, int64_t *h_out_off,
@@ -264,7 +293,9 @@ bzg_status bzg_huff_build(const uint16_t *d_sym, int32_t s,
                           void *stream);
 
 /* huffman_encode (include/compression.h:89-94, huffman_encode.cu:72-143):
- * MSB-first packing into d_words (ceil(17*s/32)+2 words); *h_bits = W. */
+ * MSB-first packing into d_words (ceil(17*s/32)+2 words); *h_bits = W.
+ * Code lengths above 17 (the encoder's maximum, huffman.cu:321), symbols of
+ * 258 or more and selectors of n_groups or more give BZG_ERR_INVALID. */
 bzg_status bzg_huff_encode(const uint16_t *d_sym, int32_t s,
                            const uint8_t *h_len, const int32_t *h_code,
                            const uint8_t *d_sel, int32_t n_sel,

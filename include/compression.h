@@ -45,6 +45,13 @@ int huffman_encode(uint16_t *dev_data_in, int data_in_len, int alphabet_size,
                    int32_t code[max_n_groups][max_alphabet_size], uint8_t *dev_selectors,
                    int32_t num_selectors, int n_groups, hipStream_t stream);
 
+// Extension with no counterpart in the reference: bzip2_gpu_compress with an
+// explicit segment size, the raw bytes compressed per pipeline run while
+// the next segment is read (rounded down to whole chunks, at least one
+// chunk; bzip2_gpu_compress uses 256 MiB).  The output does not depend on it.
+void bzip2_gpu_compress_segmented(BZFileInputStream &in_stream, int n, std::vector<uint8_t> &out,
+                                  size_t segment_bytes);
+
 // Extension with no counterpart in the reference, whose decompression is not
 // built (TODO.md:28-34): decodes the concatenated .bz2 streams in[0, len) on
 // the device into `out` (replaced).  Returns 0; 2 for malformed or corrupt

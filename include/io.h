@@ -5,8 +5,10 @@
 // returns -1 once the input is exhausted.  The caller releases each chunk
 // with hipHostFree.  Differences from the reference, all on its failure
 // paths: no zero-length chunk is ever handed out, an unreadable input ends
-// the stream instead of blocking fetch() forever, and the reader thread is
-// joined by the destructor.
+// the stream instead of blocking fetch() forever, a read error part-way
+// through the input ends it with `failed` set (the chunks read before the
+// error are still handed out), and the reader thread is joined by the
+// destructor.
 #ifndef BZG_IO_H
 #define BZG_IO_H
 
@@ -28,7 +30,7 @@ typedef struct BZFileInputStream {
   int len = 0;
   int chunks_read = 0;
   bool is_closed = false;
-  bool failed = false;  // input could not be opened/read
+  bool failed = false;  // input could not be opened, or a read failed
   uint8_t **buffers = nullptr;
   int *buffer_lengths = nullptr;
   std::mutex mtx;
@@ -38,6 +40,9 @@ typedef struct BZFileInputStream {
 
   BZFileInputStream(int _n_buffers, int _buffer_size);
   BZFileInputStream(int _n_buffers, int _buffer_size, std::string filename);
+  // Extension (no counterpart in the reference): reads from a caller-owned
+  // stream, which must outlive this object.
+  BZFileInputStream(int _n_buffers, int _buffer_size, std::istream *stream);
   ~BZFileInputStream();
 
   void setup_buffers(int _n_buffers, int _buffer_size);
