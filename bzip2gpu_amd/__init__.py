@@ -14,6 +14,7 @@ from ._lib import (  # noqa: F401
     bwt,
     compress,
     compress_batch_device,
+    compress_batch_device_noalloc,
     compress_bound,
     compress_plain,
     compress_plain_device,

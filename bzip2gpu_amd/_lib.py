@@ -90,6 +90,8 @@ def lib():
         "bzg_decompress_fetch": ([P, sz, SZP], i32),
         "bzg_decompress_device": ([P, sz, ctypes.POINTER(ctypes.c_void_p), SZP, P], i32),
         "bzg_last_decode_times": ([P, I32P, I32P], i32),
+        "bzg_compress_batch_device_noalloc": ([P, i32, P, P, P, i32, P, sz, P, P, P], i32),
+        "bzg_compress_segment": ([P, sz, i32, i32, U32P, P, sz, SZP, I64P, P], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -174,6 +176,25 @@ def compress_batch_device(d_in, offsets, lengths, levels, block_bytes: int = 0, 
         ctypes.c_void_p(stream) if stream else None)
     _check(st, "bzg_compress_batch_device")
     return dout.value, bo, by
+
+
+def compress_batch_device_noalloc(d_in, offsets, lengths, levels, d_out, block_bytes: int = 0, stream=None):
+    """Allocation-free batched path: the streams go to the caller's device
+    tensor d_out (uint8); the library allocates nothing.  Returns
+    (byte_offsets np.int64[n], sizes np.int64[n]) into d_out."""
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    ln = np.ascontiguousarray(lengths, dtype=np.uint64)
+    lv = np.ascontiguousarray(levels, dtype=np.int32)
+    n = off.size
+    bo = np.zeros(n, dtype=np.int64)
+    by = np.zeros(n, dtype=np.int64)
+    st = lib().bzg_compress_batch_device_noalloc(
+        ctypes.c_void_p(d_in.data_ptr()), n, off.ctypes.data_as(ctypes.c_void_p),
+        ln.ctypes.data_as(ctypes.c_void_p), lv.ctypes.data_as(ctypes.c_void_p), block_bytes,
+        ctypes.c_void_p(d_out.data_ptr()), d_out.numel(), bo.ctypes.data_as(ctypes.c_void_p),
+        by.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(stream) if stream else None)
+    _check(st, "bzg_compress_batch_device_noalloc")
+    return bo, by
 
 
 def decompress(data) -> bytes:

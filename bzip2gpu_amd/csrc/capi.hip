@@ -22,10 +22,21 @@ namespace {
 thread_local std::string g_err;
 thread_local int g_dev = 0;
 
+// Workspace of the allocation-free entry point
+// (bzg_compress_batch_device_noalloc): a fixed array of this library in
+// device memory, one copy per GPU the library runs on, held until the
+// process ends.  In host code `scratch` only names the array; its device
+// address comes from hipGetSymbolAddress, per device (Ctx::scratch_ptr).
+constexpr size_t kScratchBytes = size_t(BZG_NOALLOC_SCRATCH_MIB) << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
 struct Ctx {
-  explicit Ctx(int dev) : eng(dev), stage(dev), dec(dev), device(dev) {}
+  explicit Ctx(int dev) : eng(dev), stage(dev), noalloc(dev), dec(dev), device(dev) {}
   Engine eng;      // batched / plain paths
   Engine stage;    // per-stage entry points
+  Engine noalloc;  // the allocation-free batched path, in `scratch`
+  void* scratch_ptr = nullptr;  // this device's copy of `scratch`
+  int scratch_err = 0;          // set-up failure of the allocation-free path
   Decoder dec;     // .bz2 -> raw
   DecodeTimes dlast{};
   int device;
@@ -70,10 +81,24 @@ Ctx* ctx_for(int dev, bzg_status* st) {
       return nullptr;
     }
     g_ctx[dev].reset(new Ctx(dev));
-    if (hipStreamCreateWithFlags(&g_ctx[dev]->stream, hipStreamNonBlocking) != hipSuccess) {
+    Ctx* c = g_ctx[dev].get();
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
       g_ctx[dev].reset();
       *st = fail(BZG_ERR_HIP, "hipStreamCreate failed");
       return nullptr;
+    }
+    // Set-up of the allocation-free path, once per device with the device
+    // current: the device address of this GPU's `scratch`, the engine that
+    // runs in it (eager launches, round events made now).
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p) {
+      c->scratch_ptr = p;
+      c->noalloc.use_fixed_workspace(p, kScratchBytes);
+      c->noalloc.set_graphs(false);
+      c->scratch_err = c->noalloc.reserve_round_events(kMaxRounds);
+    } else {
+      c->scratch_err = e != hipSuccess ? (int)e : (int)hipErrorInvalidSymbol;
     }
   }
   *st = BZG_OK;
@@ -289,6 +314,56 @@ bzg_status bzg_compress_batch_device(const uint8_t* d_in, int32_t n_inputs, cons
   hipError_t he = hipStreamSynchronize(s);
   if (he != hipSuccess) return hip_fail((int)he, "batch");
   c->last = sum;
+  return BZG_OK;
+}
+
+bzg_status bzg_compress_batch_device_noalloc(const uint8_t* d_in, int32_t n_inputs, const uint64_t* in_off,
+                                             const uint64_t* lens, const int32_t* levels, int32_t block_bytes,
+                                             uint8_t* d_out, size_t out_cap, int64_t* h_byte_off, int64_t* h_bytes,
+                                             void* stream) {
+  if (n_inputs < 0 || (n_inputs > 0 && (!d_in || !in_off || !lens || !levels || !d_out)) || !h_byte_off || !h_bytes)
+    return fail(BZG_ERR_INVALID, "null pointer");
+  if (block_bytes > BZG_REF_BLOCK_BYTES) return fail(BZG_ERR_INVALID, "block_bytes out of range");
+  uint64_t total = 0, blocks = 0;
+  for (int i = 0; i < n_inputs; i++) {
+    const uint64_t bb = block_bytes > 0 ? (uint64_t)block_bytes : 100000ull * (uint64_t)clamp_level(levels[i]);
+    total += lens[i];
+    blocks += (lens[i] + bb - 1) / bb;
+  }
+  if (total > BZG_NOALLOC_MAX_BYTES || blocks > BZG_NOALLOC_MAX_BLOCKS)
+    return fail(BZG_ERR_INVALID, "allocation-free path takes at most " + std::to_string(BZG_NOALLOC_MAX_BYTES) +
+                                     " raw bytes in " + std::to_string(BZG_NOALLOC_MAX_BLOCKS) + " blocks (got " +
+                                     std::to_string(total) + " bytes in " + std::to_string(blocks) + " blocks)");
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (c->scratch_err) return hip_fail(c->scratch_err, "allocation-free workspace set-up");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(BZG_ERR_HIP, "hipSetDevice failed");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  std::vector<StreamSpec> ss;
+  for (int i = 0; i < n_inputs; i++) {
+    const int lv = clamp_level(levels[i]);
+    ss.push_back(StreamSpec{in_off[i], lens[i], lv, block_bytes > 0 ? block_bytes : 100000 * lv});
+  }
+  Engine& E = c->noalloc;
+  int e;
+  if ((e = E.plan(ss))) return hip_fail(e, "plan");
+  if ((size_t)E.planned_out_bytes() > out_cap)
+    return fail(BZG_ERR_CAPACITY, "output buffer too small: needs " + std::to_string(E.planned_out_bytes()) + " bytes");
+  if ((e = E.setup(d_in, s, d_out))) {
+    if (e == (int)hipErrorOutOfMemory)
+      return fail(BZG_ERR_INTERNAL, "batch needs " + std::to_string(E.needed_workspace()) +
+                                        " bytes of workspace, the fixed array holds " + std::to_string(kScratchBytes));
+    return hip_fail(e, "setup");
+  }
+  if ((e = E.run(s, true, nullptr))) return hip_fail(e, "run");
+  std::vector<StreamState> lay;
+  if ((e = E.stream_layout(lay, s))) return hip_fail(e, "layout");
+  for (int i = 0; i < n_inputs; i++) {
+    h_byte_off[i] = lay[(size_t)i].byte_off;
+    h_bytes[i] = lay[(size_t)i].bytes;
+  }
   return BZG_OK;
 }
 

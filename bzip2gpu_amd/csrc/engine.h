@@ -47,6 +47,9 @@ struct StageTimes {
 
 class Engine;
 
+// Most prefix-doubling rounds a suffix sort runs (prefix 4 * 2^40 > any block).
+constexpr int kMaxRounds = 42;
+
 // Plain path: the blocks of one stream, each through its own pipeline run
 // on `s`, results copied back and serialised on the host (the reference's
 // compress_block + BitWriter structure).  `d_in` is device memory.
@@ -78,6 +81,15 @@ class Engine {
   int setup(const uint8_t* d_in, hipStream_t s, uint8_t* out = nullptr);
   // Device bytes the planned streams may take (worst case).
   int64_t planned_out_bytes() const { return out_cap_ + 64; }
+  // Runs every pipeline in caller-provided device memory instead of a
+  // workspace of its own: setup then allocates nothing and fails with
+  // hipErrorOutOfMemory when a plan needs more than `bytes`
+  // (needed_workspace() tells how much the last setup wanted).
+  void use_fixed_workspace(void* base, size_t bytes);
+  size_t needed_workspace() const { return need_; }
+  // Creates the per-round timing events of up to `rounds` suffix-sort
+  // rounds now, so that runs with at most that many create none.
+  int reserve_round_events(int rounds);
   // Full pipeline on the planned batch.  With assemble = true the .bz2
   // streams are written to the device output buffer.
   int run(hipStream_t s, bool assemble, StageTimes* times = nullptr);
@@ -136,6 +148,8 @@ class Engine {
   Dev d_{};
   void* ws_ = nullptr;
   size_t ws_bytes_ = 0;
+  bool fixed_ws_ = false;
+  size_t need_ = 0;
   int32_t* h_ctr_ = nullptr;  // pinned
   std::vector<BlockDesc> desc_;
   std::vector<StreamState> ss_;

@@ -130,7 +130,7 @@ Engine::Engine(int device) : device_(device) {
 
 Engine::~Engine() {
   (void)hipSetDevice(device_);
-  if (ws_) {  // allocated stream-ordered (grow)
+  if (ws_ && !fixed_ws_) {  // allocated stream-ordered (grow)
     (void)hipFreeAsync(ws_, nullptr);
     (void)hipStreamSynchronize(nullptr);
   }
@@ -205,7 +205,9 @@ int Engine::plan(const std::vector<StreamSpec>& streams, bool allow_big) {
 // stream the caller synchronised before this call), so that one engine's
 // growth does not wait for other engines' work on the device.
 int Engine::grow(size_t bytes, hipStream_t s) {
+  need_ = bytes;
   if (bytes <= ws_bytes_) return 0;
+  if (fixed_ws_) return (int)hipErrorOutOfMemory;
   if (ws_) {
     BZG_TRY(hipFreeAsync(ws_, s));
     ws_ = nullptr;
@@ -214,6 +216,27 @@ int Engine::grow(size_t bytes, hipStream_t s) {
   size_t want = bytes + bytes / 8;
   BZG_TRY(hipMallocAsync(&ws_, want, s));
   ws_bytes_ = want;
+  return 0;
+}
+
+void Engine::use_fixed_workspace(void* base, size_t bytes) {
+  ws_ = base;
+  ws_bytes_ = bytes;
+  fixed_ws_ = true;
+}
+
+int Engine::reserve_round_events(int rounds) {
+  while ((int)rev_.size() < rounds) {
+    hipEvent_t a = nullptr, b = nullptr;
+    BZG_TRY(hipEventCreate(&a));
+    BZG_TRY(hipEventCreate(&b));
+    rev_.emplace_back(a, b);
+  }
+  while ((int)tev_.size() < rounds) {
+    std::array<hipEvent_t, 5> ev{};
+    for (auto& e : ev) BZG_TRY(hipEventCreate(&e));
+    tev_.push_back(ev);
+  }
   return 0;
 }
 
@@ -367,7 +390,7 @@ int Engine::run_bwt(hipStream_t s, int* rounds_out) {
     const int32_t nseg = h_ctr_[CTR_NSEG];
     if (h_ctr_[CTR_ERR]) return (int)hipErrorIllegalState;
     if (nseg == 0) break;
-    if (round > 40) return (int)hipErrorIllegalState;
+    if (round >= kMaxRounds - 1) return (int)hipErrorIllegalState;
     const int32_t small = h_ctr_[CTR_SMALL];
     const int32_t npacks = (small + kHalf - 1) / kHalf;
     const int32_t ntiles = h_ctr_[CTR_LTILES];

@@ -87,6 +87,36 @@ bzg_status bzg_compress_batch_device(const uint8_t *d_in, int32_t n_inputs,
                                      int64_t *h_byte_off, int64_t *h_bytes,
                                      void *stream);
 
+/* Allocation-free variant of bzg_compress_batch_device.  It has no
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same inputs and stream semantics,
+ * except that the streams go to the caller's d_out (out_cap bytes; stream i
+ * at d_out + h_byte_off[i], h_bytes[i] bytes long) and the pipeline's
+ * workspace is a fixed array of this library in device memory
+ * (BZG_NOALLOC_SCRATCH_MIB MiB per GPU, set up when the library first runs
+ * on the GPU and held until the process ends) instead of memory it
+ * allocates.  It takes at most BZG_NOALLOC_MAX_BYTES raw bytes in total, cut
+ * into at most BZG_NOALLOC_MAX_BLOCKS blocks; bigger batches give
+ * BZG_ERR_INVALID, and an out_cap below the worst-case size of the planned
+ * streams gives BZG_ERR_CAPACITY.  All calls on one GPU share its array,
+ * which keeps the last call's contents (every run initialises what it
+ * reads); the library serialises the calls and each returns after its work
+ * on `stream` has finished, so two calls never overlap on the array, but
+ * device work that did overlap a call (e.g. the same kernels replayed from a
+ * capture) would race on it. */
+#define BZG_NOALLOC_SCRATCH_MIB 96
+#define BZG_NOALLOC_MAX_BYTES (1u << 20)
+#define BZG_NOALLOC_MAX_BLOCKS 64
+bzg_status bzg_compress_batch_device_noalloc(const uint8_t *d_in, int32_t n_inputs,
+                                             const uint64_t *in_off,
+                                             const uint64_t *lens,
+                                             const int32_t *levels,
+                                             int32_t block_bytes,
+                                             uint8_t *d_out, size_t out_cap,
+                                             int64_t *h_byte_off,
+                                             int64_t *h_bytes, void *stream);
+
 /* One segment of a stream compressed piece by piece, so that ingestion
  * overlaps compression (bzip2_gpu_compress's chunk loop over the reading
  * ring, src/compression/compress.cu:225-278 with src/utils/io.cpp:34-55,
