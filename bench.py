@@ -1,15 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark: bzip2 compression input GB/s on one MI355X.
 
-Default workload (mix256): one 256 MiB stream at level 9, cut into
-900000-byte blocks (299 blocks), content = 1 MiB segments drawn 50/25/25
-from order-2 Markov text (Zipf 1.2), uniform bytes and runs with geometric
-lengths of mean 64, seeded.  A step compresses the whole input with the
-batched pipeline (every block in one device pipeline, stream assembled on
-the device), input already resident in HBM.  The plain per-block path (one
-block per pipeline run, host serialisation) is timed once on the same
-input beside it, and both outputs are checked byte-for-byte against each
-other and against the CPU oracle.
+Default workload (batch4096): 4096 independent inputs already resident in
+HBM, sizes floor(exp(U(ln 1 KiB, ln 8 MiB))) (about 3.84 GB in all), levels
+U{1..9}, each cut into 100000*level-byte blocks, content = slices of a
+seeded 256 MiB pool of 1 MiB segments drawn 50/25/25 from order-2 Markov
+text (Zipf 1.2), uniform bytes and runs with geometric lengths of mean 64.
+A step compresses every input with the batched pipeline (all blocks of a
+sub-batch of at most 1 GiB in one device pipeline, streams assembled on the
+device).  Beside it, on the same inputs: the plain per-block path on one
+stream, the plain path with the reference's own concurrency (16 host
+threads, one HIP stream each, one block per pipeline run), and the CPU
+oracle and stock libbz2 on the host cores; every GPU output is checked
+byte-for-byte against the others and against the oracle.
+
+Other workloads: mix256 (one 256 MiB stream, level 9, 900000-byte blocks),
+text1, periodic64, serve, decompress.
 
 Prints one JSON line (see the keys below).
 """
@@ -31,15 +37,19 @@ def _args():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="mix256",
+    p.add_argument("--workload", default="batch4096",
                    choices=["mix256", "text1", "periodic64", "batch4096", "serve", "decompress"])
     p.add_argument("--inputs", type=int, default=4096, help="batch4096: number of inputs")
     p.add_argument("--lambdas", default="2000,8000,32000", help="serve: Poisson arrival rates (req/s)")
-    p.add_argument("--serve-seconds", type=float, default=10.0, help="serve: arrival window per rate")
+    p.add_argument("--serve-seconds", type=float, default=60.0, help="serve: arrival window per rate")
     p.add_argument("--plain-seconds", type=float, default=1.0, help="serve: arrival window of the plain-path run")
     p.add_argument("--workers", type=int, default=4, help="serve: pipeline workers (HIP streams)")
     p.add_argument("--mib", type=int, default=0, help="override input size (MiB)")
     p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--plain-threads", type=int, default=16,
+                   help="batch4096: host threads of the reference-style plain path (0: skip)")
+    p.add_argument("--cpu-sample-mib", type=int, default=256,
+                   help="batch4096: raw MiB of inputs (taken in order) the CPU baselines compress per run")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-check", action="store_true")
     p.add_argument("--probe", default="k_lscatter", help="kernel timed live with HIP events for the roofline")
@@ -90,6 +100,99 @@ def _batch_inputs(n_inputs, pool_bytes, seed=20261205):
     return offs, sizes, levels
 
 
+def _cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def _copy_peak_gbps():
+    """Achievable HBM bandwidth on this box: a 1 GiB device-to-device copy
+    (read + write), best of 5."""
+    import torch
+    a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    best = None
+    for _ in range(5):
+        t = time.perf_counter()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        best = dt if best is None else min(best, dt)
+    del a, b
+    return 2 * (1 << 30) / best / 1e9
+
+
+def _alg_bytes(tot):
+    """Algorithmic HBM bytes of a batch: per block
+    2n + 9m + 20 m R + 12 s + W/8 (CRC and RLE1, suffix sort with R
+    prefix-doubling rounds, last column and MTF, RLE2, Huffman selection and
+    encoding, payload), plus W/4 for the stream assembly on the device."""
+    return (2 * tot["n"] + 9 * tot["m"] + 20 * tot["m_rounds"] + 12 * tot["s"] + tot["bits"] / 8
+            + tot["bits"] / 4)
+
+
+def _median_runs(fn, runs=5):
+    fn()  # warm-up
+    ts = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    ts.sort()
+    return ts[len(ts) // 2], ts
+
+
+def _cpu_baselines_batch(pool, offs, sizes, levels, sample_bytes):
+    """The oracle (the reference path restated in C) and stock libbz2 (Python's
+    bz2, libbz2 1.0.8), each over the first inputs of the batch up to
+    sample_bytes, input-parallel on min(nproc, 16) host threads (16 is the
+    reference's worker cap); 1 warm-up + median of 5 runs."""
+    import bz2
+    from concurrent.futures import ThreadPoolExecutor
+    import oracle
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    k, acc = 0, 0
+    while k < len(sizes) and acc + int(sizes[k]) <= sample_bytes:
+        acc += int(sizes[k])
+        k += 1
+    k = max(k, 1)
+    acc = int(sizes[:k].sum())
+    items = [(pool[int(offs[i]):int(offs[i] + sizes[i])], int(levels[i])) for i in range(k)]
+    outs = {}
+
+    def run_oracle():
+        with ThreadPoolExecutor(threads) as ex:
+            res = list(ex.map(lambda it: oracle.compress(it[0], it[1], block_bytes=100000 * it[1]), items))
+        outs["oracle"] = res
+
+    def run_bz2():
+        with ThreadPoolExecutor(threads) as ex:
+            res = list(ex.map(lambda it: bz2.compress(it[0].tobytes(), it[1]), items))
+        outs["bz2"] = res
+
+    mo, ro = _median_runs(run_oracle)
+    mb, rb = _median_runs(run_bz2)
+    sample = (f"the first {k} of the batch's inputs ({acc} raw bytes, levels and 100000*level blocks as in "
+              f"the batch), input-parallel on {threads} host threads; 1 warm-up + median of 5 runs")
+    base = {"value": round(acc / mo / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": "oracle: " + sample, "runs_s": [round(x, 3) for x in ro]}
+    base.update(_cpu_info())
+    base["libbz2"] = {"value": round(acc / mb / 1e9, 4), "unit": "GB/s", "cores": threads,
+                      "sample": "Python bz2 (libbz2 1.0.8, BZ2_bzBuffToBuffCompress semantics): " + sample,
+                      "runs_s": [round(x, 3) for x in rb]}
+    return base, outs["oracle"], outs["bz2"], k
+
+
 def run_batch(a):
     import numpy as np
     import torch
@@ -97,16 +200,20 @@ def run_batch(a):
     import bzip2gpu_amd.datagen as dg
     pool = dg.mix(256 * MiB, dg.SEED0 + 2)
     offs, sizes, levels = _batch_inputs(a.inputs, pool.size)
+    offs_u, sizes_u = offs.astype(np.uint64), sizes.astype(np.uint64)
     total = int(sizes.sum())
     d_pool = torch.from_numpy(pool).cuda()
     torch.cuda.synchronize()
 
     def batched():
-        return bz.compress_batch_device(d_pool, offs.astype(np.uint64), sizes.astype(np.uint64), levels, 0)
+        return bz.compress_batch_device(d_pool, offs_u, sizes_u, levels, 0)
 
     tune = _autotune(batched)
     for _ in range(max(0, a.warmup)):
         batched()
+    torch.cuda.synchronize()
+    bz.probe_kernel(a.probe)
+    bz.probe_read()
     lat = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
@@ -114,25 +221,55 @@ def run_batch(a):
         ptr, bo, by = batched()
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    probe_ms, probe_n = bz.probe_read()
+    bz.probe_kernel(None)
     stages, rounds, rows = bz.last_stage_times()
-    value = total / (wall / a.steps) / 1e9
+    totals = bz.last_batch_totals()
+    small_rows, large_rows, pass_rows = bz.last_sort_rows()
+    step_s = wall / a.steps
+    value = total / step_s / 1e9
     end = int(bo[-1] + by[-1])
     allout = _hip_copy_d2h(ptr, end)
     outs = [allout[int(bo[i]):int(bo[i] + by[i])] for i in range(a.inputs)]
     res = {
         "metric": "input GB/s at 1 MI355X, batched vs plain per-block (bit-exact); p50/p99 latency",
         "value": round(value, 4), "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": round(wall / a.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "ms_per_step": round(step_s * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u8", "data": "synthetic",
-        "config": {"workload": "batch", "inputs": a.inputs, "bytes": total,
-                   "sizes": "log-uniform 1 KiB..8 MiB", "levels": "U{1..9}", "block_bytes": "100000*level",
-                   "content": "slices of a seeded 256 MiB mix pool", "path": "batched"},
+        "config": {"workload": "batch", "inputs": a.inputs, "bytes": total, "blocks": totals["blocks"],
+                   "sizes": "floor(exp(U(ln 1 KiB, ln 8 MiB)))", "levels": "U{1..9}", "block_bytes": "100000*level",
+                   "content": "slices of a seeded 256 MiB pool of 1 MiB segments: 50% order-2 text Zipf1.2 / "
+                              "25% uniform / 25% runs mean 64",
+                   "path": "batched (sub-batches of <= 1 GiB raw)"},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "stage_ms": {k: round(v, 3) for k, v in stages.items()}, "bwt_rounds": rounds,
         "compressed_bytes": int(sum(by)), "autotune": tune,
+        "sort_rows": {"small_groups": small_rows, "large_groups": large_rows, "large_x_passes": pass_rows},
     }
+    per_unit, unit = ALG_BYTES.get(a.probe, (0, ""))
+    units = {"radix_pass_rows": pass_rows}.get(unit, 0)
+    copy_peak = _copy_peak_gbps()
+    if probe_n and units and per_unit:
+        per_launch = per_unit * units * a.steps / probe_n
+        achieved = per_launch / (probe_ms / probe_n / 1e3) / 1e9
+        traffic = None
+        if a.traffic and os.path.exists(a.traffic):
+            with open(a.traffic) as f:
+                traffic = json.load(f).get(f"batch:{a.probe}")
+        res["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
+                           "frac": round(achieved / 8000.0, 4), "traffic": traffic,
+                           "kernel": a.probe, "launches": probe_n,
+                           "avg_launch_us": round(probe_ms / probe_n * 1e3, 2),
+                           "alg_bytes_per_launch": int(per_launch), "alg_bytes_rule": f"{per_unit} B x {unit}"}
+    alg = _alg_bytes(totals)
+    pipe = alg / step_s / 1e9
+    res.setdefault("roofline", {})["pipeline"] = {
+        "alg_bytes_per_step": int(alg), "achieved": round(pipe, 2), "frac": round(pipe / 8000.0, 4),
+        "copy_peak_gbps": round(copy_peak, 1), "frac_of_copy_peak": round(pipe / copy_peak, 4),
+        "rule": "per block 2n + 9m + 20 m R + 12 s + W/8, + W/4 assembly", "totals": totals}
     if not a.no_plain:
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -143,18 +280,34 @@ def run_batch(a):
             same = same and (o == outs[i])
         dtp = time.perf_counter() - t
         res["plain"] = {"value": round(total / dtp / 1e9, 4), "unit": "GB/s", "seconds": round(dtp, 3),
+                        "path": "one block per pipeline run, one HIP stream, host serialisation",
                         "identical_to_batched": bool(same)}
         res["batched_over_plain"] = round(value / (total / dtp / 1e9), 2)
-    if not a.no_check:
+    if a.plain_threads > 0:
+        bz.compress_plain_threads(d_pool, offs_u[:64], sizes_u[:64], levels[:64], 0, a.plain_threads)  # warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        pt = bz.compress_plain_threads(d_pool, offs_u, sizes_u, levels, 0, a.plain_threads)
+        dtt = time.perf_counter() - t
+        res["plain_threads"] = {"value": round(total / dtt / 1e9, 4), "unit": "GB/s", "seconds": round(dtt, 3),
+                                "threads": a.plain_threads,
+                                "path": "the reference's concurrency: host threads with one HIP stream each, one "
+                                        "block per pipeline run, host serialisation",
+                                "identical_to_batched": pt == outs}
+        res["batched_over_plain_threads"] = round(value / (total / dtt / 1e9), 2)
+    if not a.no_cpu:
+        cpu, o_outs, b_outs, k = _cpu_baselines_batch(pool, offs, sizes, levels, a.cpu_sample_mib * MiB)
+        res["cpu_baseline"] = cpu
+        res["identical_to_oracle"] = all(o_outs[i] == outs[i] for i in range(k))
+        res["oracle_checked_inputs"] = k
+        res["libbz2_identical_inputs"] = sum(1 for i in range(k) if b_outs[i] == outs[i])
+    elif not a.no_check:
         import oracle
         idx = np.random.default_rng(5).choice(a.inputs, size=min(64, a.inputs), replace=False)
-        ok = True
-        for i in idx:
-            want = oracle.compress(pool[int(offs[i]):int(offs[i] + sizes[i])], int(levels[i]),
-                                   block_bytes=100000 * int(levels[i]), threads=8)
-            ok = ok and (want == outs[int(i)])
-        res["identical_to_oracle"] = bool(ok)
-        res["oracle_sample_inputs"] = int(idx.size)
+        res["identical_to_oracle"] = all(
+            oracle.compress(pool[int(offs[i]):int(offs[i] + sizes[i])], int(levels[i]),
+                            block_bytes=100000 * int(levels[i]), threads=8) == outs[int(i)] for i in idx)
+        res["oracle_checked_inputs"] = int(idx.size)
     print(json.dumps(res), flush=True)
 
 
@@ -337,18 +490,36 @@ def _hip_copy_d2h(ptr, nbytes):
     return out.tobytes()
 
 
-def _cpu_baseline(data, level, block_bytes):
-    """The oracle (a restatement of the reference path) on the host cores,
-    block-parallel over min(nproc, 16) threads (the reference's worker cap),
-    on the whole workload (a bounded 10-40 CPU-second sample)."""
+def _cpu_baseline(data, level, block_bytes, sample_bytes):
+    """The oracle (a restatement of the reference path) and stock libbz2
+    (Python's bz2) on the host cores, block-parallel over min(nproc, 16)
+    threads (the reference's worker cap), on the first sample_bytes of the
+    workload; 1 warm-up + median of 5 runs.  Returns the baseline record and
+    the oracle's stream of the whole workload (the checker)."""
+    import bz2
+    from concurrent.futures import ThreadPoolExecutor
     import oracle
     threads = max(1, min(os.cpu_count() or 1, 16))
-    t = time.perf_counter()
+    n = min(data.size, max(block_bytes, sample_bytes // block_bytes * block_bytes))
+    sample = data[:n]
+    mo, ro = _median_runs(lambda: oracle.compress(sample, level, block_bytes=block_bytes, threads=threads))
+    chunks = [sample[i:i + block_bytes].tobytes() for i in range(0, n, block_bytes)]
+
+    def run_bz2():
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda c: bz2.compress(c, level), chunks))
+
+    mb, rb = _median_runs(run_bz2)
+    rec = {"value": round(n / mo / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+           "sample": f"oracle: the first {n} bytes of the workload, block-parallel on {threads} host threads; "
+                     f"1 warm-up + median of 5 runs", "runs_s": [round(x, 3) for x in ro]}
+    rec.update(_cpu_info())
+    rec["libbz2"] = {"value": round(n / mb / 1e9, 4), "unit": "GB/s", "cores": threads,
+                     "sample": f"Python bz2 (libbz2 1.0.8) on the same {n} bytes as independent "
+                               f"{block_bytes}-byte chunks on {threads} threads; 1 warm-up + median of 5 runs",
+                     "runs_s": [round(x, 3) for x in rb]}
     out = oracle.compress(data, level, block_bytes=block_bytes, threads=threads)
-    dt = time.perf_counter() - t
-    return {"value": round(data.size / dt / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"the whole {data.size // MiB} MiB workload, oracle block-parallel on {threads} threads, "
-                      f"{dt:.2f} s wall"}, out
+    return rec, out
 
 
 def main():
@@ -455,8 +626,16 @@ def main():
         res["plain"] = {"value": round(n / dtp / 1e9, 4), "unit": "GB/s", "seconds": round(dtp, 3),
                         "identical_to_batched": out_plain == out_batched}
         res["batched_over_plain"] = round(value / (n / dtp / 1e9), 2)
+    totals = bz.last_batch_totals()
+    alg = _alg_bytes(totals)
+    pipe = alg / (wall / a.steps) / 1e9
+    copy_peak = _copy_peak_gbps()
+    res.setdefault("roofline", {})["pipeline"] = {
+        "alg_bytes_per_step": int(alg), "achieved": round(pipe, 2), "frac": round(pipe / 8000.0, 4),
+        "copy_peak_gbps": round(copy_peak, 1), "frac_of_copy_peak": round(pipe / copy_peak, 4),
+        "rule": "per block 2n + 9m + 20 m R + 12 s + W/8, + W/4 assembly", "totals": totals}
     if not a.no_cpu:
-        cpu, cpu_out = _cpu_baseline(data, level, bb)
+        cpu, cpu_out = _cpu_baseline(data, level, bb, a.cpu_sample_mib * MiB)
         res["cpu_baseline"] = cpu
         res["identical_to_oracle"] = cpu_out == out_batched
     elif not a.no_check:

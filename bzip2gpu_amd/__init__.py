@@ -15,6 +15,8 @@ from ._lib import (  # noqa: F401
     compress,
     compress_batch_device,
     compress_batch_device_noalloc,
+    compress_plain_threads,
+    last_batch_totals,
     compress_bound,
     compress_plain,
     compress_plain_device,

@@ -92,6 +92,8 @@ def lib():
         "bzg_last_decode_times": ([P, I32P, I32P], i32),
         "bzg_compress_batch_device_noalloc": ([P, i32, P, P, P, i32, P, sz, P, P, P], i32),
         "bzg_compress_segment": ([P, sz, i32, i32, U32P, P, sz, SZP, I64P, P], i32),
+        "bzg_compress_plain_threads": ([P, i32, P, P, P, i32, i32, P, sz, P, P], i32),
+        "bzg_last_batch_totals": ([P], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -176,6 +178,31 @@ def compress_batch_device(d_in, offsets, lengths, levels, block_bytes: int = 0, 
         ctypes.c_void_p(stream) if stream else None)
     _check(st, "bzg_compress_batch_device")
     return dout.value, bo, by
+
+
+def compress_plain_threads(d_in, offsets, lengths, levels, block_bytes: int = 0, threads: int = 16):
+    """The reference's concurrency on the plain path: `threads` host threads,
+    one pipeline and HIP stream each, one block per pipeline run; host
+    serialisation.  Returns the list of streams (bytes)."""
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    ln = np.ascontiguousarray(lengths, dtype=np.uint64)
+    lv = np.ascontiguousarray(levels, dtype=np.int32)
+    n = off.size
+    cap = sum(compress_bound(int(x), block_bytes if block_bytes > 0 else 100000 * int(v)) for x, v in zip(ln, lv))
+    out = np.empty(max(1, cap), dtype=np.uint8)
+    bo = np.zeros(n, dtype=np.int64)
+    by = np.zeros(n, dtype=np.int64)
+    c = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    _check(lib().bzg_compress_plain_threads(ctypes.c_void_p(d_in.data_ptr()), n, c(off), c(ln), c(lv), block_bytes,
+                                            threads, c(out), out.size, c(bo), c(by)), "bzg_compress_plain_threads")
+    return [out[int(bo[i]):int(bo[i] + by[i])].tobytes() for i in range(n)]
+
+
+def last_batch_totals():
+    """Totals of the last batched call: dict of blocks, n, m, s, bits, m_rounds."""
+    t = np.zeros(6, np.int64)
+    _check(lib().bzg_last_batch_totals(t.ctypes.data_as(ctypes.c_void_p)), "bzg_last_batch_totals")
+    return dict(zip(["blocks", "n", "m", "s", "bits", "m_rounds"], [int(x) for x in t]))
 
 
 def compress_batch_device_noalloc(d_in, offsets, lengths, levels, d_out, block_bytes: int = 0, stream=None):

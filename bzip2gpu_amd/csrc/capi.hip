@@ -1,6 +1,8 @@
 // capi.hip - the extern "C" boundary (include/bzg.h) over the batch engine.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -35,6 +37,9 @@ struct Ctx {
   Engine eng;      // batched / plain paths
   Engine stage;    // per-stage entry points
   Engine noalloc;  // the allocation-free batched path, in `scratch`
+  // engines and streams of the reference-style threaded plain path
+  std::vector<std::unique_ptr<Engine>> pool;
+  std::vector<hipStream_t> pool_streams;
   void* scratch_ptr = nullptr;  // this device's copy of `scratch`
   int scratch_err = 0;          // set-up failure of the allocation-free path
   Decoder dec;     // .bz2 -> raw
@@ -45,9 +50,14 @@ struct Ctx {
   size_t d_in_cap = 0;
   hipStream_t stream = nullptr;
   StageTimes last{};
+  int64_t last_totals[kTotals] = {};  // of the last batched call
   uint64_t max_batch_bytes = 1ull << 30;  // raw bytes per device sub-batch
   uint8_t* d_res = nullptr;                // multi-sub-batch results
   size_t d_res_cap = 0;
+  ~Ctx() {
+    (void)hipSetDevice(device);
+    for (auto ps : pool_streams) (void)hipStreamDestroy(ps);
+  }
 };
 
 std::mutex g_mu;
@@ -290,6 +300,7 @@ bzg_status bzg_compress_batch_device(const uint8_t* d_in, int32_t n_inputs, cons
     if ((e = ensure_result(c, (size_t)region.back()))) return hip_fail(e, "result buffer");
   }
   StageTimes sum{};
+  int64_t totals[kTotals] = {};
   for (size_t p = 0; p < parts.size(); p++) {
     const int i0 = parts[p].first, i1 = parts[p].second;
     specs(i0, i1, ss);
@@ -305,6 +316,7 @@ bzg_status bzg_compress_batch_device(const uint8_t* d_in, int32_t n_inputs, cons
     sum.large_rows += t.large_rows;
     sum.large_pass_rows += t.large_pass_rows;
     if ((e = c->eng.stream_layout(lay, s))) return hip_fail(e, "layout");
+    for (int k = 0; k < kTotals; k++) totals[k] += c->eng.totals()[k];
     for (int i = i0; i < i1; i++) {
       h_byte_off[i] = region[p] + lay[i - i0].byte_off;
       h_bytes[i] = lay[i - i0].bytes;
@@ -314,6 +326,78 @@ bzg_status bzg_compress_batch_device(const uint8_t* d_in, int32_t n_inputs, cons
   hipError_t he = hipStreamSynchronize(s);
   if (he != hipSuccess) return hip_fail((int)he, "batch");
   c->last = sum;
+  std::memcpy(c->last_totals, totals, sizeof totals);
+  return BZG_OK;
+}
+
+bzg_status bzg_last_batch_totals(int64_t* h_totals6) {
+  if (!h_totals6) return fail(BZG_ERR_INVALID, "null pointer");
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  std::lock_guard<std::mutex> lk(c->mu);
+  for (int k = 0; k < kTotals; k++) h_totals6[k] = c->last_totals[k];
+  return BZG_OK;
+}
+
+bzg_status bzg_compress_plain_threads(const uint8_t* d_in, int32_t n_inputs, const uint64_t* in_off,
+                                      const uint64_t* lens, const int32_t* levels, int32_t block_bytes,
+                                      int32_t threads, uint8_t* h_out, size_t out_cap, int64_t* h_byte_off,
+                                      int64_t* h_bytes) {
+  if (n_inputs < 0 || (n_inputs > 0 && (!d_in || !in_off || !lens || !levels)) || !h_out || !h_byte_off ||
+      !h_bytes)
+    return fail(BZG_ERR_INVALID, "null pointer");
+  if (block_bytes > BZG_REF_BLOCK_BYTES) return fail(BZG_ERR_INVALID, "block_bytes out of range");
+  if (threads < 1 || threads > 16) return fail(BZG_ERR_INVALID, "threads must be 1..16");
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(BZG_ERR_HIP, "hipSetDevice failed");
+  while ((int)c->pool.size() < threads) {
+    hipStream_t ps = nullptr;
+    if (hipStreamCreateWithFlags(&ps, hipStreamNonBlocking) != hipSuccess) return fail(BZG_ERR_HIP, "hipStreamCreate");
+    c->pool.emplace_back(new Engine(c->device));
+    c->pool.back()->set_graphs(false);
+    c->pool_streams.push_back(ps);
+  }
+  // the work list: every block of every input, in input and block order
+  struct Job { int input; uint64_t off; int32_t n, level; };
+  std::vector<Job> jobs;
+  std::vector<size_t> first(n_inputs + 1, 0);
+  for (int i = 0; i < n_inputs; i++) {
+    const int lv = clamp_level(levels[i]);
+    const uint64_t bb = block_bytes > 0 ? (uint64_t)block_bytes : 100000ull * (uint64_t)lv;
+    first[i] = jobs.size();
+    for (uint64_t o = 0; o < lens[i]; o += bb) jobs.push_back(Job{i, in_off[i] + o, (int32_t)std::min(bb, lens[i] - o), lv});
+  }
+  first[n_inputs] = jobs.size();
+  std::vector<BlockResult> res(jobs.size());
+  std::atomic<size_t> next{0};
+  std::atomic<int> err{0};
+  auto work = [&](int t) {
+    if (hipSetDevice(c->device) != hipSuccess) { err = (int)hipErrorInvalidDevice; return; }
+    for (size_t j; !err && (j = next++) < jobs.size();) {
+      const int e = compress_one_block(*c->pool[t], d_in + jobs[j].off, jobs[j].n, jobs[j].level, res[j],
+                                       c->pool_streams[t]);
+      if (e) err = e;
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 0; t < threads; t++) th.emplace_back(work, t);
+  for (auto& x : th) x.join();
+  if (err) return hip_fail(err, "plain threads");
+  size_t pos = 0;
+  std::vector<uint8_t> one;
+  for (int i = 0; i < n_inputs; i++) {
+    one.clear();
+    serialize_stream(levels[i], res.data() + first[i], first[i + 1] - first[i], one);
+    if (pos + one.size() > out_cap) return fail(BZG_ERR_CAPACITY, "output buffer too small");
+    memcpy(h_out + pos, one.data(), one.size());
+    h_byte_off[i] = (int64_t)pos;
+    h_bytes[i] = (int64_t)one.size();
+    pos += one.size();
+  }
   return BZG_OK;
 }
 

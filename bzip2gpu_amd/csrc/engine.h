@@ -55,6 +55,12 @@ constexpr int kMaxRounds = 42;
 // compress_block + BitWriter structure).  `d_in` is device memory.
 int compress_plain_device(Engine& eng, const uint8_t* d_in, size_t len, int level, int32_t block_bytes,
                           std::vector<uint8_t>& out, hipStream_t s);
+// One block alone through a pipeline run on `s`, its result copied back
+// (the reference's compress_block, src/compression/compress.cu:60-182).
+int compress_one_block(Engine& eng, const uint8_t* d_block, int32_t n, int level, BlockResult& r, hipStream_t s);
+// Host serialisation of one stream from its blocks' results, in order
+// (the reference's BitWriter pass, src/compression/compress.cu:284-375).
+void serialize_stream(int level, const BlockResult* blocks, size_t n_blocks, std::vector<uint8_t>& out);
 
 // Stage groups other than the suffix sort run as captured HIP graphs
 // (process-wide switch; BZG_NO_GRAPHS=1 starts with graphs off).
@@ -105,8 +111,10 @@ class Engine {
   // capacity for `len` elements (no raw input).
   int setup_single(int32_t len, hipStream_t s);
 
-  // Copies per-stream layout (byte_off, bytes) to the host.
+  // Copies per-stream layout (byte_off, bytes) and the batch totals of the
+  // last run (totals()) to the host.
   int stream_layout(std::vector<StreamState>& out, hipStream_t s);
+  const int64_t* totals() const { return totals_; }
   // Copies one block's results to the host.
   int block_result(int b, BlockResult& r, hipStream_t s);
 
@@ -150,6 +158,7 @@ class Engine {
   size_t ws_bytes_ = 0;
   bool fixed_ws_ = false;
   size_t need_ = 0;
+  int64_t totals_[kTotals] = {};
   int32_t* h_ctr_ = nullptr;  // pinned
   std::vector<BlockDesc> desc_;
   std::vector<StreamState> ss_;

@@ -312,6 +312,7 @@ int Engine::setup(const uint8_t* d_in, hipStream_t s, uint8_t* out) {
     d.hist = c.take<uint32_t>(ltile_cap * 256);
     d.lsum = c.take<int32_t>(ltile_cap * 2);
     d.ctr = c.take<int32_t>(CTR_N);
+    d.totals = c.take<int64_t>(kTotals);
     d.scan_part = c.take<int32_t>(3 * kScanWG);
     if (!out) d.out = c.take<uint8_t>(out_cap_ + 64);
     return d;
@@ -544,6 +545,7 @@ int Engine::run(hipStream_t s, bool assemble, StageTimes* times) {
     if (rc) return rc;
     trace(s, "assemble done");
   }
+  launch_block_totals(d_, s);
   if (times) {
     BZG_TRY(hipEventRecord(ev_[6], s));
     BZG_TRY(hipEventSynchronize(ev_[6]));
@@ -560,8 +562,10 @@ int Engine::run(hipStream_t s, bool assemble, StageTimes* times) {
 
 int Engine::stream_layout(std::vector<StreamState>& out, hipStream_t s) {
   out.resize(ss_.size());
+  for (auto& t : totals_) t = 0;
   if (ss_.empty()) return 0;
   BZG_TRY(hipMemcpyAsync(out.data(), d_.ss, ss_.size() * sizeof(StreamState), hipMemcpyDeviceToHost, s));
+  if (!desc_.empty()) BZG_TRY(hipMemcpyAsync(totals_, d_.totals, sizeof totals_, hipMemcpyDeviceToHost, s));
   BZG_TRY(hipStreamSynchronize(s));
   return 0;
 }
@@ -615,30 +619,13 @@ struct HostBits {
 };
 }  // namespace
 
-int compress_plain_device(Engine& eng, const uint8_t* d_in, size_t len, int level, int32_t block_bytes,
-                          std::vector<uint8_t>& out, hipStream_t s) {
+void serialize_stream(int level, const BlockResult* blocks, size_t n_blocks, std::vector<uint8_t>& out) {
   level = std::min(9, std::max(1, level));
-  out.clear();
-  out.reserve(len / 2 + 64);
   HostBits bw(out);
   bw.put(0x42, 8); bw.put(0x5A, 8); bw.put(0x68, 8); bw.put((uint32_t)(0x30 + level), 8);
   uint32_t combined = 0;
-  BlockResult r;
-  int e;
-  // every block is a new one-block plan: the stage groups run eagerly
-  struct GraphsOff {
-    Engine& g;
-    bool was;
-    explicit GraphsOff(Engine& x) : g(x), was(x.graphs()) { g.set_graphs(false); }
-    ~GraphsOff() { g.set_graphs(was); }
-  } graphs_off(eng);
-  for (size_t off = 0; off < len; off += (size_t)block_bytes) {
-    size_t n = len - off < (size_t)block_bytes ? len - off : (size_t)block_bytes;
-    std::vector<StreamSpec> ss{StreamSpec{0, n, level, block_bytes}};
-    if ((e = eng.plan(ss))) return e;
-    if ((e = eng.setup(d_in + off, s))) return e;
-    if ((e = eng.run(s, false, nullptr))) return e;
-    if ((e = eng.block_result(0, r, s))) return e;
+  for (size_t i = 0; i < n_blocks; i++) {
+    const BlockResult& r = blocks[i];
     combined = ((combined << 1) | (combined >> 31)) ^ r.crc;
     bw.put_words(r.hdr, r.hdr_bits);
     bw.put_words(r.words, r.bits);
@@ -646,6 +633,37 @@ int compress_plain_device(Engine& eng, const uint8_t* d_in, size_t len, int leve
   bw.put(0x17, 8); bw.put(0x72, 8); bw.put(0x45, 8); bw.put(0x38, 8); bw.put(0x50, 8); bw.put(0x90, 8);
   bw.put(combined, 32);
   bw.pad();
+}
+
+int compress_one_block(Engine& eng, const uint8_t* d_block, int32_t n, int level, BlockResult& r, hipStream_t s) {
+  int e;
+  std::vector<StreamSpec> ss{StreamSpec{0, (uint64_t)n, level, n}};
+  if ((e = eng.plan(ss))) return e;
+  if ((e = eng.setup(d_block, s))) return e;
+  if ((e = eng.run(s, false, nullptr))) return e;
+  return eng.block_result(0, r, s);
+}
+
+int compress_plain_device(Engine& eng, const uint8_t* d_in, size_t len, int level, int32_t block_bytes,
+                          std::vector<uint8_t>& out, hipStream_t s) {
+  level = std::min(9, std::max(1, level));
+  out.clear();
+  out.reserve(len / 2 + 64);
+  // every block is a new one-block plan: the stage groups run eagerly
+  struct GraphsOff {
+    Engine& g;
+    bool was;
+    explicit GraphsOff(Engine& x) : g(x), was(x.graphs()) { g.set_graphs(false); }
+    ~GraphsOff() { g.set_graphs(was); }
+  } graphs_off(eng);
+  std::vector<BlockResult> res((len + (size_t)block_bytes - 1) / (size_t)block_bytes);
+  for (size_t k = 0; k < res.size(); k++) {
+    const size_t off = k * (size_t)block_bytes;
+    const size_t n = len - off < (size_t)block_bytes ? len - off : (size_t)block_bytes;
+    const int e = compress_one_block(eng, d_in + off, (int32_t)n, level, res[k], s);
+    if (e) return e;
+  }
+  serialize_stream(level, res.data(), res.size(), out);
   return 0;
 }
 

@@ -90,6 +90,7 @@ struct Dev {
   int64_t ltile_cap;
   int64_t pack_cap;
   int32_t* ctr;        // counters: [0],[1] list sizes; [2..] plan totals
+  int64_t* totals;     // [kTotals] batch totals of the last run (k_block_totals)
   int32_t* scan_part;  // [3][kScanWG] scan partials
   // output
   uint8_t* out;
@@ -113,6 +114,10 @@ enum : int {
   CTR_LELEMS = 7,  // total elements in large groups
   CTR_N = 16
 };
+
+// batch totals: blocks, raw bytes n, RLE1 bytes m, RLE2 symbols s, payload
+// bits W, and the sum over blocks of m x suffix-sort rounds
+constexpr int kTotals = 6;
 
 // ---- launchers (each enqueues kernels on `s`, no host synchronisation)
 void launch_block_init(const Dev& d, hipStream_t s);
@@ -139,5 +144,6 @@ void launch_check_codes(const Dev& d, int n_groups, hipStream_t s);  // st.err i
 void launch_huffman(const Dev& d, hipStream_t s);    // tables, selectors, codes, headers
 void launch_pack(const Dev& d, hipStream_t s);       // payload words per block
 void launch_assemble(const Dev& d, hipStream_t s);   // stream layout + placement into out
+void launch_block_totals(const Dev& d, hipStream_t s);  // totals[0..kTotals)
 
 }  // namespace bzg

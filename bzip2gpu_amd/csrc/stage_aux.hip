@@ -74,6 +74,30 @@ __global__ void __launch_bounds__(256) k_check_codes(Dev d, int n_groups) {
   if (__syncthreads_or(bad) && threadIdx.x == 0) d.st[b].err = 1;
 }
 
+// Totals of a batch after its pipeline run (for the algorithmic-byte count
+// of a benchmark): one workgroup sums the blocks' sizes.
+__global__ void __launch_bounds__(256) k_block_totals(Dev d) {
+  __shared__ int64_t part[4][kTotals];
+  int64_t v[kTotals] = {0, 0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < d.nblk; b += 256) {
+    const BlockState& st = d.st[b];
+    v[0] += 1;
+    v[1] += d.desc[b].n;
+    v[2] += st.m;
+    v[3] += st.s;
+    v[4] += st.bits;
+    v[5] += (int64_t)st.m * st.rounds;
+  }
+#pragma unroll
+  for (int k = 0; k < kTotals; k++) v[k] = wave_sum(v[k]);
+  if (lane_id() == 0)
+#pragma unroll
+    for (int k = 0; k < kTotals; k++) part[threadIdx.x >> 6][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < kTotals)
+    d.totals[threadIdx.x] = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+}
+
 // A caller's suffix array -> internal entries (start | preceding byte << 24).
 __global__ void __launch_bounds__(256) k_sa_attach(Dev d, const int32_t* sa) {
   const int t = blockIdx.x;
@@ -104,6 +128,7 @@ void launch_sa_strip(const Dev& d, int32_t* sa, hipStream_t s) { k_sa_strip<<<d.
 void launch_presence(const Dev& d, hipStream_t s) { k_presence<<<d.ntile_c, 256, 0, s>>>(d); }
 void launch_fnz(const Dev& d, hipStream_t s) { k_fnz<<<d.ntile_c, 256, 0, s>>>(d); }
 void launch_sym_hist(const Dev& d, hipStream_t s) { k_sym_hist<<<d.ntile_c, 256, 0, s>>>(d); }
+void launch_block_totals(const Dev& d, hipStream_t s) { k_block_totals<<<1, 256, 0, s>>>(d); }
 void launch_check_codes(const Dev& d, int n_groups, hipStream_t s) { k_check_codes<<<d.ntile_c, 256, 0, s>>>(d, n_groups); }
 
 }  // namespace bzg

@@ -73,6 +73,23 @@ bzg_status bzg_compress_plain_device(const uint8_t *d_in, size_t len, int level,
                                      size_t out_cap, size_t *out_len,
                                      void *stream);
 
+/* The reference's own concurrency for the plain path, over many inputs
+ * already resident on the device (the worker pool of bzip2_gpu_compress,
+ * src/compression/compress.cu:225-278, at most 16 threads): `threads` host
+ * threads, each with its own pipeline and HIP stream, take the next block
+ * of any input and compress it alone (one block per pipeline run, result
+ * copied back); the host then serialises each input's stream in block
+ * order.  Inputs as for bzg_compress_batch_device; stream i is written to
+ * h_out + h_byte_off[i], h_bytes[i] bytes long (streams back to back).
+ * Same bytes as the batched and single-stream plain paths. */
+bzg_status bzg_compress_plain_threads(const uint8_t *d_in, int32_t n_inputs,
+                                      const uint64_t *in_off,
+                                      const uint64_t *lens,
+                                      const int32_t *levels,
+                                      int32_t block_bytes, int32_t threads,
+                                      uint8_t *h_out, size_t out_cap,
+                                      int64_t *h_byte_off, int64_t *h_bytes);
+
 /* Batched path over many independent inputs already resident on the
  * device: input i is d_in[in_off[i] .. in_off[i]+lens[i]) and becomes its
  * own stream with level levels[i].  block_bytes <= 0 selects 100000*level
@@ -182,6 +199,12 @@ int32_t bzg_tune_state(int32_t knob, int32_t size_class, int32_t *h_values,
  * aid, no counterpart in the reference). */
 bzg_status bzg_last_sort_rows(int64_t *h_small_rows, int64_t *h_large_rows,
                               int64_t *h_large_pass_rows);
+
+/* Totals over the blocks of the last bzg_compress_batch_device call
+ * (benchmarking aid, no counterpart in the reference): [0] blocks, [1] raw
+ * bytes n, [2] RLE1 bytes m, [3] RLE2 symbols s incl. end-of-block, [4]
+ * payload bits W, [5] the sum over blocks of m x suffix-sort rounds. */
+bzg_status bzg_last_batch_totals(int64_t *h_totals6);
 
 /* Rows sorted and GPU milliseconds of every prefix-doubling round of the
  * last pipeline run (its last sub-batch); returns the number of rounds and

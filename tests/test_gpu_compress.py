@@ -180,3 +180,24 @@ def test_stage_graph_replays_match_eager(dev):
         assert r1 - r0 >= 3 * 5
     finally:
         bz.set_stage_graphs(True)
+
+
+def test_plain_threads_matches_batched(dev):
+    """The reference-style plain path (host threads, one HIP stream and one
+    pipeline each, one block per run) gives the batched path's bytes."""
+    import torch
+    import bzip2gpu_amd as bz
+    import bzip2gpu_amd.datagen as dg
+    sizes = [0, 1, 5000, 250000, 1234567, 2 * 900000 + 3, 100000]
+    levels = np.array([9, 1, 4, 2, 9, 9, 1], np.int32)
+    offs = np.cumsum([0] + [s + (-s) % 64 for s in sizes[:-1]]).astype(np.uint64)
+    pool = np.zeros(int(offs[-1]) + sizes[-1] + 64, np.uint8)
+    for i, sz in enumerate(sizes):
+        pool[int(offs[i]):int(offs[i]) + sz] = dg.mix(sz, 60 + i) if sz else 0
+    t = torch.from_numpy(pool).to(dev)
+    _case(f"plain threads: {len(sizes)} inputs, 4 threads")
+    got = bz.compress_plain_threads(t, offs, sizes, levels, 0, 4)
+    for i, sz in enumerate(sizes):
+        lv = int(levels[i])
+        _same(got[i], oracle.compress(pool[int(offs[i]):int(offs[i]) + sz], lv, block_bytes=100000 * lv),
+              f"input {i}")
