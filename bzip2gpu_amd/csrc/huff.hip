@@ -223,8 +223,8 @@ __device__ inline int2 heap_up(int2* heap, int zz, int2 tmp, int lane, int2* las
 // LDS of one table's code-length build.
 struct HeapLds {
   __align__(16) int2 heap[kHeapCap];
-  int16_t anc[2][520];  // pointer-jumping ancestors (node ids <= 2*258-1)
-  int16_t dep[2][520];
+  int16_t anc[520];  // pointer-jumping ancestors (node ids <= 2*258-1)
+  int16_t dep[520];
   int32_t leafw[260];
 };
 // code_too_long's two queues reuse heap/anc/dep: kQueueCap entries each
@@ -343,8 +343,8 @@ __device__ void build_lengths(HeapLds& H, const int32_t* fr, uint8_t* len, int a
       heap_down(H.heap, nHeap, t2, lane, wl, 0, &unused);
       nNodes++;
       if (lane == 0) {
-        H.anc[0][n1.y] = (int16_t)nNodes;
-        H.anc[0][n2.y] = (int16_t)nNodes;
+        H.anc[n1.y] = (int16_t)nNodes;
+        H.anc[n2.y] = (int16_t)nNodes;
       }
       const int w1 = n1.x, w2 = n2.x;
       const int d1 = w1 & 0xff, d2 = w2 & 0xff;
@@ -352,26 +352,31 @@ __device__ void build_lengths(HeapLds& H, const int32_t* fr, uint8_t* len, int a
       nHeap++;
       root = heap_up<true>(H.heap, nHeap, nn, lane, &last);
     }
-    // depth of every node by pointer jumping (root: ancestor itself, depth 0)
+    // depth of every node by pointer jumping (root: ancestor itself, depth
+    // 0), in place: a node's ancestor has a larger id, so it lies in the
+    // same 64-node chunk (whose loads all precede its stores) or in a later
+    // one, and every step reads the previous step's values, as a second
+    // buffer would give
     for (int v = lane + 1; v <= nNodes; v += 64) {
       const bool root = v == nNodes;
-      if (root) H.anc[0][v] = (int16_t)v;
-      H.dep[0][v] = root ? 0 : 1;
+      if (root) H.anc[v] = (int16_t)v;
+      H.dep[v] = root ? 0 : 1;
     }
     wave_lds_sync();
-    int src = 0;
     for (int it = 0; it < 9; it++) {
       for (int v = lane + 1; v <= nNodes; v += 64) {
-        const int a = H.anc[src][v];
-        H.dep[src ^ 1][v] = (int16_t)(H.dep[src][v] + H.dep[src][a]);
-        H.anc[src ^ 1][v] = H.anc[src][a];
+        const int a = H.anc[v];
+        const int da = H.dep[a], aa = H.anc[a];
+        const int dv = H.dep[v];
+        __builtin_amdgcn_wave_barrier();
+        H.dep[v] = (int16_t)(dv + da);
+        H.anc[v] = (int16_t)aa;
       }
       wave_lds_sync();
-      src ^= 1;
     }
     bool tooLong = false;
     for (int i = lane + 1; i <= alpha; i += 64) {
-      const int j = H.dep[src][i];
+      const int j = H.dep[i];
       len[i - 1] = (uint8_t)j;
       if (j > 17) tooLong = true;
     }
@@ -414,12 +419,18 @@ __global__ void __launch_bounds__(256) k_huff_order(Dev d) {
   for (int b = tid; b < d.nblk; b += 256) d.horder[atomicAdd(&cnt[259 - min(d.st[b].n_in_use + 2, 259)], 1)] = b;
 }
 
-constexpr int kHuffThreads = 512;  // 8 waves: all select, waves 0..5 build
-__global__ void __launch_bounds__(kHuffThreads) k_huff_build(Dev d) {
-  __shared__ uint64_t PL[260];
+// 6 waves: all assign groups, wave t builds table t.  39 KB of LDS and at
+// most 80 VGPRs let four blocks share a CU, so that the heap chains of
+// different blocks overlap.
+constexpr int kHuffThreads = 384;
+__global__ void __launch_bounds__(kHuffThreads) __attribute__((amdgpu_waves_per_eu(6))) k_huff_build(Dev d) {
   __shared__ int32_t rf[6][260];
   __shared__ uint8_t L[6][260];
   __shared__ HeapLds H[6];
+  // the six lengths of a symbol packed in one word, used while the groups
+  // are assigned; it shares table 0's heap, which is idle then
+  static_assert(sizeof(H[0].heap) >= 260 * sizeof(uint64_t), "packed lengths fit table 0's heap");
+  uint64_t* PL = reinterpret_cast<uint64_t*>(H[0].heap);
   const int b = d.horder[blockIdx.x];
   const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
   const BlockDesc D = d.desc[b];
