@@ -388,17 +388,20 @@ def run_serve(a):
     print(json.dumps(res), flush=True)
 
 
-def _autotune(run_once, max_runs=8):
+def _autotune(run_once, max_runs=16):
     """Untimed set-up: lets the library's autotuner time each launch shape of
-    the radix scatter on this workload until every size class it met has
-    settled (each shape is timed twice per class)."""
+    the radix scatter (knob 0), then each launch policy of the suffix-sort
+    rounds (knob 1), on this workload until every size class it met has
+    settled (each variant is timed twice per class)."""
     import torch
     import bzip2gpu_amd as bz
 
+    def seen(knob):
+        out = [(c, bz.tune_state(knob, c)) for c in range(4)]
+        return [(c, st) for c, st in out if any(n for _, _, n in st[1]) or st[0] >= 0]
+
     def unsettled():
-        seen = [(c, bz.tune_state(0, c)) for c in range(4)]
-        seen = [(c, st) for c, st in seen if any(n for _, _, n in st[1]) or st[0] >= 0]
-        return not seen or any(st[0] < 0 for _, st in seen)
+        return any(not seen(k) or any(st[0] < 0 for _, st in seen(k)) for k in (0, 1))
 
     runs = 0
     while runs < max_runs and unsettled():
@@ -406,11 +409,14 @@ def _autotune(run_once, max_runs=8):
         runs += 1
     torch.cuda.synchronize()
     out = {"setup_runs": runs}
-    for c in range(4):
-        chosen, rows = bz.tune_state(0, c)
-        if chosen >= 0 or any(n for _, _, n in rows):
-            out[f"class{c}"] = {"radix_scatter_threads": rows[chosen][0] if chosen >= 0 else None,
-                                "ns_per_row_pass": {str(v): round(ns, 4) for v, ns, _ in rows}}
+    names = {0: ("radix_scatter_threads", "ns_per_row_pass"), 1: ("sort_launch", "ns_per_sorted_row")}
+    for k in (0, 1):
+        for c, (chosen, rows) in seen(k):
+            key, unit = names[k]
+            val = rows[chosen][0] if chosen >= 0 else None
+            if k == 1 and val is not None:
+                val = ("per_round", "device_driven")[val]
+            out[f"knob{k}_class{c}"] = {key: val, unit: {str(v): round(ns, 4) for v, ns, _ in rows}}
     return out
 
 

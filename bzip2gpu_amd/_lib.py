@@ -312,14 +312,14 @@ def tune_force(knob: int, variant: int) -> None:
 
 def tune_state(knob: int = 0, size_class: int = 0):
     """(chosen variant or -1, [(launch value, mean ns per unit, samples)])."""
-    vals = np.zeros(4, dtype=np.int32)
+    vals = np.full(4, -1, dtype=np.int32)
     ns = np.zeros(4, dtype=np.float32)
     n = np.zeros(4, dtype=np.int32)
     c = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     chosen = lib().bzg_tune_state(int(knob), int(size_class), c(vals), c(ns), c(n), 4)
     if chosen == -2:
         raise BzgError("bzg_tune_state: bad argument")
-    return int(chosen), [(int(vals[i]), float(ns[i]), int(n[i])) for i in range(4) if vals[i]]
+    return int(chosen), [(int(vals[i]), float(ns[i]), int(n[i])) for i in range(4) if vals[i] >= 0]
 
 
 def probe_kernel(name) -> None:

@@ -13,10 +13,13 @@
 // sorted by start index instead (the reference's stable sort from the
 // identity gives the same order, tests/compression/fbwt.cu:22).
 //
-// Work per round:
+// Work per round (every kernel has a fixed, persistent grid and takes its
+// counts from device memory, so rounds are enqueued, or replayed from a
+// captured graph, without any host readback; a round after the last one
+// finds an empty group list and does nothing):
 //   plan      fixed-grid scans over the group list: prefix sums of small
 //             group lengths (packed 2048-element LDS sorts) and of large
-//             group tile counts.
+//             group tile counts; the round counter advances.
 //   keys      gather key(i) for every row of every listed group.
 //   small     one workgroup per pack: LDS radix sort of (group, key)
 //             composite keys, new heads by a workgroup max-scan, RK/SA
@@ -25,6 +28,8 @@
 //             groups (histogram -> per-group scan -> stable scatter), then
 //             boundary summaries and heads, as above.
 #include "pipeline.h"
+
+#include <algorithm>
 
 namespace bzg {
 
@@ -51,6 +56,33 @@ __device__ inline uint32_t wave_rank_digit(uint32_t digit, bool valid, uint32_t*
 
 __device__ inline int bits_for(uint32_t v) {  // bits to hold 0..v
   return v == 0 ? 0 : 32 - __clz(v);
+}
+
+// Wave-uniform reads.  Inside a persistent kernel's item loop the compiler
+// cannot prove that the item tables (group list, tile maps, block
+// descriptors) are unchanged by the earlier items' stores, so it loads them
+// with vector loads and treats the values as per-lane: every address derived
+// from them would become per-lane arithmetic.  readfirstlane puts the
+// (uniform) values back in scalar registers.
+__device__ inline int u32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline uint64_t u64(uint64_t v) {
+  return ((uint64_t)(uint32_t)u32((int)(v >> 32)) << 32) | (uint32_t)u32((int)(uint32_t)v);
+}
+__device__ inline int ctr_get(const Dev& d, int slot) { return u32(d.ctr[slot]); }
+__device__ inline Seg seg_get(const Seg* p) {
+  const Seg g = *p;
+  return Seg{u32(g.blk), u32(g.start), u32(g.len), 0};
+}
+
+// The thread index as a value the compiler cannot hoist out of a persistent
+// kernel's item loop: without it, every per-lane address offset of an item
+// (sixteen per load sequence) is computed once before the loop and kept
+// live across all items, which costs registers and occupancy.  The empty
+// asm only marks the register as changed.
+__device__ inline int loop_tid() {
+  int t = (int)threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
 }
 
 // ------------------------------------------------------------------ init
@@ -106,6 +138,10 @@ __global__ void __launch_bounds__(256) k_bwt_init(Dev d) {
       }
     }
   }
+  if (t == 0 && threadIdx.x == 0) {
+    d.ctr[CTR_ROUND] = -1;
+    d.ctr[CTR_RAN] = 0;
+  }
   if (k == 0 && threadIdx.x == 0) {
     if (m > 1) {
       int idx = atomicAdd(&d.ctr[CTR_LIST0], 1);
@@ -124,9 +160,11 @@ __device__ inline PlanVals plan_vals(const Seg& g) {
   return PlanVals{small ? g.len : 0, small ? 0 : tiles_for(g.len), small ? 0 : 1, small ? 0 : g.len};
 }
 
-__global__ void __launch_bounds__(256) k_plan_scan1(Dev d, int cur) {
+// the group list a round reads: round r sorts list r & 1 and fills the other
+__global__ void __launch_bounds__(256) k_plan_scan1(Dev d) {
   __shared__ int32_t tmp[4][8];
-  const int n = d.ctr[cur];
+  const int cur = (ctr_get(d, CTR_ROUND) + 1) & 1;  // the round being planned
+  const int n = ctr_get(d, cur);
   const int per = (n + kScanWG - 1) / kScanWG;
   const int i0 = blockIdx.x * per, i1 = min(n, i0 + per);
   int sa = 0, sb = 0, sc = 0, se = 0;
@@ -148,16 +186,40 @@ __global__ void __launch_bounds__(256) k_plan_scan1(Dev d, int cur) {
   }
 }
 
-__global__ void __launch_bounds__(kScanWG) k_plan_scan2(Dev d, int cur) {
+// Totals of the round's plan; advances the round counter and records the
+// rows the round sorts.  A list still holding groups after the last round
+// the sort allows is an error (the round is then emptied).
+__global__ void __launch_bounds__(kScanWG) k_plan_scan2(Dev d) {
   __shared__ int32_t tmp[kScanWG / 64 + 1];
+  const int r = d.ctr[CTR_ROUND] + 1;
+  const int cur = r & 1;
+  int tots[4];
   for (int q = 0; q < 4; q++) {
     int v = d.scan_part[q * kScanWG + threadIdx.x];
     int tot;
     int ex = wg_excl_sum<kScanWG, int>(v, tmp, &tot);
     d.scan_part[q * kScanWG + threadIdx.x] = ex;
-    if (threadIdx.x == 0) d.ctr[q < 3 ? CTR_SMALL + q : CTR_LELEMS] = tot;
+    tots[q] = tot;
   }
-  if (threadIdx.x == 0) d.ctr[CTR_NSEG] = d.ctr[cur];
+  if (threadIdx.x == 0) {
+    int nseg = d.ctr[cur];
+    if (nseg > 0 && r >= kMaxSortRounds - 1) {
+      d.ctr[CTR_ERR] = 1;
+      nseg = 0;
+      tots[0] = tots[1] = tots[2] = tots[3] = 0;
+    }
+    d.ctr[CTR_SMALL] = tots[0];
+    d.ctr[CTR_LTILES] = tots[1];
+    d.ctr[CTR_NLARGE] = tots[2];
+    d.ctr[CTR_LELEMS] = tots[3];
+    d.ctr[CTR_NSEG] = nseg;
+    d.ctr[CTR_ROUND] = r;
+    if (nseg > 0) d.ctr[CTR_RAN] = r + 1;
+    if (r < kMaxSortRounds) {  // rows in small groups, then in large groups
+      d.round_rows[r] = nseg > 0 ? tots[0] : 0;
+      d.round_rows[kMaxSortRounds + r] = nseg > 0 ? tots[3] : 0;
+    }
+  }
 }
 
 // Per group: exclusive prefixes of the small-group rows, large-group tiles
@@ -165,11 +227,12 @@ __global__ void __launch_bounds__(kScanWG) k_plan_scan2(Dev d, int cur) {
 // tid in thread tid's slot q (coalesced, all loads in flight); the 64 (slot,
 // wave) runs are in group order, so wave scans, a scan over the runs' totals
 // by one wave, and the carried totals give every prefix.
-__global__ void __launch_bounds__(256) k_plan_scan3(Dev d, int cur) {
+__global__ void __launch_bounds__(256) k_plan_scan3(Dev d) {
   constexpr int kPer = 16;
   __shared__ int32_t tw[3][64];
   __shared__ int32_t tt[3];
-  const int n = d.ctr[cur];
+  const int cur = ctr_get(d, CTR_ROUND) & 1;
+  const int n = ctr_get(d, CTR_NSEG);
   const int per = (n + kScanWG - 1) / kScanWG;
   const int i0 = blockIdx.x * per, i1 = min(n, i0 + per);
   const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
@@ -216,37 +279,44 @@ __global__ void __launch_bounds__(256) k_plan_scan3(Dev d, int cur) {
   }
 }
 
-void launch_bwt_plan(const Dev& d, int cur, int /*nseg*/, hipStream_t s) {
-  k_plan_scan1<<<kScanWG, 256, 0, s>>>(d, cur);
-  k_plan_scan2<<<1, kScanWG, 0, s>>>(d, cur);
-  k_plan_scan3<<<kScanWG, 256, 0, s>>>(d, cur);
+void launch_bwt_plan(const Dev& d, hipStream_t s) {
+  k_plan_scan1<<<kScanWG, 256, 0, s>>>(d);
+  k_plan_scan2<<<1, kScanWG, 0, s>>>(d);
+  k_plan_scan3<<<kScanWG, 256, 0, s>>>(d);
 }
+
+__device__ inline int round_packs(const Dev& d) { return (ctr_get(d, CTR_SMALL) + kHalf - 1) / kHalf; }
 
 // pack k = small groups whose pa lies in [k*kHalf, (k+1)*kHalf)
-__global__ void __launch_bounds__(256) k_mk_packs(Dev d, int npacks, int nseg) {
-  int k = blockIdx.x * 256 + threadIdx.x;
-  if (k > npacks) return;
-  if (k == npacks) { d.packs[k] = nseg; return; }
-  int target = k * kHalf;
-  int lo = 0, hi = nseg;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (d.pa[mid] < target) lo = mid + 1; else hi = mid;
+__global__ void __launch_bounds__(256) k_mk_packs(Dev d) {
+  const int npacks = round_packs(d), nseg = ctr_get(d, CTR_NSEG);
+  for (int k = blockIdx.x * 256 + threadIdx.x; k <= npacks; k += gridDim.x * 256) {
+    if (k == npacks) {
+      d.packs[k] = nseg;
+      continue;
+    }
+    int target = k * kHalf;
+    int lo = 0, hi = nseg;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (d.pa[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    d.packs[k] = lo;
   }
-  d.packs[k] = lo;
 }
 
-__global__ void __launch_bounds__(256) k_mk_large(Dev d, int cur, int nseg) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i == 0) d.ctr[cur ^ 1] = 0;  // next list starts empty
-  if (i >= nseg) return;
-  Seg g = d.seg[cur][i];
-  if (g.len <= kHalf) return;
-  d.llist[d.pc[i]] = i;
-  int nt = tiles_for(g.len), t0 = d.pb[i];
-  for (int t = 0; t < nt; t++) {
-    d.ltile_seg[t0 + t] = i;
-    d.ltile_pos[t0 + t] = t * kTile;
+__global__ void __launch_bounds__(256) k_mk_large(Dev d) {
+  const int cur = ctr_get(d, CTR_ROUND) & 1, nseg = ctr_get(d, CTR_NSEG);
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctr[cur ^ 1] = 0;  // next list starts empty
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < nseg; i += gridDim.x * 256) {
+    Seg g = d.seg[cur][i];
+    if (g.len <= kHalf) continue;
+    d.llist[d.pc[i]] = i;
+    int nt = tiles_for(g.len), t0 = d.pb[i];
+    for (int t = 0; t < nt; t++) {
+      d.ltile_seg[t0 + t] = i;
+      d.ltile_pos[t0 + t] = t * kTile;
+    }
   }
 }
 
@@ -264,10 +334,11 @@ struct PackLds {
 // Loads the small groups of pack k into LDS (a pack holds <= 1023 groups:
 // each has >= 2 rows and the pack <= kPack-1 rows).
 __device__ inline void load_pack(const Dev& d, int cur, int k, PackLds& P) {
-  const int i0 = d.packs[k], i1 = d.packs[k + 1];
+  const int tid_ = loop_tid();
+  const int i0 = u32(d.packs[k]), i1 = u32(d.packs[k + 1]);
   int count = 0;
   for (int base = i0; base < i1; base += 256) {
-    int i = base + threadIdx.x;
+    int i = base + tid_;
     Seg g{0, 0, 0, 0};
     bool small = false;
     if (i < i1) { g = d.seg[cur][i]; small = g.len <= kHalf; }
@@ -287,11 +358,11 @@ __device__ inline void load_pack(const Dev& d, int cur, int k, PackLds& P) {
   if (count > 0) {
     const int base_pa = P.lofs[0];
     __syncthreads();
-    for (int j = threadIdx.x; j < count; j += 256) P.lofs[j] -= base_pa;
+    for (int j = tid_; j < count; j += 256) P.lofs[j] -= base_pa;
     __syncthreads();
-    if (threadIdx.x == 0) P.total = P.lofs[count - 1] + P.last_len;
+    if (tid_ == 0) P.total = P.lofs[count - 1] + P.last_len;
   }
-  if (threadIdx.x == 0) P.count = count;
+  if (tid_ == 0) P.count = count;
   __syncthreads();
 }
 
@@ -304,46 +375,18 @@ __device__ inline int find_seg(const PackLds& P, int e) {  // last j with lofs[j
   return lo;
 }
 
-// key of the row at block-local SA index j
-__device__ inline uint32_t make_key(const Dev& d, uint64_t off, int m, int j, int round, int P) {
-  uint32_t v = d.SA[off + j] & kSAMask;
-  if (round == 0) {
-    uint32_t k = 0;
-    uint32_t p = v;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      k = (k << 8) | d.T[off + p];
-      p = p + 1 == (uint32_t)m ? 0 : p + 1;
-    }
-    return k;
-  }
-  if (P >= m) return v;  // identical rotations: order by start index
-  uint32_t p = v + (uint32_t)P;
-  if (p >= (uint32_t)m) p -= (uint32_t)m;
-  return d.RK[off + p];
-}
-
 // ------------------------------------------------------------------- keys
-__global__ void __launch_bounds__(256) k_keys(Dev d, int cur, int round, int P, int npacks) {
-  __shared__ PackLds L;
-  const int w = (int)xcd_remap(blockIdx.x, gridDim.x);
+// Rounds r >= 1: key(i) = RK[(SA[i] + P) mod m] for every row of every
+// listed group (P >= m: identical rotations, ordered by start index).
+// Item w < npacks is a small-group pack, the others large-group tiles.
+__device__ inline void keys_item(const Dev& d, int cur, int round, int P, int npacks, int w, PackLds& L) {
+  const int tid_ = loop_tid();
   if (w < npacks) {
     load_pack(d, cur, w, L);
     if (L.count == 0) return;
     const int total = L.total;
-    for (int j = threadIdx.x; j < L.count; j += 256)
+    for (int j = tid_; j < L.count; j += 256)
       if (j == 0 || L.blk[j] != L.blk[j - 1]) atomicMax(&d.st[L.blk[j]].rounds, round + 1);
-    if (round == 0) {
-      for (int e = threadIdx.x; e < total; e += 256) {
-        int s = find_seg(L, e);
-        int b = L.blk[s];
-        const BlockDesc& D = d.desc[b];
-        int m = d.st[b].m;
-        int j = L.start[s] + (e - L.lofs[s]);
-        d.KEY[D.off + j] = make_key(d, D.off, m, j, round, P);
-      }
-      return;
-    }
     // rows' segment and block first (block data reloaded only when the block
     // changes), then all SA loads, then all rank loads, then the stores
     int jj[kPack / 256], mm[kPack / 256];
@@ -352,7 +395,7 @@ __global__ void __launch_bounds__(256) k_keys(Dev d, int cur, int round, int P, 
     uint64_t coff = 0;
 #pragma unroll
     for (int q = 0; q < kPack / 256; q++) {
-      const int e = threadIdx.x + 256 * q;
+      const int e = tid_ + 256 * q;
       jj[q] = -1;
       mm[q] = 1;
       oo[q] = 0;
@@ -385,32 +428,25 @@ __global__ void __launch_bounds__(256) k_keys(Dev d, int cur, int round, int P, 
       if (jj[q] >= 0) d.KEY[oo[q] + jj[q]] = v[q];
   } else {
     const int t = w - npacks;
-    const int si = d.ltile_seg[t];
-    const Seg g = d.seg[cur][si];
-    const int pos = d.ltile_pos[t];
-    const uint64_t off = d.desc[g.blk].off;
-    const int m = d.st[g.blk].m;
-    if (pos == 0 && threadIdx.x == 0) atomicMax(&d.st[g.blk].rounds, round + 1);
+    const int si = u32(d.ltile_seg[t]);
+    const Seg g = seg_get(&d.seg[cur][si]);
+    const int pos = u32(d.ltile_pos[t]);
+    const uint64_t off = u64(d.desc[g.blk].off);
+    const int m = u32(d.st[g.blk].m);
+    if (pos == 0 && tid_ == 0) atomicMax(&d.st[g.blk].rounds, round + 1);
     const int n = min(kTile, g.len - pos);
-    if (round == 0) {
-      for (int e = threadIdx.x; e < n; e += 256) {
-        int j = g.start + pos + e;
-        d.KEY[off + j] = make_key(d, off, m, j, round, P);
-      }
-      return;
-    }
     // all SA loads, then all rank loads, then the stores
     const int j0 = g.start + pos;
     uint32_t v[kTile / 256];
 #pragma unroll
     for (int q = 0; q < kTile / 256; q++) {
-      const int e = threadIdx.x + 256 * q;
+      const int e = tid_ + 256 * q;
       v[q] = e < n ? d.SA[off + j0 + e] & kSAMask : 0u;
     }
     if (P < m) {
 #pragma unroll
       for (int q = 0; q < kTile / 256; q++) {
-        const int e = threadIdx.x + 256 * q;
+        const int e = tid_ + 256 * q;
         if (e < n) {
           uint32_t p = v[q] + (uint32_t)P;
           if (p >= (uint32_t)m) p -= (uint32_t)m;
@@ -420,9 +456,19 @@ __global__ void __launch_bounds__(256) k_keys(Dev d, int cur, int round, int P, 
     }
 #pragma unroll
     for (int q = 0; q < kTile / 256; q++) {
-      const int e = threadIdx.x + 256 * q;
+      const int e = tid_ + 256 * q;
       if (e < n) d.KEY[off + j0 + e] = v[q];
     }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_keys(Dev d) {
+  __shared__ PackLds L;
+  const int round = ctr_get(d, CTR_ROUND), cur = round & 1, P = round_prefix(round);
+  const int npacks = round_packs(d);
+  for (ItemLoop it(npacks + ctr_get(d, CTR_LTILES)); it.i < it.end; it.i += it.step) {
+    __syncthreads();  // the previous item's pack is no longer read
+    keys_item(d, cur, round, P, npacks, it.i, L);
   }
 }
 
@@ -468,6 +514,7 @@ constexpr int kCountMax = 64;
 // composite key (group in pack << keybits | key): 32 bits whenever they fit.
 template <typename KT>
 __device__ inline void sort_small_body(const Dev& d, int cur, int keybits, int k) {
+  const int tid_ = loop_tid();
   __shared__ PackLds L;
   __shared__ KT K0[kPack], K1[kPack];
   __shared__ uint32_t V0[kPack], V1[kPack];
@@ -475,7 +522,7 @@ __device__ inline void sort_small_body(const Dev& d, int cur, int keybits, int k
   __shared__ int32_t xs[256];
   __shared__ int32_t s_base;
   __shared__ int32_t tmp[8];
-  const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
+  const int tid = tid_, wv = tid >> 6, lane = (tid_ & 63);
   const int nxt = cur ^ 1;
   load_pack(d, cur, k, L);
   if (L.count == 0) return;
@@ -715,13 +762,18 @@ __device__ inline void sort_small_body(const Dev& d, int cur, int keybits, int k
   }
 }
 
-__global__ void __launch_bounds__(256) k_sort_small(Dev d, int cur, int keybits) {
-  sort_small_body<uint32_t>(d, cur, keybits, (int)xcd_remap(blockIdx.x, gridDim.x));
+template <typename KT>
+__device__ inline void sort_small_loop(const Dev& d, int keybits) {
+  const int cur = ctr_get(d, CTR_ROUND) & 1;
+  for (ItemLoop it(round_packs(d)); it.i < it.end; it.i += it.step) {
+    __syncthreads();  // the previous pack's LDS is no longer read
+    sort_small_body<KT>(d, cur, keybits, it.i);
+  }
 }
 
-__global__ void __launch_bounds__(256) k_sort_small64(Dev d, int cur, int keybits) {
-  sort_small_body<uint64_t>(d, cur, keybits, (int)xcd_remap(blockIdx.x, gridDim.x));
-}
+__global__ void __launch_bounds__(256) k_sort_small(Dev d, int keybits) { sort_small_loop<uint32_t>(d, keybits); }
+
+__global__ void __launch_bounds__(256) k_sort_small64(Dev d, int keybits) { sort_small_loop<uint64_t>(d, keybits); }
 
 // ----------------------------------------------------------- large groups
 // Digit histogram of a large-group tile (per-wave counters, so that skewed
@@ -729,13 +781,13 @@ __global__ void __launch_bounds__(256) k_sort_small64(Dev d, int cur, int keybit
 // With GEN (round 0's first pass, whose groups are whole blocks) the digit
 // is read from the text: the key's low byte is T[(i+3) mod m].
 template <bool GEN>
-__global__ void __launch_bounds__(256) k_lhist(Dev d, int cur, int shift, int src) {
+__device__ inline void lhist_item(const Dev& d, int cur, int shift, int src, int t) {
+  const int tid_ = loop_tid();
   __shared__ uint32_t h[4][256];
-  const int tid = threadIdx.x, wv = tid >> 6;
-  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
-  const Seg g = d.seg[cur][d.ltile_seg[t]];
-  const int pos = d.ltile_pos[t];
-  const uint64_t off = d.desc[g.blk].off;
+  const int tid = tid_, wv = tid >> 6;
+  const Seg g = seg_get(&d.seg[cur][u32(d.ltile_seg[t])]);
+  const int pos = u32(d.ltile_pos[t]);
+  const uint64_t off = u64(d.desc[g.blk].off);
   const uint32_t* K = (src ? d.KEY2 : d.KEY) + off + g.start + pos;
   h[0][tid] = 0; h[1][tid] = 0; h[2][tid] = 0; h[3][tid] = 0;
   __syncthreads();
@@ -791,17 +843,27 @@ __global__ void __launch_bounds__(256) k_lhist(Dev d, int cur, int shift, int sr
   d.hist[(int64_t)t * 256 + tid] = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
 }
 
+template <bool GEN>
+__global__ void __launch_bounds__(256) k_lhist(Dev d, int shift, int src) {
+  const int cur = ctr_get(d, CTR_ROUND) & 1;
+  for (ItemLoop it(ctr_get(d, CTR_LTILES)); it.i < it.end; it.i += it.step) {
+    __syncthreads();
+    lhist_item<GEN>(d, cur, shift, src, it.i);
+  }
+}
+
 // per large group: hist[t][digit] -> absolute destination base
 // Per large group and digit: exclusive prefix of the digit's count over the
 // group's tiles.  The first tile's prefix is always 0, so its entry holds
 // the digit's global start instead (group start + rows of smaller digits);
 // k_lscatter adds it back for the other tiles.
-__global__ void __launch_bounds__(256) k_lscan(Dev d, int cur) {
+__device__ inline void lscan_item(const Dev& d, int cur, int li) {
+  const int tid_ = loop_tid();
   __shared__ int32_t tmp[8];
-  const int si = d.llist[blockIdx.x];
-  const Seg g = d.seg[cur][si];
-  const int t0 = d.pb[si], nt = tiles_for(g.len);
-  const int dg = threadIdx.x;
+  const int si = u32(d.llist[li]);
+  const Seg g = seg_get(&d.seg[cur][si]);
+  const int t0 = u32(d.pb[si]), nt = tiles_for(g.len);
+  const int dg = tid_;
   uint32_t* hp = d.hist + (int64_t)t0 * 256 + dg;
   uint32_t run = 0;
   for (int t = 0; t < nt; t += 16) {
@@ -819,13 +881,19 @@ __global__ void __launch_bounds__(256) k_lscan(Dev d, int cur) {
   hp[0] = (uint32_t)(g.start + base);
 }
 
+__global__ void __launch_bounds__(256) k_lscan(Dev d) {
+  const int cur = ctr_get(d, CTR_ROUND) & 1;
+  for (ItemLoop it(ctr_get(d, CTR_NLARGE)); it.i < it.end; it.i += it.step) lscan_item(d, cur, it.i);
+}
+
 // One LSD pass over a large-group tile: stable rank of every row by digit,
 // rows placed in LDS in tile-sorted order, then written out so that rows of
 // one digit go to consecutive global addresses (coalesced runs).  NT threads
 // (NT/64 waves), 4096/NT rows each; the shape is an autotuned choice and
 // every shape gives the same result.
 template <int NT, bool GEN>
-__global__ void __launch_bounds__(NT) k_lscatter(Dev d, int cur, int shift, int src) {
+__device__ inline void lscatter_item(const Dev& d, int cur, int shift, int src, int t) {
+  const int tid_ = loop_tid();
   constexpr int NW = NT / 64, PER = kTile / NT;
   __shared__ uint32_t cnt[NW][256];
   __shared__ uint32_t lstart[256];   // tile-local start of each digit
@@ -833,11 +901,10 @@ __global__ void __launch_bounds__(NT) k_lscatter(Dev d, int cur, int shift, int 
   __shared__ uint32_t skey[kTile];
   __shared__ uint32_t sval[kTile];
   __shared__ int32_t tmp[NW + 1];
-  const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
-  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
-  const Seg g = d.seg[cur][d.ltile_seg[t]];
-  const int pos = d.ltile_pos[t];
-  const uint64_t off = d.desc[g.blk].off;
+  const int tid = tid_, wv = tid >> 6, lane = (tid_ & 63);
+  const Seg g = seg_get(&d.seg[cur][u32(d.ltile_seg[t])]);
+  const int pos = u32(d.ltile_pos[t]);
+  const uint64_t off = u64(d.desc[g.blk].off);
   const uint32_t* Ks = src ? d.KEY2 : d.KEY;
   const uint32_t* Ss = src ? d.SA2 : d.SA;
   uint32_t* Kd = src ? d.KEY : d.KEY2;
@@ -947,13 +1014,22 @@ __global__ void __launch_bounds__(NT) k_lscatter(Dev d, int cur, int shift, int 
   }
 }
 
+template <int NT, bool GEN>
+__global__ void __launch_bounds__(NT) k_lscatter(Dev d, int shift, int src) {
+  const int cur = ctr_get(d, CTR_ROUND) & 1;
+  for (ItemLoop it(ctr_get(d, CTR_LTILES)); it.i < it.end; it.i += it.step) {
+    __syncthreads();  // the previous tile's LDS is no longer read
+    lscatter_item<NT, GEN>(d, cur, shift, src, it.i);
+  }
+}
+
 // first / last group boundary (seg-relative) of each large tile
-__global__ void __launch_bounds__(256) k_lsum(Dev d, int cur, int fin) {
+__device__ inline void lsum_item(const Dev& d, int cur, int fin, int t) {
+  const int tid_ = loop_tid();
   __shared__ int32_t tmp[8];
-  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
-  const Seg g = d.seg[cur][d.ltile_seg[t]];
-  const int pos = d.ltile_pos[t];
-  const uint64_t base = d.desc[g.blk].off + g.start;
+  const Seg g = seg_get(&d.seg[cur][u32(d.ltile_seg[t])]);
+  const int pos = u32(d.ltile_pos[t]);
+  const uint64_t base = u64(d.desc[g.blk].off) + g.start;
   const uint32_t* K = fin ? d.KEY2 : d.KEY;
   const int n = min(kTile, g.len - pos);
   int fb = 0x7fffffff, lb = -1;
@@ -962,23 +1038,23 @@ __global__ void __launch_bounds__(256) k_lsum(Dev d, int cur, int fin) {
   uint32_t kv[16], kp[16];
 #pragma unroll
   for (int q = 0; q < 16; q++) {
-    const int e = (int)threadIdx.x + 256 * q;
+    const int e = (int)tid_ + 256 * q;
     kv[q] = e < n ? K[base + pos + e] : 0u;
-    kp[q] = lane_id() == 0 && e < n && pos + e > 0 ? K[base + pos + e - 1] : 0u;
+    kp[q] = (tid_ & 63) == 0 && e < n && pos + e > 0 ? K[base + pos + e - 1] : 0u;
   }
 #pragma unroll
   for (int q = 0; q < 16; q++) {
-    const int e = (int)threadIdx.x + 256 * q;
+    const int e = (int)tid_ + 256 * q;
     const uint32_t below = (uint32_t)__shfl_up((int)kv[q], 1, 64);
-    const uint32_t prev = lane_id() == 0 ? kp[q] : below;
+    const uint32_t prev = (tid_ & 63) == 0 ? kp[q] : below;
     const int p = pos + e;
     if (e < n && (p == 0 || kv[q] != prev)) { fb = min(fb, p); lb = max(lb, p); }
   }
   fb = wave_min(fb);
   lb = wave_max(lb);
-  if (lane_id() == 0) { tmp[threadIdx.x >> 6] = fb; tmp[4 + (threadIdx.x >> 6)] = lb; }
+  if ((tid_ & 63) == 0) { tmp[tid_ >> 6] = fb; tmp[4 + (tid_ >> 6)] = lb; }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid_ == 0) {
     int f = tmp[0], l2 = tmp[4];
     for (int w = 1; w < 4; w++) { f = min(f, tmp[w]); l2 = max(l2, tmp[4 + w]); }
     d.lsum[2 * t] = f;
@@ -986,7 +1062,16 @@ __global__ void __launch_bounds__(256) k_lsum(Dev d, int cur, int fin) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
+__global__ void __launch_bounds__(256) k_lsum(Dev d, int fin) {
+  const int cur = ctr_get(d, CTR_ROUND) & 1;
+  for (ItemLoop it(ctr_get(d, CTR_LTILES)); it.i < it.end; it.i += it.step) {
+    __syncthreads();
+    lsum_item(d, cur, fin, it.i);
+  }
+}
+
+__device__ inline void lheads_item(const Dev& d, int cur, int fin, int t) {
+  const int tid_ = loop_tid();
   __shared__ int32_t tmp[8];
   __shared__ int32_t s_carry, s_base;
   __shared__ int32_t xs[256];
@@ -995,14 +1080,13 @@ __global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
   // per-bucket counters reuse the key staging area once the keys are read
   int32_t* bcnt = reinterpret_cast<int32_t*>(sk);
   static_assert((kStageMax + kRkBucket - 1) / kRkBucket <= kTile, "bucket counters fit the key area");
-  const int tid = threadIdx.x;
-  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
-  const int si = d.ltile_seg[t];
-  const Seg g = d.seg[cur][si];
-  const int pos = d.ltile_pos[t];
+  const int tid = tid_;
+  const int si = u32(d.ltile_seg[t]);
+  const Seg g = seg_get(&d.seg[cur][si]);
+  const int pos = u32(d.ltile_pos[t]);
   const int kt = pos / kTile;           // tile index within the group
   const int t0 = t - kt;
-  const uint64_t off = d.desc[g.blk].off;
+  const uint64_t off = u64(d.desc[g.blk].off);
   const uint64_t base = off + g.start;
   const uint32_t* K = fin ? d.KEY2 : d.KEY;
   const uint32_t* S = fin ? d.SA2 : d.SA;
@@ -1016,7 +1100,7 @@ __global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
       if (v >= 0) { lbv = v; break; }
     }
     lbv = wave_max(lbv);
-    if (lane_id() == 0) tmp[tid >> 6] = lbv;
+    if ((tid_ & 63) == 0) tmp[tid >> 6] = lbv;
     __syncthreads();
     if (tid == 0) s_carry = max(max(tmp[0], tmp[1]), max(tmp[2], tmp[3]));
     __syncthreads();
@@ -1092,8 +1176,8 @@ __global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
   // starts (k_rk_apply writes them), so that a bucket's updates land in one
   // 32 KiB window that is staged in LDS.
   {
-    const BlockDesc D = d.desc[g.blk];
-    const int nbk = (d.st[g.blk].m + kRkBucket - 1) / kRkBucket;
+    const int tile_c0 = u32(d.desc[g.blk].tile_c0);
+    const int nbk = (u32(d.st[g.blk].m) + kRkBucket - 1) / kRkBucket;
     for (int j = tid; j < nbk; j += 256) bcnt[j] = 0;
     __syncthreads();
     uint32_t lr[16];
@@ -1105,7 +1189,7 @@ __global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
     __syncthreads();
     for (int j = tid; j < nbk; j += 256) {
       const int c = bcnt[j];
-      if (c) bcnt[j] = atomicAdd(&d.rkb_cnt[D.tile_c0 + j * (kRkBucket / kTile)], c);
+      if (c) bcnt[j] = atomicAdd(&d.rkb_cnt[tile_c0 + j * (kRkBucket / kTile)], c);
     }
     __syncthreads();
 #pragma unroll
@@ -1148,29 +1232,38 @@ __global__ void __launch_bounds__(256) k_lheads(Dev d, int cur, int fin) {
   }
 }
 
-// Rank-update buckets: cleared before a round's large groups, applied after.
-__global__ void __launch_bounds__(256) k_rk_zero(Dev d, int ntile) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < ntile) d.rkb_cnt[i] = 0;
+__global__ void __launch_bounds__(256) k_lheads(Dev d, int fin) {
+  const int cur = ctr_get(d, CTR_ROUND) & 1;
+  for (ItemLoop it(ctr_get(d, CTR_LTILES)); it.i < it.end; it.i += it.step) {
+    __syncthreads();  // the previous tile's LDS is no longer read
+    lheads_item(d, cur, fin, it.i);
+  }
+}
+
+// Rank-update buckets: cleared before a round's large groups, applied after
+// (nothing to do in a round without large groups).
+__global__ void __launch_bounds__(256) k_rk_zero(Dev d) {
+  if (ctr_get(d, CTR_LTILES) == 0) return;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < d.ntile_c; i += gridDim.x * 256) d.rkb_cnt[i] = 0;
 }
 
 // A bucket's updates are applied to an LDS copy of its window of RK, then
 // the touched entries are written out in order.
-__global__ void __launch_bounds__(256) k_rk_apply(Dev d) {
+__device__ inline void rk_apply_item(const Dev& d, int t) {
+  const int tid_ = loop_tid();
   __shared__ uint32_t win[kRkBucket];
   __shared__ uint8_t touched[kRkBucket];
-  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
-  const int b = d.tc_blk[t];
-  const BlockDesc D = d.desc[b];
-  const int k = t - D.tile_c0;
+  const int b = u32(d.tc_blk[t]);
+  const int k = t - u32(d.desc[b].tile_c0);
   if (k % (kRkBucket / kTile)) return;
-  const int cnt = d.rkb_cnt[t];
+  const int cnt = u32(d.rkb_cnt[t]);
   if (cnt == 0) return;
+  const uint64_t boff = u64(d.desc[b].off);
   const uint32_t base = (uint32_t)k * kTile;  // first start of the bucket
-  for (int i = threadIdx.x; i < kRkBucket; i += 256) touched[i] = 0;
+  for (int i = tid_; i < kRkBucket; i += 256) touched[i] = 0;
   __syncthreads();
-  const uint64_t* src = d.RKB + D.off + base;
-  for (int i0 = threadIdx.x; i0 < cnt; i0 += 4096) {
+  const uint64_t* src = d.RKB + boff + base;
+  for (int i0 = tid_; i0 < cnt; i0 += 4096) {
     uint64_t p[16];  // 16 loads in flight per thread
 #pragma unroll
     for (int q = 0; q < 16; q++) p[q] = i0 + 256 * q < cnt ? src[i0 + 256 * q] : ~0ull;
@@ -1184,63 +1277,119 @@ __global__ void __launch_bounds__(256) k_rk_apply(Dev d) {
     }
   }
   __syncthreads();
-  const int span = min(kRkBucket, d.st[b].m - (int)base);
-  for (int i = threadIdx.x; i < span; i += 256)
-    if (touched[i]) d.RK[D.off + base + i] = win[i];
+  const int span = min(kRkBucket, u32(d.st[b].m) - (int)base);
+  for (int i = tid_; i < span; i += 256)
+    if (touched[i]) d.RK[boff + base + i] = win[i];
+}
+
+__global__ void __launch_bounds__(256) k_rk_apply(Dev d) {
+  if (ctr_get(d, CTR_LTILES) == 0) return;
+  // a bucket covers kRkBucket / kTile capacity tiles; its first tile applies it
+  for (ItemLoop it(d.ntile_c); it.i < it.end; it.i += it.step) {
+    __syncthreads();
+    rk_apply_item(d, it.i);
+  }
 }
 
 void launch_bwt_init(const Dev& d, hipStream_t s) {
   k_bwt_init<<<d.ntile_c, 256, 0, s>>>(d);
 }
 
-void launch_bwt_round(const Dev& d, int cur, int round, int32_t P, int key_bits,
-                      int32_t npacks, int32_t ntiles, int32_t nlarge, int32_t nseg,
-                      hipStream_t s, const int* scatter_nt, const hipEvent_t* pass_ev) {
-  k_mk_packs<<<(npacks + 1 + 255) / 256, 256, 0, s>>>(d, npacks, nseg);
-  k_mk_large<<<(nseg + 255) / 256, 256, 0, s>>>(d, cur, nseg);
-  if (round > 0 && npacks + ntiles > 0) {  // round-0 keys come from k_bwt_init
+namespace {
+// persistent grid of a kernel: every CU filled to the kernel's residency,
+// a multiple of the 8 XCDs
+template <typename K>
+int grid_for(K kernel, int threads, int cus) {
+  int per = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1) per = 1;
+  return std::max(8, (cus * per + 7) / 8 * 8);
+}
+}  // namespace
+
+SortGrids sort_grids(int device) {
+  SortGrids g;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 256;
+  g.light = grid_for(k_lhist<false>, 256, cus);
+  g.keys = grid_for(k_keys, 256, cus);
+  g.small = grid_for(k_sort_small, 256, cus);
+  g.small64 = grid_for(k_sort_small64, 256, cus);
+  g.scatter[0] = grid_for(k_lscatter<256, false>, 256, cus);
+  g.scatter[1] = grid_for(k_lscatter<512, false>, 512, cus);
+  g.scatter[2] = grid_for(k_lscatter<1024, false>, 1024, cus);
+  g.heads = grid_for(k_lheads, 256, cus);
+  g.apply = grid_for(k_rk_apply, 256, cus);
+  return g;
+}
+
+namespace {
+int round8(int n) { return n < 8 ? 8 : (n + 7) / 8 * 8; }
+}  // namespace
+
+void launch_bwt_round(const Dev& d, const SortGrids& g, bool first, int late_bits, int scatter_nt,
+                      hipStream_t s, const hipEvent_t* pass_ev, const int32_t* exact) {
+  // Without `exact`, the round plans itself and every kernel has its
+  // persistent grid.  With `exact` (the round's counters, read back after
+  // launch_bwt_plan), each kernel gets one workgroup per work item and is
+  // skipped when it has none.
+  int npacks = 0, ntiles = 0, nlarge = 0, nseg = 0;
+  if (exact) {
+    npacks = (exact[CTR_SMALL] + kHalf - 1) / kHalf;
+    ntiles = exact[CTR_LTILES];
+    nlarge = exact[CTR_NLARGE];
+    nseg = exact[CTR_NSEG];
+  } else {
+    launch_bwt_plan(d, s);
+  }
+  auto grid = [&](int persistent, int items) { return exact ? round8(items) : persistent; };
+  k_mk_packs<<<exact ? (npacks + 1 + 255) / 256 : g.light, 256, 0, s>>>(d);
+  k_mk_large<<<exact ? std::max(1, (nseg + 255) / 256) : g.light, 256, 0, s>>>(d);
+  if (!first && (!exact || npacks + ntiles > 0)) {  // round-0 keys come from the text
     probe_begin("k_keys", s);
-    k_keys<<<npacks + ntiles, 256, 0, s>>>(d, cur, round, P, npacks);
+    k_keys<<<grid(g.keys, npacks + ntiles), 256, 0, s>>>(d);
     probe_end("k_keys", s);
   }
-  if (npacks > 0) {
-    // composite key: group index in the pack (< 1024) above the key bits
+  // composite key of a pack sort: group index in the pack (< 1024) above
+  // the key bits
+  const int key_bits = first ? 32 : late_bits;
+  if (!exact || npacks > 0) {
     probe_begin("k_sort_small", s);
-    if (key_bits + 10 <= 32) k_sort_small<<<npacks, 256, 0, s>>>(d, cur, key_bits);
-    else k_sort_small64<<<npacks, 256, 0, s>>>(d, cur, key_bits);
+    if (key_bits + 10 <= 32) k_sort_small<<<grid(g.small, npacks), 256, 0, s>>>(d, key_bits);
+    else k_sort_small64<<<grid(g.small64, npacks), 256, 0, s>>>(d, key_bits);
     probe_end("k_sort_small", s);
   }
-  if (ntiles > 0) {
-    const int npass = (key_bits + 7) / 8;
-    for (int p = 0; p < npass; p++) {
-      if (pass_ev) (void)hipEventRecord(pass_ev[p], s);
-      // round 0's first pass reads the text (k_bwt_init leaves KEY / SA of
-      // the large blocks unwritten); it is not one of the probed passes
-      const bool gen = round == 0 && p == 0;
-      if (gen) k_lhist<true><<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
-      else k_lhist<false><<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
-      k_lscan<<<nlarge, 256, 0, s>>>(d, cur);
-      if (gen) {
-        if (scatter_nt[p] == 256) k_lscatter<256, true><<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
-        else if (scatter_nt[p] == 1024) k_lscatter<1024, true><<<ntiles, 1024, 0, s>>>(d, cur, 8 * p, p & 1);
-        else k_lscatter<512, true><<<ntiles, 512, 0, s>>>(d, cur, 8 * p, p & 1);
-        continue;
-      }
-      probe_begin("k_lscatter", s);
-      if (scatter_nt[p] == 256) k_lscatter<256, false><<<ntiles, 256, 0, s>>>(d, cur, 8 * p, p & 1);
-      else if (scatter_nt[p] == 1024) k_lscatter<1024, false><<<ntiles, 1024, 0, s>>>(d, cur, 8 * p, p & 1);
-      else k_lscatter<512, false><<<ntiles, 512, 0, s>>>(d, cur, 8 * p, p & 1);
-      probe_end("k_lscatter", s);
+  if (exact && ntiles == 0) return;
+  const int sv = scatter_nt == 256 ? 0 : (scatter_nt == 1024 ? 2 : 1);
+  const int sg = grid(g.scatter[sv], ntiles);
+  const int npass = (key_bits + 7) / 8;
+  for (int p = 0; p < npass; p++) {
+    if (pass_ev) (void)hipEventRecord(pass_ev[p], s);
+    // round 0's first pass reads the text (k_bwt_init leaves KEY / SA of the
+    // large blocks unwritten); it is not one of the probed passes
+    const bool gen = first && p == 0;
+    if (gen) k_lhist<true><<<grid(g.light, ntiles), 256, 0, s>>>(d, 8 * p, p & 1);
+    else k_lhist<false><<<grid(g.light, ntiles), 256, 0, s>>>(d, 8 * p, p & 1);
+    k_lscan<<<grid(g.light, nlarge), 256, 0, s>>>(d);
+    if (gen) {
+      if (sv == 0) k_lscatter<256, true><<<sg, 256, 0, s>>>(d, 8 * p, p & 1);
+      else if (sv == 2) k_lscatter<1024, true><<<sg, 1024, 0, s>>>(d, 8 * p, p & 1);
+      else k_lscatter<512, true><<<sg, 512, 0, s>>>(d, 8 * p, p & 1);
+      continue;
     }
-    if (pass_ev) (void)hipEventRecord(pass_ev[npass], s);
-    const int fin = npass & 1;
-    k_rk_zero<<<(d.ntile_c + 255) / 256, 256, 0, s>>>(d, d.ntile_c);
-    k_lsum<<<ntiles, 256, 0, s>>>(d, cur, fin);
-    probe_begin("k_lheads", s);
-    k_lheads<<<ntiles, 256, 0, s>>>(d, cur, fin);
-    probe_end("k_lheads", s);
-    k_rk_apply<<<d.ntile_c, 256, 0, s>>>(d);
+    probe_begin("k_lscatter", s);
+    if (sv == 0) k_lscatter<256, false><<<sg, 256, 0, s>>>(d, 8 * p, p & 1);
+    else if (sv == 2) k_lscatter<1024, false><<<sg, 1024, 0, s>>>(d, 8 * p, p & 1);
+    else k_lscatter<512, false><<<sg, 512, 0, s>>>(d, 8 * p, p & 1);
+    probe_end("k_lscatter", s);
   }
+  if (pass_ev) (void)hipEventRecord(pass_ev[npass], s);
+  const int fin = npass & 1;
+  k_rk_zero<<<exact ? std::max(1, (d.ntile_c + 255) / 256) : g.light, 256, 0, s>>>(d);
+  k_lsum<<<grid(g.light, ntiles), 256, 0, s>>>(d, fin);
+  probe_begin("k_lheads", s);
+  k_lheads<<<grid(g.heads, ntiles), 256, 0, s>>>(d, fin);
+  probe_end("k_lheads", s);
+  k_rk_apply<<<grid(g.apply, d.ntile_c), 256, 0, s>>>(d);
 }
 
 }  // namespace bzg

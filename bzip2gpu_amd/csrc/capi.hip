@@ -621,7 +621,7 @@ int64_t bzg_graph_replays(void) {
   Ctx* c = ctx_for(g_dev, &st);
   if (!c) return -1;
   int64_t n = 0;
-  for (int i = 0; i < 5; i++) n += c->eng.graph_replays(i);
+  for (int i = 0; i < Engine::kStageGroups; i++) n += c->eng.graph_replays(i);
   return n;
 }
 
@@ -630,7 +630,7 @@ int64_t bzg_graph_captures(void) {
   Ctx* c = ctx_for(g_dev, &st);
   if (!c) return -1;
   int64_t n = 0;
-  for (int i = 0; i < 5; i++) n += c->eng.graph_captures(i);
+  for (int i = 0; i < Engine::kStageGroups; i++) n += c->eng.graph_captures(i);
   return n;
 }
 

@@ -47,8 +47,8 @@ struct StageTimes {
 
 class Engine;
 
-// Most prefix-doubling rounds a suffix sort runs (prefix 4 * 2^40 > any block).
-constexpr int kMaxRounds = 42;
+// Most prefix-doubling rounds a suffix sort runs.
+constexpr int kMaxRounds = kMaxSortRounds;
 
 // Plain path: the blocks of one stream, each through its own pipeline run
 // on `s`, results copied back and serialised on the host (the reference's
@@ -67,8 +67,10 @@ void serialize_stream(int level, const BlockResult* blocks, size_t n_blocks, std
 void set_stage_graphs(bool on);
 bool stage_graphs();
 
-// per-kernel HIP-event probe (one target kernel name at a time)
+// per-kernel HIP-event probe (one target kernel name at a time); while a
+// target is set, stage groups run eagerly instead of as graphs
 void probe_set(const char* kernel);
+bool probe_active();
 int probe_read(float* total_ms, int* launches);
 
 class Engine {
@@ -129,7 +131,8 @@ class Engine {
   const std::vector<float>& round_ms() const { return round_ms_; }
   int device() const { return device_; }
   // graph captures and replays of stage group i (0 front, 1 MTF+RLE2,
-  // 2 Huffman, 3 pack, 4 assembly)
+  // 2 Huffman, 3 pack, 4 assembly, 5 suffix-sort round 0, 6 later round)
+  static constexpr int kStageGroups = 7;
   int64_t graph_captures(int i) const { return sgraph_[i].captures; }
   // per-engine switch (the plain per-block path plans a new batch per block,
   // where a capture per block would not pay off)
@@ -139,18 +142,28 @@ class Engine {
 
  private:
   int grow(size_t bytes, hipStream_t s);
-  int run_stage(int idx, hipStream_t s, int (Engine::*fn)(hipStream_t));
+  int run_stage(int idx, hipStream_t s, int (Engine::*fn)(hipStream_t), int variant = 0);
+  // one suffix-sort round from device-side counts (round 0 / a later round)
+  int sort_round0(hipStream_t s);
+  int sort_round(hipStream_t s);
   // Captured graphs of a stage group, one per recent device-side plan (a
   // batch split into sub-batches cycles through several plans per call).
   static constexpr int kGraphCache = 8;
   struct StageGraph {
     hipGraphExec_t exec[kGraphCache] = {};
     Dev key[kGraphCache] = {};
+    int variant[kGraphCache] = {};   // launch choices captured with it
     int64_t used[kGraphCache] = {};  // last use, for eviction
     int64_t clock = 0;
     int64_t captures = 0, replays = 0;
   };
-  StageGraph sgraph_[5];
+  StageGraph sgraph_[kStageGroups];
+  // suffix sort: persistent grids, key bits and scatter shapes of the run
+  SortGrids grids_{};
+  bool grids_ready_ = false;
+  int late_bits_ = 21;
+  int shape0_ = 256, shape1_ = 256;
+  bool late_large_seen_ = true;  // a later round had large groups (last timed run)
   bool use_graphs_ = true;
   int device_;
   Dev d_{};
@@ -171,7 +184,8 @@ class Engine {
   std::vector<std::array<hipEvent_t, 5>> tev_;  // around each radix pass, per round
   std::vector<int64_t> round_rows_;
   std::vector<float> round_ms_;
-  hipEvent_t ev_[8] = {};
+  hipEvent_t ev_[10] = {};  // stage bounds [0..6], BWT stage [7..8]
+  int32_t h_ctr_prev_[CTR_N] = {};  // counters of a round whose radix passes are timed
   int init_err_ = 0;
 };
 

@@ -88,6 +88,8 @@ void probe_end(const char* kernel, hipStream_t s) {
   P.open = false;
 }
 
+bool probe_active() { return !probe().target.empty(); }
+
 void probe_set(const char* kernel) {
   ProbeState& P = probe();
   P.target = kernel ? kernel : "";
@@ -312,6 +314,7 @@ int Engine::setup(const uint8_t* d_in, hipStream_t s, uint8_t* out) {
     d.hist = c.take<uint32_t>(ltile_cap * 256);
     d.lsum = c.take<int32_t>(ltile_cap * 2);
     d.ctr = c.take<int32_t>(CTR_N);
+    d.round_rows = c.take<int64_t>(2 * kMaxSortRounds);
     d.totals = c.take<int64_t>(kTotals);
     d.scan_part = c.take<int32_t>(3 * kScanWG);
     if (!out) d.out = c.take<uint8_t>(out_cap_ + 64);
@@ -362,84 +365,140 @@ int Engine::run_front(hipStream_t s) {
   return (int)hipGetLastError();
 }
 
+int Engine::sort_round0(hipStream_t s) {
+  launch_bwt_round(d_, grids_, true, late_bits_, shape0_, s, nullptr, nullptr);
+  return (int)hipGetLastError();
+}
+
+int Engine::sort_round(hipStream_t s) {
+  launch_bwt_round(d_, grids_, false, late_bits_, shape1_, s, nullptr, nullptr);
+  return (int)hipGetLastError();
+}
+
+// The suffix sort, in one of two launch policies (an autotuner knob, both
+// bit-identical):
+//  * per round: the round's plan runs, its totals are read back, and every
+//    kernel is launched with one workgroup per work item (skipped when it
+//    has none); the readback also ends the loop;
+//  * device-driven: every round plans itself and its kernels run on
+//    persistent grids that take their sizes from device memory, so round 0
+//    and then chunks of later rounds are enqueued (or replayed as captured
+//    graphs) back to back, with one readback per chunk to see whether the
+//    group list has run empty; rounds past the last one do nothing.
+// While the tuner still times a radix-scatter shape, rounds run per round
+// with events around the radix passes (the tuner's unit: rows x passes).
 int Engine::run_bwt(hipStream_t s, int* rounds_out) {
   active_elems_ = 0;
   small_rows_ = large_rows_ = large_pass_rows_ = 0;
   round_rows_.clear();
+  round_ms_.clear();
+  if (!grids_ready_) {
+    grids_ = sort_grids(device_);
+    grids_ready_ = true;
+    if (trace_on())
+      fprintf(stderr, "[bzg] sort grids: light %d keys %d small %d/%d scatter %d/%d/%d heads %d apply %d\n",
+              grids_.light, grids_.keys, grids_.small, grids_.small64, grids_.scatter[0], grids_.scatter[1],
+              grids_.scatter[2], grids_.heads, grids_.apply);
+  }
+  if (const int rc = reserve_round_events(kMaxRounds)) return rc;
+  late_bits_ = 1;
+  while ((1 << late_bits_) < max_cap_ + 1) late_bits_++;
+  const int late_pass = (late_bits_ + 7) / 8;
+  Tuner& tu = Tuner::get();
+  const int v0 = tu.pick(TUNE_RADIX_SHAPE, TUNE_CLASS_ROUND0), v1 = tu.pick(TUNE_RADIX_SHAPE, TUNE_CLASS_LATER);
+  shape0_ = tu.value(TUNE_RADIX_SHAPE, v0);
+  shape1_ = tu.value(TUNE_RADIX_SHAPE, v1);
+  const bool shape_tuning = tu.state(TUNE_RADIX_SHAPE, TUNE_CLASS_ROUND0, nullptr, nullptr) < 0 ||
+                            (late_large_seen_ && tu.state(TUNE_RADIX_SHAPE, TUNE_CLASS_LATER, nullptr, nullptr) < 0);
+  // launch policy per batch size class: launch latency weighs on small
+  // batches, exact grids on large ones
+  const int mode_cls = total_cap_ < (8 << 20) ? 0 : total_cap_ < (128 << 20) ? 1 : 2;
+  const int mode = shape_tuning ? 0 : tu.pick(TUNE_SORT_LAUNCH, mode_cls);
+  BZG_TRY(hipEventRecord(ev_[7], s));
   launch_bwt_init(d_, s);
   BZG_TRY(hipGetLastError());
-  int cur = 0, round = 0;
-  int32_t P = 0;
-  int keybits_late = 1;
-  while ((1 << keybits_late) < max_cap_ + 1) keybits_late++;
-  // The radix scatter's launch shape is autotuned: a round with large groups
-  // runs the shape the tuner picks for its size class and reports the radix
-  // passes' time per row and pass once the next readback has synchronised.
-  struct PendingTune { int round = -1, cls = 0, npass = 0, var = 0; int64_t units = 0; } tune;
-  auto report_tune = [&]() {
-    if (tune.round < 0) return;
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, tev_[tune.round][0], tev_[tune.round][tune.npass]) == hipSuccess && tune.units > 0)
-      Tuner::get().report(TUNE_RADIX_SHAPE, tune.cls, tune.var, (double)ms * 1e6 / (double)tune.units);
-    tune.round = -1;
-  };
-  for (;;) {
-    launch_bwt_plan(d_, cur, 0, s);
-    BZG_TRY(hipMemcpyAsync(h_ctr_, d_.ctr, CTR_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    BZG_TRY(hipStreamSynchronize(s));
-    report_tune();
-    const int32_t nseg = h_ctr_[CTR_NSEG];
-    if (h_ctr_[CTR_ERR]) return (int)hipErrorIllegalState;
-    if (nseg == 0) break;
-    if (round >= kMaxRounds - 1) return (int)hipErrorIllegalState;
-    const int32_t small = h_ctr_[CTR_SMALL];
-    const int32_t npacks = (small + kHalf - 1) / kHalf;
-    const int32_t ntiles = h_ctr_[CTR_LTILES];
-    const int32_t nlarge = h_ctr_[CTR_NLARGE];
-    const int keybits = round == 0 ? 32 : keybits_late;
-    active_elems_ += (int64_t)small + (int64_t)h_ctr_[CTR_LELEMS];
-    small_rows_ += small;
-    large_rows_ += h_ctr_[CTR_LELEMS];
-    // radix passes that read KEY / SA (round 0's first pass reads the text)
-    large_pass_rows_ += (int64_t)h_ctr_[CTR_LELEMS] * ((keybits + 7) / 8 - (round == 0 ? 1 : 0));
-    trace(s, "bwt round start", round, nseg, ntiles);
-    if (rev_.size() < (size_t)round + 1) {
-      hipEvent_t a = nullptr, b = nullptr;
-      BZG_TRY(hipEventCreate(&a));
-      BZG_TRY(hipEventCreate(&b));
-      rev_.emplace_back(a, b);
-    }
-    BZG_TRY(hipEventRecord(rev_[round].first, s));
-    int scatter_nt[4] = {512, 512, 512, 512};
-    const hipEvent_t* pass_ev = nullptr;
-    if (ntiles > 0) {
-      while (tev_.size() < (size_t)round + 1) {
-        std::array<hipEvent_t, 5> ev{};
-        for (auto& e : ev) BZG_TRY(hipEventCreate(&e));
-        tev_.push_back(ev);
+  int launched = 0;
+  if (mode == 0) {
+    bool late_large = false;
+    int timed = -1, timed_pass = 0;  // a round whose radix passes await their report
+    for (;; launched++) {
+      launch_bwt_plan(d_, s);
+      BZG_TRY(hipMemcpyAsync(h_ctr_, d_.ctr, CTR_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      BZG_TRY(hipStreamSynchronize(s));
+      if (timed >= 0) {  // the previous round is complete
+        float ms = 0.f;
+        const int64_t units = (int64_t)h_ctr_prev_[CTR_LELEMS] * timed_pass;
+        if (units > 0 && hipEventElapsedTime(&ms, tev_[timed][0], tev_[timed][timed_pass]) == hipSuccess)
+          tu.report(TUNE_RADIX_SHAPE, timed == 0 ? TUNE_CLASS_ROUND0 : TUNE_CLASS_LATER, timed == 0 ? v0 : v1,
+                    (double)ms * 1e6 / (double)units);
+        timed = -1;
       }
-      Tuner& tu = Tuner::get();
-      tune.round = round;
-      tune.cls = tune_radix_class(h_ctr_[CTR_LELEMS], nlarge);
-      tune.npass = (keybits + 7) / 8;
-      tune.units = (int64_t)h_ctr_[CTR_LELEMS] * tune.npass;
-      tune.var = tu.pick(TUNE_RADIX_SHAPE, tune.cls);
-      for (int p = 0; p < 4; p++) scatter_nt[p] = tu.value(TUNE_RADIX_SHAPE, tune.var);
-      pass_ev = tev_[round].data();
+      if (h_ctr_[CTR_ERR]) return (int)hipErrorIllegalState;
+      if (h_ctr_[CTR_NSEG] == 0) break;
+      if (launched >= kMaxRounds - 1) return (int)hipErrorIllegalState;
+      const bool first = launched == 0;
+      if (!first && h_ctr_[CTR_LTILES] > 0) late_large = true;
+      trace(s, "bwt round start", launched, h_ctr_[CTR_NSEG], h_ctr_[CTR_LTILES]);
+      const hipEvent_t* pass_ev = nullptr;
+      if (shape_tuning && h_ctr_[CTR_LTILES] > 0) {
+        pass_ev = tev_[launched].data();
+        timed = launched;
+        timed_pass = first ? 4 : late_pass;
+        std::memcpy(h_ctr_prev_, h_ctr_, sizeof h_ctr_prev_);
+      }
+      BZG_TRY(hipEventRecord(rev_[launched].first, s));
+      launch_bwt_round(d_, grids_, first, late_bits_, first ? shape0_ : shape1_, s, pass_ev, h_ctr_);
+      BZG_TRY(hipGetLastError());
+      BZG_TRY(hipEventRecord(rev_[launched].second, s));
     }
-    launch_bwt_round(d_, cur, round, P, keybits, npacks, ntiles, nlarge, nseg, s, scatter_nt, pass_ev);
-    BZG_TRY(hipGetLastError());
-    BZG_TRY(hipEventRecord(rev_[round].second, s));
-    round_rows_.push_back((int64_t)small + (int64_t)h_ctr_[CTR_LELEMS]);
-    trace(s, "bwt round done", round, npacks, nlarge);
-    cur ^= 1;
-    P = round == 0 ? 4 : (P >= (1 << 29) ? P : 2 * P);
-    round++;
+    if (shape_tuning) late_large_seen_ = late_large;
+  } else {
+    BZG_TRY(hipEventRecord(rev_[0].first, s));
+    int rc = run_stage(5, s, &Engine::sort_round0, shape0_);
+    if (rc) return rc;
+    BZG_TRY(hipEventRecord(rev_[0].second, s));
+    launched = 1;
+    for (int chunk = 2;; chunk = 4) {
+      BZG_TRY(hipMemcpyAsync(h_ctr_, d_.ctr, CTR_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      BZG_TRY(hipStreamSynchronize(s));
+      if (h_ctr_[CTR_ERR]) return (int)hipErrorIllegalState;
+      if (h_ctr_[launched & 1] == 0) break;  // the list of the next round is empty
+      if (launched >= kMaxRounds) return (int)hipErrorIllegalState;
+      for (int k = 0; k < chunk && launched < kMaxRounds; k++, launched++) {
+        BZG_TRY(hipEventRecord(rev_[launched].first, s));
+        rc = run_stage(6, s, &Engine::sort_round, shape1_);
+        if (rc) return rc;
+        BZG_TRY(hipEventRecord(rev_[launched].second, s));
+      }
+      trace(s, "bwt rounds", launched);
+    }
   }
-  report_tune();  // (the loop exits right after a synchronised readback)
-  if (rounds_out) *rounds_out = round;
-  round_ms_.assign((size_t)round, 0.f);
-  for (int r = 0; r < round; r++) BZG_TRY(hipEventElapsedTime(&round_ms_[r], rev_[r].first, rev_[r].second));
+  // rows of every round (small groups, then large groups) and round count
+  int64_t rr[2 * kMaxRounds];
+  BZG_TRY(hipEventRecord(ev_[8], s));
+  BZG_TRY(hipMemcpyAsync(rr, d_.round_rows, sizeof rr, hipMemcpyDeviceToHost, s));
+  BZG_TRY(hipMemcpyAsync(h_ctr_, d_.ctr, CTR_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  BZG_TRY(hipStreamSynchronize(s));
+  if (h_ctr_[CTR_ERR]) return (int)hipErrorIllegalState;
+  const int ran = std::min(h_ctr_[CTR_RAN], launched);
+  for (int r = 0; r < ran; r++) {
+    const int64_t sm = rr[r], lg = rr[kMaxRounds + r];
+    small_rows_ += sm;
+    large_rows_ += lg;
+    // radix passes that read KEY / SA (round 0's first pass reads the text)
+    large_pass_rows_ += lg * (r == 0 ? 3 : late_pass);
+    round_rows_.push_back(sm + lg);
+  }
+  active_elems_ = small_rows_ + large_rows_;
+  if (rounds_out) *rounds_out = ran;
+  round_ms_.assign((size_t)ran, 0.f);
+  for (int r = 0; r < ran; r++) BZG_TRY(hipEventElapsedTime(&round_ms_[r], rev_[r].first, rev_[r].second));
+  // the launch policy's cost per sorted row (BWT stage, both policies)
+  if (!shape_tuning && tu.state(TUNE_SORT_LAUNCH, mode_cls, nullptr, nullptr) < 0 && active_elems_ > 0) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev_[7], ev_[8]) == hipSuccess)
+      tu.report(TUNE_SORT_LAUNCH, mode_cls, mode, (double)ms * 1e6 / (double)active_elems_);
+  }
   return 0;
 }
 
@@ -468,14 +527,15 @@ int Engine::run_assemble(hipStream_t s) {
 // pointers, block/tile/stream counts, hence every grid size) and replayed
 // while that plan recurs; the stage groups have no host-sized launches,
 // copies or memsets, so a replay does exactly what the eager launches do.
-int Engine::run_stage(int idx, hipStream_t s, int (Engine::*fn)(hipStream_t)) {
-  // eager on the legacy stream, inside a caller's own capture, or when off
-  if (!use_graphs_ || !stage_graphs() || s == nullptr || capturing(s)) return (this->*fn)(s);
+int Engine::run_stage(int idx, hipStream_t s, int (Engine::*fn)(hipStream_t), int variant) {
+  // eager on the legacy stream, inside a caller's own capture, when off, or
+  // while a kernel is probed (events cannot bracket a kernel inside a graph)
+  if (!use_graphs_ || !stage_graphs() || s == nullptr || capturing(s) || probe_active()) return (this->*fn)(s);
   StageGraph& g = sgraph_[idx];
   g.clock++;
   int slot = -1, victim = 0;
   for (int i = 0; i < kGraphCache; i++) {
-    if (g.exec[i] && std::memcmp(&g.key[i], &d_, sizeof(Dev)) == 0) {
+    if (g.exec[i] && g.variant[i] == variant && std::memcmp(&g.key[i], &d_, sizeof(Dev)) == 0) {
       slot = i;
       break;
     }
@@ -500,6 +560,7 @@ int Engine::run_stage(int idx, hipStream_t s, int (Engine::*fn)(hipStream_t)) {
       return (int)ei;
     }
     std::memcpy(&g.key[slot], &d_, sizeof(Dev));
+    g.variant[slot] = variant;
     g.captures++;
   }
   g.used[slot] = g.clock;

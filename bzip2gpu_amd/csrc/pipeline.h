@@ -89,7 +89,8 @@ struct Dev {
   int32_t* lsum;       // [ltile][2] first/last group boundary in tile
   int64_t ltile_cap;
   int64_t pack_cap;
-  int32_t* ctr;        // counters: [0],[1] list sizes; [2..] plan totals
+  int32_t* ctr;        // counters: [0],[1] list sizes; [2..] plan totals, round
+  int64_t* round_rows; // [kMaxSortRounds] rows sorted in each suffix-sort round
   int64_t* totals;     // [kTotals] batch totals of the last run (k_block_totals)
   int32_t* scan_part;  // [3][kScanWG] scan partials
   // output
@@ -112,7 +113,42 @@ enum : int {
   CTR_NSEG = 5,    // number of groups in the current list
   CTR_ERR = 6,
   CTR_LELEMS = 7,  // total elements in large groups
+  CTR_ROUND = 8,   // index of the current suffix-sort round (-1 before round 0)
+  CTR_RAN = 9,     // rounds that had groups to sort
   CTR_N = 16
+};
+
+// Most prefix-doubling rounds a suffix sort runs (prefix 4 * 2^40 > any block).
+constexpr int kMaxSortRounds = 42;
+
+// Prefix length the groups of round r >= 1 already share (4, 8, 16, ...,
+// capped at 2^29, far beyond any block).
+__host__ __device__ inline int32_t round_prefix(int r) { return r >= 28 ? (1 << 29) : (4 << (r - 1)); }
+
+// Work-item loop of a persistent kernel (grid a multiple of 8): workgroup b
+// runs on XCD b mod 8, and the workgroups of XCD x take the items of the
+// x-th contiguous eighth of [0, n), so that neighbouring items (the tiles
+// of one block) share an L2.
+struct ItemLoop {
+  int i, end, step;
+  __device__ explicit ItemLoop(int n) {
+    const int x = (int)(blockIdx.x & 7u), per = (int)(gridDim.x >> 3);
+    i = (int)((int64_t)n * x / 8) + (int)(blockIdx.x >> 3);
+    end = (int)((int64_t)n * (x + 1) / 8);
+    step = per;
+  }
+};
+
+// Persistent grid sizes of the suffix-sort kernels (workgroups; multiples
+// of 8), from the device's CU count and each kernel's residency.
+struct SortGrids {
+  int light = 2048;   // plan, list, histogram, head-summary kernels
+  int keys = 2048;
+  int small = 1024;   // LDS pack sorts (32-bit composite keys)
+  int small64 = 512;  // LDS pack sorts (64-bit composite keys)
+  int scatter[3] = {1024, 512, 256};  // radix scatter, 256/512/1024 threads
+  int heads = 1024;
+  int apply = 1024;
 };
 
 // batch totals: blocks, raw bytes n, RLE1 bytes m, RLE2 symbols s, payload
@@ -123,13 +159,23 @@ constexpr int kTotals = 6;
 void launch_block_init(const Dev& d, hipStream_t s);
 void launch_crc_rle1(const Dev& d, hipStream_t s);   // CRC + RLE1 -> T, m, present
 void launch_bwt_init(const Dev& d, hipStream_t s);   // SA = identity, first group list
-// one suffix-sort round; returns via host the number of groups processed
-void launch_bwt_plan(const Dev& d, int cur, int nseg, hipStream_t s);
-// scatter_nt[p]: threads per radix-scatter workgroup of pass p (256, 512 or
-// 1024); pass_ev (may be null): npass + 1 events around the radix passes.
-void launch_bwt_round(const Dev& d, int cur, int round, int32_t P, int key_bits,
-                      int32_t npacks, int32_t ntiles, int32_t nlarge, int32_t nseg,
-                      hipStream_t s, const int* scatter_nt, const hipEvent_t* pass_ev);
+// The grid sizes of the sort kernels for this device (occupancy queries).
+SortGrids sort_grids(int device);
+// The plan of the next suffix-sort round: totals of its group list, round
+// counter advanced (the counters a host reads back to size the round).
+void launch_bwt_plan(const Dev& d, hipStream_t s);
+// One suffix-sort round.  Every kernel takes its work from the round's
+// group list in device memory and does nothing when there is none.
+// exact == null: the round plans itself and the kernels run on persistent
+// grids, so rounds are enqueued (or captured in a graph) with no host
+// readback, and a round after the last one does nothing.  exact = the
+// counters read back after launch_bwt_plan: one workgroup per work item.
+// first: round 0 (keys from the text, 32-bit keys); otherwise a later round
+// (keys gathered from the ranks, late_bits-bit keys).  scatter_nt: threads
+// per radix-scatter workgroup (256, 512 or 1024); pass_ev (may be null):
+// one event before each radix pass and one after the last.
+void launch_bwt_round(const Dev& d, const SortGrids& g, bool first, int late_bits, int scatter_nt,
+                      hipStream_t s, const hipEvent_t* pass_ev, const int32_t* exact);
 void launch_mtf_rle2(const Dev& d, hipStream_t s);   // MTF, orig_ptr, RLE2 -> SYM, s, freq
 void launch_mtf(const Dev& d, hipStream_t s);        // L gather, MTF, orig_ptr, n_in_use
 void launch_rle2(const Dev& d, hipStream_t s);       // RLE2 (needs tc_fnz, n_in_use)

@@ -30,12 +30,15 @@ Tuner::Tuner() {
 }
 
 int Tuner::variants(int knob) const {
-  return knob == TUNE_RADIX_SHAPE ? (int)(sizeof(kRadixShapes) / sizeof(kRadixShapes[0])) : 0;
+  if (knob == TUNE_RADIX_SHAPE) return (int)(sizeof(kRadixShapes) / sizeof(kRadixShapes[0]));
+  if (knob == TUNE_SORT_LAUNCH) return 2;
+  return 0;
 }
 
 int Tuner::value(int knob, int variant) const {
-  if (knob == TUNE_RADIX_SHAPE && variant >= 0 && variant < variants(knob)) return kRadixShapes[variant];
-  return 0;
+  if (variant < 0 || variant >= variants(knob)) return 0;
+  if (knob == TUNE_RADIX_SHAPE) return kRadixShapes[variant];
+  return variant;  // TUNE_SORT_LAUNCH: 0 per-round, 1 device-driven
 }
 
 int Tuner::pick(int knob, int cls) {
@@ -95,14 +98,6 @@ void Tuner::save_locked() {
     for (int c = 0; c < kTuneClasses; c++)
       if (slot_[k][c].chosen >= 0) std::fprintf(f, "%d %d %d\n", k, c, slot_[k][c].chosen);
   std::fclose(f);
-}
-
-int tune_radix_class(int64_t rows, int64_t groups) {
-  const int64_t mean = groups > 0 ? rows / groups : 0;
-  if (mean >= (1 << 18)) return 0;  // whole blocks (round 0)
-  if (mean >= (1 << 14)) return 1;
-  if (mean >= (1 << 12)) return 2;
-  return 3;
 }
 
 }  // namespace bzg

@@ -17,7 +17,9 @@ namespace bzg {
 
 enum TuneKnob : int {
   TUNE_RADIX_SHAPE = 0,  // threads per large-group radix-scatter workgroup
-  TUNE_KNOBS = 1
+  TUNE_SORT_LAUNCH = 1,  // suffix-sort rounds: per-round readback with exact
+                         // grids (0) or device-driven persistent grids (1)
+  TUNE_KNOBS = 2
 };
 constexpr int kTuneMaxVariants = 4;
 constexpr int kTuneClasses = 4;
@@ -48,7 +50,9 @@ class Tuner {
   int forced_[TUNE_KNOBS];
 };
 
-// size class of the radix knob from the mean large-group length
-int tune_radix_class(int64_t rows, int64_t groups);
+// Size classes of the radix knob: round 0 sorts whole blocks, the later
+// rounds the groups left over (their sizes are not known to the host when
+// the rounds are enqueued from device-side counts).
+enum : int { TUNE_CLASS_ROUND0 = 0, TUNE_CLASS_LATER = 1 };
 
 }  // namespace bzg

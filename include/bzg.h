@@ -167,23 +167,34 @@ bzg_status bzg_last_stage_times(float *h_ms6, int32_t *h_rounds,
                                 int64_t *h_bwt_rows);
 
 /* Stage graphs (no counterpart in the reference): the batched path runs the
- * CRC/RLE1, MTF/RLE2, Huffman, pack and assembly stage groups as HIP graphs
- * captured once per device-side plan and replayed while it is unchanged; the
- * suffix-sort rounds, whose launch sizes come from per-round readbacks, run
- * eagerly.  bzg_set_stage_graphs(0) runs everything eagerly (also
- * BZG_NO_GRAPHS=1 at start-up).  bzg_graph_captures / bzg_graph_replays
- * count stage-group captures and graph launches of this thread's device
- * context (-1 on error). */
+ * CRC/RLE1, MTF/RLE2, Huffman, pack and assembly stage groups and the
+ * suffix-sort rounds (round 0, and the later round replayed until the group
+ * list runs empty) as HIP graphs captured once per device-side plan and
+ * replayed while that plan recurs.  Every suffix-sort kernel takes its sizes
+ * from device memory, so no round needs a host readback; the host checks
+ * for the end once per chunk of rounds.  bzg_set_stage_graphs(0) runs
+ * everything eagerly (also BZG_NO_GRAPHS=1 at start-up), as does an active
+ * kernel probe (bzg_probe_kernel).  bzg_graph_captures / bzg_graph_replays
+ * count captures and graph launches of this thread's device context (-1 on
+ * error). */
 bzg_status bzg_set_stage_graphs(int32_t on);
 int64_t bzg_graph_captures(void);
 int64_t bzg_graph_replays(void);
 
-/* Launch-shape autotuner (no counterpart in the reference).  Knob 0 is the
- * threads per workgroup of the large-group radix scatter, variants 0/1/2 =
- * 256/512/1024; every variant gives identical output.  Choices are made per
- * size class (0: groups of >= 256Ki rows, i.e. whole blocks; 1: >= 16Ki;
- * 2: >= 4Ki; 3: smaller) after timing each variant twice, and are kept in
- * the file named by BZG_TUNE_CACHE when that variable is set.
+/* Launch-shape autotuner (no counterpart in the reference).  Every variant
+ * of every knob gives identical output.
+ * Knob 0 is the threads per workgroup of the large-group radix scatter,
+ * variants 0/1/2 = 256/512/1024, chosen per size class (0: suffix-sort
+ * round 0, whole blocks; 1: the later rounds) from the radix passes' time
+ * per row and pass (rounds run one at a time while it is being timed).
+ * Knob 1 is the launch policy of the suffix-sort rounds, variants 0/1 =
+ * per round (a readback of the round's totals, then one workgroup per work
+ * item) / device-driven (persistent grids sized from device memory, rounds
+ * enqueued or replayed as graphs in chunks), chosen per batch size class
+ * (0: under 8 MiB of block capacity, 1: under 128 MiB, 2: larger) from the
+ * sort stage's time per sorted row, once knob 0 has settled.
+ * A choice is made after timing each variant twice and is kept in the file
+ * named by BZG_TUNE_CACHE when that variable is set.
  * bzg_tune_force pins a variant for every class (-1: automatic).
  * bzg_tune_state returns the chosen variant of a class (-1 while exploring,
  * -2 for a bad argument) and fills, per variant, its launch value, mean

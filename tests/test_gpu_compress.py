@@ -138,6 +138,35 @@ def test_radix_shapes_identical(dev, variant):
         bz.tune_force(0, -1)
 
 
+@pytest.mark.parametrize("policy", [0, 1])
+def test_sort_launch_policies_identical(dev, policy):
+    """Both launch policies of the suffix-sort rounds (per round with exact
+    grids; device-driven persistent grids) give the oracle's stream, on a
+    batch that mixes short blocks, full blocks and a periodic block that
+    needs many rounds, and report the oracle's round count."""
+    import torch
+    import bzip2gpu_amd as bz
+    import bzip2gpu_amd.datagen as dg
+    per = dg.periodic(900000, 31, 7000, 0.001)
+    data = np.concatenate([dg.mix(3 << 20, 79), per])
+    want = oracle.compress(data, 9, threads=8)
+    try:
+        bz.tune_force(0, 1)
+        bz.tune_force(1, policy)
+        for graphs in (False, True, True):
+            _case(f"sort launch policy {policy}, stage graphs {graphs}")
+            bz.set_stage_graphs(graphs)
+            _same(bz.compress(data.tobytes(), 9), want, f"policy {policy} graphs {graphs}")
+            torch.cuda.synchronize()
+        _, rounds, _ = bz.last_stage_times()
+        r1 = np.frombuffer(oracle.rle1(per.tobytes()), np.uint8)
+        assert rounds >= oracle.bwt(r1)[1] - 1
+    finally:
+        bz.set_stage_graphs(True)
+        bz.tune_force(1, -1)
+        bz.tune_force(0, -1)
+
+
 def test_autotuner_explores_then_chooses(dev):
     import bzip2gpu_amd as bz
     import bzip2gpu_amd.datagen as dg
