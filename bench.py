@@ -415,7 +415,7 @@ def _autotune(run_once, max_runs=16):
             key, unit = names[k]
             val = rows[chosen][0] if chosen >= 0 else None
             if k == 1 and val is not None:
-                val = ("per_round", "device_driven")[val]
+                val = ("per_round", "device_driven_x1", "device_driven_x4")[val]
             out[f"knob{k}_class{c}"] = {key: val, unit: {str(v): round(ns, 4) for v, ns, _ in rows}}
     return out
 

@@ -1328,10 +1328,13 @@ int round8(int n) { return n < 8 ? 8 : (n + 7) / 8 * 8; }
 
 void launch_bwt_round(const Dev& d, const SortGrids& g, bool first, int late_bits, int scatter_nt,
                       hipStream_t s, const hipEvent_t* pass_ev, const int32_t* exact) {
-  // Without `exact`, the round plans itself and every kernel has its
-  // persistent grid.  With `exact` (the round's counters, read back after
-  // launch_bwt_plan), each kernel gets one workgroup per work item and is
-  // skipped when it has none.
+  // Without `exact`, the round plans itself.  Round 0's work is bounded by
+  // the plan (at most one pack, list entry and large group per block, at
+  // most one tile per capacity tile), so its kernels get one workgroup per
+  // possible item and the ones past the device-side count exit at once;
+  // later rounds run on persistent grids.  With `exact` (the round's
+  // counters, read back after launch_bwt_plan), each kernel gets one
+  // workgroup per work item and is skipped when it has none.
   int npacks = 0, ntiles = 0, nlarge = 0, nseg = 0;
   if (exact) {
     npacks = (exact[CTR_SMALL] + kHalf - 1) / kHalf;
@@ -1340,10 +1343,15 @@ void launch_bwt_round(const Dev& d, const SortGrids& g, bool first, int late_bit
     nseg = exact[CTR_NSEG];
   } else {
     launch_bwt_plan(d, s);
+    if (first) {
+      npacks = nlarge = nseg = d.nblk;
+      ntiles = d.ntile_c;
+    }
   }
-  auto grid = [&](int persistent, int items) { return exact ? round8(items) : persistent; };
-  k_mk_packs<<<exact ? (npacks + 1 + 255) / 256 : g.light, 256, 0, s>>>(d);
-  k_mk_large<<<exact ? std::max(1, (nseg + 255) / 256) : g.light, 256, 0, s>>>(d);
+  const bool sized = exact || first;
+  auto grid = [&](int persistent, int items) { return sized ? round8(items) : persistent; };
+  k_mk_packs<<<sized ? (npacks + 1 + 255) / 256 : g.light, 256, 0, s>>>(d);
+  k_mk_large<<<sized ? std::max(1, (nseg + 255) / 256) : g.light, 256, 0, s>>>(d);
   if (!first && (!exact || npacks + ntiles > 0)) {  // round-0 keys come from the text
     probe_begin("k_keys", s);
     k_keys<<<grid(g.keys, npacks + ntiles), 256, 0, s>>>(d);
@@ -1384,7 +1392,7 @@ void launch_bwt_round(const Dev& d, const SortGrids& g, bool first, int late_bit
   }
   if (pass_ev) (void)hipEventRecord(pass_ev[npass], s);
   const int fin = npass & 1;
-  k_rk_zero<<<exact ? std::max(1, (d.ntile_c + 255) / 256) : g.light, 256, 0, s>>>(d);
+  k_rk_zero<<<sized ? std::max(1, (d.ntile_c + 255) / 256) : g.light, 256, 0, s>>>(d);
   k_lsum<<<grid(g.light, ntiles), 256, 0, s>>>(d, fin);
   probe_begin("k_lheads", s);
   k_lheads<<<grid(g.heads, ntiles), 256, 0, s>>>(d, fin);

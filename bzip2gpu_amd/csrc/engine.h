@@ -159,7 +159,7 @@ class Engine {
   };
   StageGraph sgraph_[kStageGroups];
   // suffix sort: persistent grids, key bits and scatter shapes of the run
-  SortGrids grids_{};
+  SortGrids grids_{}, late_grids_{};
   bool grids_ready_ = false;
   int late_bits_ = 21;
   int shape0_ = 256, shape1_ = 256;

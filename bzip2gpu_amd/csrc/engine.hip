@@ -90,6 +90,26 @@ void probe_end(const char* kernel, hipStream_t s) {
 
 bool probe_active() { return !probe().target.empty(); }
 
+// Whether the probed kernel may launch inside stage group `idx` (such a
+// group runs eagerly so that its launches are timed; kernels of unknown
+// stage make every group eager).
+bool probe_in_stage(int idx) {
+  const std::string& k = probe().target;
+  if (k.empty()) return false;
+  static const struct { const char* kernel; int stage; } kStage[] = {
+      {"k_front1", 0}, {"k_rle1_write", 0}, {"k_mtf_gather", 1}, {"k_mtf_apply", 1}, {"k_rle2_write", 1},
+      {"k_huff_build", 2}, {"k_huff_codes", 2}, {"k_pack", 3}, {"k_place", 4},
+      {"k_lscatter", 5}, {"k_lscatter", 6}, {"k_keys", 6}, {"k_sort_small", 5}, {"k_sort_small", 6},
+      {"k_lheads", 5}, {"k_lheads", 6}};
+  bool known = false;
+  for (const auto& e : kStage) {
+    if (k != e.kernel) continue;
+    known = true;
+    if (e.stage == idx) return true;
+  }
+  return !known;
+}
+
 void probe_set(const char* kernel) {
   ProbeState& P = probe();
   P.target = kernel ? kernel : "";
@@ -371,20 +391,22 @@ int Engine::sort_round0(hipStream_t s) {
 }
 
 int Engine::sort_round(hipStream_t s) {
-  launch_bwt_round(d_, grids_, false, late_bits_, shape1_, s, nullptr, nullptr);
+  launch_bwt_round(d_, late_grids_, false, late_bits_, shape1_, s, nullptr, nullptr);
   return (int)hipGetLastError();
 }
 
-// The suffix sort, in one of two launch policies (an autotuner knob, both
+// The suffix sort, in one of three launch policies (an autotuner knob, all
 // bit-identical):
 //  * per round: the round's plan runs, its totals are read back, and every
 //    kernel is launched with one workgroup per work item (skipped when it
 //    has none); the readback also ends the loop;
-//  * device-driven: every round plans itself and its kernels run on
-//    persistent grids that take their sizes from device memory, so round 0
-//    and then chunks of later rounds are enqueued (or replayed as captured
-//    graphs) back to back, with one readback per chunk to see whether the
-//    group list has run empty; rounds past the last one do nothing.
+//  * device-driven: every round plans itself; round 0's kernels get grids
+//    bounded by the plan and the later rounds' kernels persistent grids (one
+//    or four residencies of the device) that take their work from device
+//    memory, so round 0 and then chunks of later rounds are enqueued (or
+//    replayed as captured graphs) back to back, with one readback per chunk
+//    to see whether the group list has run empty; rounds past the last one
+//    do nothing.
 // While the tuner still times a radix-scatter shape, rounds run per round
 // with events around the radix passes (the tuner's unit: rows x passes).
 int Engine::run_bwt(hipStream_t s, int* rounds_out) {
@@ -414,6 +436,14 @@ int Engine::run_bwt(hipStream_t s, int* rounds_out) {
   // batches, exact grids on large ones
   const int mode_cls = total_cap_ < (8 << 20) ? 0 : total_cap_ < (128 << 20) ? 1 : 2;
   const int mode = shape_tuning ? 0 : tu.pick(TUNE_SORT_LAUNCH, mode_cls);
+  // device-driven later rounds: persistent grids of 1 or 4 residencies
+  late_grids_ = grids_;
+  if (mode == 2) {
+    SortGrids& g = late_grids_;
+    for (int* v : {&g.light, &g.keys, &g.small, &g.small64, &g.scatter[0], &g.scatter[1], &g.scatter[2], &g.heads,
+                   &g.apply})
+      *v *= 4;
+  }
   BZG_TRY(hipEventRecord(ev_[7], s));
   launch_bwt_init(d_, s);
   BZG_TRY(hipGetLastError());
@@ -466,7 +496,7 @@ int Engine::run_bwt(hipStream_t s, int* rounds_out) {
       if (launched >= kMaxRounds) return (int)hipErrorIllegalState;
       for (int k = 0; k < chunk && launched < kMaxRounds; k++, launched++) {
         BZG_TRY(hipEventRecord(rev_[launched].first, s));
-        rc = run_stage(6, s, &Engine::sort_round, shape1_);
+        rc = run_stage(6, s, &Engine::sort_round, shape1_ | mode << 16);
         if (rc) return rc;
         BZG_TRY(hipEventRecord(rev_[launched].second, s));
       }
@@ -530,7 +560,7 @@ int Engine::run_assemble(hipStream_t s) {
 int Engine::run_stage(int idx, hipStream_t s, int (Engine::*fn)(hipStream_t), int variant) {
   // eager on the legacy stream, inside a caller's own capture, when off, or
   // while a kernel is probed (events cannot bracket a kernel inside a graph)
-  if (!use_graphs_ || !stage_graphs() || s == nullptr || capturing(s) || probe_active()) return (this->*fn)(s);
+  if (!use_graphs_ || !stage_graphs() || s == nullptr || capturing(s) || probe_in_stage(idx)) return (this->*fn)(s);
   StageGraph& g = sgraph_[idx];
   g.clock++;
   int slot = -1, victim = 0;

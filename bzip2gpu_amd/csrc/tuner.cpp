@@ -31,14 +31,14 @@ Tuner::Tuner() {
 
 int Tuner::variants(int knob) const {
   if (knob == TUNE_RADIX_SHAPE) return (int)(sizeof(kRadixShapes) / sizeof(kRadixShapes[0]));
-  if (knob == TUNE_SORT_LAUNCH) return 2;
+  if (knob == TUNE_SORT_LAUNCH) return 3;
   return 0;
 }
 
 int Tuner::value(int knob, int variant) const {
   if (variant < 0 || variant >= variants(knob)) return 0;
   if (knob == TUNE_RADIX_SHAPE) return kRadixShapes[variant];
-  return variant;  // TUNE_SORT_LAUNCH: 0 per-round, 1 device-driven
+  return variant;  // TUNE_SORT_LAUNCH: 0 per round, 1/2 device-driven (x1/x4 grids)
 }
 
 int Tuner::pick(int knob, int cls) {

@@ -18,7 +18,8 @@ namespace bzg {
 enum TuneKnob : int {
   TUNE_RADIX_SHAPE = 0,  // threads per large-group radix-scatter workgroup
   TUNE_SORT_LAUNCH = 1,  // suffix-sort rounds: per-round readback with exact
-                         // grids (0) or device-driven persistent grids (1)
+                         // grids (0) or device-driven, later rounds on
+                         // persistent grids of one (1) or four (2) residencies
   TUNE_KNOBS = 2
 };
 constexpr int kTuneMaxVariants = 4;

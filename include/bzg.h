@@ -187,10 +187,12 @@ int64_t bzg_graph_replays(void);
  * variants 0/1/2 = 256/512/1024, chosen per size class (0: suffix-sort
  * round 0, whole blocks; 1: the later rounds) from the radix passes' time
  * per row and pass (rounds run one at a time while it is being timed).
- * Knob 1 is the launch policy of the suffix-sort rounds, variants 0/1 =
+ * Knob 1 is the launch policy of the suffix-sort rounds, variants 0/1/2 =
  * per round (a readback of the round's totals, then one workgroup per work
- * item) / device-driven (persistent grids sized from device memory, rounds
- * enqueued or replayed as graphs in chunks), chosen per batch size class
+ * item) / device-driven (rounds plan themselves on the device and are
+ * enqueued or replayed as graphs in chunks; round 0 on grids bounded by the
+ * plan, later rounds on persistent grids of one or four residencies of the
+ * device), chosen per batch size class
  * (0: under 8 MiB of block capacity, 1: under 128 MiB, 2: larger) from the
  * sort stage's time per sorted row, once knob 0 has settled.
  * A choice is made after timing each variant twice and is kept in the file
