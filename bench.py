@@ -46,6 +46,8 @@ def _args():
     p.add_argument("--workers", type=int, default=4, help="serve: pipeline workers (HIP streams)")
     p.add_argument("--mib", type=int, default=0, help="override input size (MiB)")
     p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--max-batch-mib", type=int, default=1024, help="batch: raw MiB per device sub-batch")
+    p.add_argument("--no-overlap", action="store_true", help="batch: run sub-batches one after another")
     p.add_argument("--plain-threads", type=int, default=16,
                    help="batch4096: host threads of the reference-style plain path (0: skip)")
     p.add_argument("--cpu-sample-mib", type=int, default=256,
@@ -204,6 +206,8 @@ def run_batch(a):
     total = int(sizes.sum())
     d_pool = torch.from_numpy(pool).cuda()
     torch.cuda.synchronize()
+    bz.set_max_batch_bytes(a.max_batch_mib * MiB)
+    bz.set_batch_overlap(not a.no_overlap)
 
     def batched():
         return bz.compress_batch_device(d_pool, offs_u, sizes_u, levels, 0)
@@ -242,7 +246,9 @@ def run_batch(a):
                    "sizes": "floor(exp(U(ln 1 KiB, ln 8 MiB)))", "levels": "U{1..9}", "block_bytes": "100000*level",
                    "content": "slices of a seeded 256 MiB pool of 1 MiB segments: 50% order-2 text Zipf1.2 / "
                               "25% uniform / 25% runs mean 64",
-                   "path": "batched (sub-batches of <= 1 GiB raw)"},
+                   "path": f"batched (sub-batches of <= {a.max_batch_mib} MiB raw"
+                           + (", one after another)" if a.no_overlap else
+                              ", alternating between two pipelines on two HIP streams, sorts in order)")},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "stage_ms": {k: round(v, 3) for k, v in stages.items()}, "bwt_rounds": rounds,

@@ -38,6 +38,7 @@ from ._lib import (  # noqa: F401
     rle1,
     rle2,
     set_max_batch_bytes,
+    set_batch_overlap,
     symbols_table,
     set_stage_graphs,
     graph_counts,

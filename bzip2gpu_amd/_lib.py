@@ -67,6 +67,7 @@ def lib():
         "bzg_server_keep": ([P, P, ctypes.c_uint64], i32),
         "bzg_server_replay": ([P, P, i32, P, P, P, P, P, P, P, P], i32),
         "bzg_set_max_batch_bytes": ([ctypes.c_uint64], i32),
+        "bzg_set_batch_overlap": ([i32], i32),
         "bzg_last_stage_times": ([P, I32P, I64P], i32),
         "bzg_last_sort_rows": ([I64P, I64P, I64P], i32),
         "bzg_tune_force": ([i32, i32], i32),
@@ -275,6 +276,11 @@ def last_round_stats(max_rounds: int = 64):
 
 def set_max_batch_bytes(n: int) -> None:
     _check(lib().bzg_set_max_batch_bytes(int(n)), "bzg_set_max_batch_bytes")
+
+
+def set_batch_overlap(on: bool) -> None:
+    """Overlap consecutive sub-batches on two pipelines; see bzg_set_batch_overlap."""
+    _check(lib().bzg_set_batch_overlap(1 if on else 0), "bzg_set_batch_overlap")
 
 
 def last_stage_times():

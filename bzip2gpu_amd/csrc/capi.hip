@@ -2,7 +2,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -54,9 +56,14 @@ struct Ctx {
   uint64_t max_batch_bytes = 1ull << 30;  // raw bytes per device sub-batch
   uint8_t* d_res = nullptr;                // multi-sub-batch results
   size_t d_res_cap = 0;
+  // second engine and stream of overlapped sub-batches
+  std::unique_ptr<Engine> eng2;
+  hipStream_t stream2 = nullptr;
+  bool overlap = true;
   ~Ctx() {
     (void)hipSetDevice(device);
     for (auto ps : pool_streams) (void)hipStreamDestroy(ps);
+    if (stream2) (void)hipStreamDestroy(stream2);
   }
 };
 
@@ -288,7 +295,6 @@ bzg_status bzg_compress_batch_device(const uint8_t* d_in, int32_t n_inputs, cons
     }
   };
   std::vector<StreamSpec> ss;
-  std::vector<StreamState> lay;
   std::vector<int64_t> region(parts.size() + 1, 0);  // result-buffer offset of each sub-batch
   int e;
   if (parts.size() > 1) {
@@ -299,28 +305,105 @@ bzg_status bzg_compress_batch_device(const uint8_t* d_in, int32_t n_inputs, cons
     }
     if ((e = ensure_result(c, (size_t)region.back()))) return hip_fail(e, "result buffer");
   }
+  // Sub-batch p runs on engine p % 2 (its own HIP stream and host thread
+  // when there are two or more sub-batches and overlap is on); the front
+  // ends and suffix sorts run one at a time in sub-batch order (a gate the
+  // engines take before their front end and leave when their sort has
+  // ended), so sub-batch p's MTF, Huffman, pack and assembly overlap
+  // sub-batch p+1's front end and sort.
+  const bool two = parts.size() > 1 && c->overlap;
+  if (two && !c->eng2) {
+    if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess)
+      return fail(BZG_ERR_HIP, "hipStreamCreate");
+    c->eng2.reset(new Engine(c->device));
+  }
+  Engine* engs[2] = {&c->eng, two ? c->eng2.get() : &c->eng};
+  hipStream_t strs[2] = {s, two ? c->stream2 : s};
+  std::vector<StageTimes> ptimes(parts.size());
+  std::vector<std::array<int64_t, kTotals>> ptot(parts.size());
+  struct Gate {
+    std::mutex m;
+    std::condition_variable cv;
+    size_t next = 0;
+    bool abort = false;
+  } gate;
+  std::atomic<int> first_err{0};
+  std::string err_where;
+  std::mutex err_mu;
+  auto note = [&](int e, const char* where) {
+    std::lock_guard<std::mutex> lk(err_mu);
+    if (!first_err.load()) {
+      first_err = e;
+      err_where = where;
+    }
+    std::lock_guard<std::mutex> lg(gate.m);
+    gate.abort = true;
+    gate.cv.notify_all();
+  };
+  auto worker = [&](int j) {
+    if (hipSetDevice(c->device) != hipSuccess) return note((int)hipErrorInvalidDevice, "device");
+    Engine& E = *engs[j];
+    std::vector<StreamSpec> wss;
+    std::vector<StreamState> wlay;
+    for (size_t p = (size_t)j; p < parts.size(); p += two ? 2 : 1) {
+      if (two)
+        E.set_sort_gate([&gate, p](bool enter) -> int {
+          std::unique_lock<std::mutex> lk(gate.m);
+          if (enter) {
+            gate.cv.wait(lk, [&] { return gate.next == p || gate.abort; });
+            return gate.abort ? (int)hipErrorNotReady : 0;  // the other engine failed
+          }
+          gate.next = p + 1;
+          gate.cv.notify_all();
+          return 0;
+        });
+      const int i0 = parts[p].first, i1 = parts[p].second;
+      specs(i0, i1, wss);
+      uint8_t* dst = parts.size() > 1 ? c->d_res + region[p] : nullptr;
+      int e2;
+      if ((e2 = E.plan(wss))) return note(e2, "plan");
+      if ((e2 = E.setup(d_in, strs[j], dst))) return note(e2, "setup");
+      if ((e2 = E.run(strs[j], true, &ptimes[p]))) return note(e2, "run");
+      if ((e2 = E.stream_layout(wlay, strs[j]))) return note(e2, "layout");
+      for (int k = 0; k < kTotals; k++) ptot[p][k] = E.totals()[k];
+      for (int i = i0; i < i1; i++) {
+        h_byte_off[i] = region[p] + wlay[i - i0].byte_off;
+        h_bytes[i] = wlay[i - i0].bytes;
+      }
+    }
+  };
+  if (two) {
+    // the second stream starts after the caller's stream's earlier work
+    hipEvent_t ev = nullptr;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return fail(BZG_ERR_HIP, "event");
+    if (hipEventRecord(ev, s) != hipSuccess || hipStreamWaitEvent(c->stream2, ev, 0) != hipSuccess) {
+      (void)hipEventDestroy(ev);
+      return fail(BZG_ERR_HIP, "stream order");
+    }
+    std::thread t1(worker, 1);
+    worker(0);
+    t1.join();
+    for (Engine* E : engs) E->set_sort_gate(nullptr);
+    // the caller's stream continues after the second stream's work
+    hipError_t he2 = hipEventRecord(ev, c->stream2);
+    if (he2 == hipSuccess) he2 = hipStreamWaitEvent(s, ev, 0);
+    (void)hipEventDestroy(ev);
+    if (!first_err.load() && he2 != hipSuccess) note((int)he2, "stream order");
+  } else {
+    worker(0);
+  }
+  if (const int e3 = first_err.load()) return hip_fail(e3, err_where.c_str());
   StageTimes sum{};
   int64_t totals[kTotals] = {};
   for (size_t p = 0; p < parts.size(); p++) {
-    const int i0 = parts[p].first, i1 = parts[p].second;
-    specs(i0, i1, ss);
-    uint8_t* dst = parts.size() > 1 ? c->d_res + region[p] : nullptr;
-    if ((e = c->eng.plan(ss))) return hip_fail(e, "plan");
-    if ((e = c->eng.setup(d_in, s, dst))) return hip_fail(e, "setup");
-    StageTimes t{};
-    if ((e = c->eng.run(s, true, &t))) return hip_fail(e, "run");
+    const StageTimes& t = ptimes[p];
     sum.front += t.front; sum.bwt += t.bwt; sum.mtf += t.mtf; sum.huff += t.huff;
     sum.pack += t.pack; sum.assemble += t.assemble; sum.rounds = std::max(sum.rounds, t.rounds);
     sum.bwt_elems += t.bwt_elems;
     sum.small_rows += t.small_rows;
     sum.large_rows += t.large_rows;
     sum.large_pass_rows += t.large_pass_rows;
-    if ((e = c->eng.stream_layout(lay, s))) return hip_fail(e, "layout");
-    for (int k = 0; k < kTotals; k++) totals[k] += c->eng.totals()[k];
-    for (int i = i0; i < i1; i++) {
-      h_byte_off[i] = region[p] + lay[i - i0].byte_off;
-      h_bytes[i] = lay[i - i0].bytes;
-    }
+    for (int k = 0; k < kTotals; k++) totals[k] += ptot[p][k];
   }
   *d_out = parts.size() > 1 ? c->d_res : c->eng.out_buffer();
   hipError_t he = hipStreamSynchronize(s);
@@ -506,6 +589,15 @@ bzg_status bzg_set_max_batch_bytes(uint64_t bytes) {
   if (!c) return st;
   std::lock_guard<std::mutex> lk(c->mu);
   c->max_batch_bytes = bytes < (1u << 20) ? (1u << 20) : bytes;
+  return BZG_OK;
+}
+
+bzg_status bzg_set_batch_overlap(int32_t on) {
+  bzg_status st;
+  Ctx* c = ctx_for(g_dev, &st);
+  if (!c) return st;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->overlap = on != 0;
   return BZG_OK;
 }
 

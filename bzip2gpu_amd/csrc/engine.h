@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <utility>
 #include <array>
+#include <functional>
 #include <vector>
 
 #include "pipeline.h"
@@ -101,6 +102,10 @@ class Engine {
   // Full pipeline on the planned batch.  With assemble = true the .bz2
   // streams are written to the device output buffer.
   int run(hipStream_t s, bool assemble, StageTimes* times = nullptr);
+  // Called with true before a run's front end is enqueued (a nonzero
+  // return aborts the run) and with false once its suffix sort has ended on
+  // the device; a caller that overlaps engines orders their sorts with it.
+  void set_sort_gate(std::function<int(bool)> gate) { gate_ = std::move(gate); }
   // Stage groups (used by run and by the per-stage entry points).
   int run_front(hipStream_t s);
   int run_bwt(hipStream_t s, int* rounds_out = nullptr);
@@ -187,6 +192,7 @@ class Engine {
   hipEvent_t ev_[10] = {};  // stage bounds [0..6], BWT stage [7..8]
   int32_t h_ctr_prev_[CTR_N] = {};  // counters of a round whose radix passes are timed
   int init_err_ = 0;
+  std::function<int(bool)> gate_;
 };
 
 }  // namespace bzg

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 
@@ -50,17 +51,21 @@ bool stage_graphs() {
 }
 
 // ------------------------------------------------------------------ probe
+// Launches of the probed kernel are timed with an event pair each; host
+// threads driving engines concurrently share the list (under a mutex) and
+// each keeps the slot it has open.
 namespace {
 struct ProbeState {
+  std::mutex mu;
   std::string target;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
   size_t used = 0;
-  bool open = false;
 };
 ProbeState& probe() {
   static ProbeState p;
   return p;
 }
+thread_local long t_probe_slot = -1;  // this thread's open slot
 }  // namespace
 
 bool capturing(hipStream_t s) {
@@ -70,6 +75,8 @@ bool capturing(hipStream_t s) {
 
 void probe_begin(const char* kernel, hipStream_t s) {
   ProbeState& P = probe();
+  std::lock_guard<std::mutex> lk(P.mu);
+  t_probe_slot = -1;
   if (P.target.empty() || P.target != kernel) return;
   if (capturing(s)) return;  // kernels inside captured stage graphs are not probed
   if (P.used == P.ev.size()) {
@@ -78,23 +85,36 @@ void probe_begin(const char* kernel, hipStream_t s) {
     if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
     P.ev.emplace_back(a, b);
   }
-  if (hipEventRecord(P.ev[P.used].first, s) == hipSuccess) P.open = true;
+  if (hipEventRecord(P.ev[P.used].first, s) != hipSuccess) return;
+  // the slot is taken now; its second event follows the launch
+  t_probe_slot = (long)P.used++;
+  (void)hipEventRecord(P.ev[t_probe_slot].second, s);
 }
 
 void probe_end(const char* kernel, hipStream_t s) {
   ProbeState& P = probe();
-  if (!P.open || P.target != kernel) return;
-  if (hipEventRecord(P.ev[P.used].second, s) == hipSuccess) P.used++;
-  P.open = false;
+  std::lock_guard<std::mutex> lk(P.mu);
+  if (t_probe_slot < 0 || P.target != kernel) return;
+  (void)hipEventRecord(P.ev[t_probe_slot].second, s);
+  t_probe_slot = -1;
 }
 
-bool probe_active() { return !probe().target.empty(); }
+bool probe_active() {
+  ProbeState& P = probe();
+  std::lock_guard<std::mutex> lk(P.mu);
+  return !P.target.empty();
+}
 
 // Whether the probed kernel may launch inside stage group `idx` (such a
 // group runs eagerly so that its launches are timed; kernels of unknown
 // stage make every group eager).
 bool probe_in_stage(int idx) {
-  const std::string& k = probe().target;
+  std::string k;
+  {
+    ProbeState& P = probe();
+    std::lock_guard<std::mutex> lk(P.mu);
+    k = P.target;
+  }
   if (k.empty()) return false;
   static const struct { const char* kernel; int stage; } kStage[] = {
       {"k_front1", 0}, {"k_rle1_write", 0}, {"k_mtf_gather", 1}, {"k_mtf_apply", 1}, {"k_rle2_write", 1},
@@ -112,13 +132,14 @@ bool probe_in_stage(int idx) {
 
 void probe_set(const char* kernel) {
   ProbeState& P = probe();
+  std::lock_guard<std::mutex> lk(P.mu);
   P.target = kernel ? kernel : "";
   P.used = 0;
-  P.open = false;
 }
 
 int probe_read(float* total_ms, int* launches) {
   ProbeState& P = probe();
+  std::lock_guard<std::mutex> lk(P.mu);
   float tot = 0;
   for (size_t i = 0; i < P.used; i++) {
     hipError_t e = hipEventSynchronize(P.ev[i].second);
@@ -610,12 +631,16 @@ int Engine::run(hipStream_t s, bool assemble, StageTimes* times) {
   }
   int rounds = 0;
   trace(s, "run start", (long)desc_.size(), (long)ss_.size());
-  if (times) BZG_TRY(hipEventRecord(ev_[0], s));
-  int rc = run_stage(0, s, &Engine::run_front);
+  int rc = gate_ ? gate_(true) : 0;
   if (rc) return rc;
+  // front end and suffix sort (the gate is left once the sort has ended:
+  // run_bwt returns after its last readback)
+  rc = times ? (int)hipEventRecord(ev_[0], s) : 0;
+  if (!rc) rc = run_stage(0, s, &Engine::run_front);
   trace(s, "front done");
-  if (times) BZG_TRY(hipEventRecord(ev_[1], s));
-  rc = run_bwt(s, &rounds);
+  if (!rc && times) rc = (int)hipEventRecord(ev_[1], s);
+  if (!rc) rc = run_bwt(s, &rounds);
+  if (gate_) (void)gate_(false);
   if (rc) return rc;
   trace(s, "bwt done", rounds);
   if (times) BZG_TRY(hipEventRecord(ev_[2], s));

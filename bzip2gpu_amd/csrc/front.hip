@@ -222,13 +222,14 @@ __global__ void __launch_bounds__(256) k_front_scan(Dev d) {
 
 // RLE1 write pass: every position's bytes are staged in LDS in input order,
 // then copied to the block's RLE1 text with word stores (byte stores at the
-// two edge words shared with the neighbouring tiles).
+// two edge words shared with the neighbouring tiles).  A tile where every
+// position emits exactly its own byte (no run reaches 4) is staged as it was
+// loaded.
 __global__ void __launch_bounds__(256) k_rle1_write(Dev d) {
   __shared__ uint8_t lastb[256];
   __shared__ int32_t tmp[8];
   __shared__ int32_t xa[256], xe[256];
-  __shared__ uint8_t seen[256];
-  __shared__ __align__(16) uint8_t ob[2 * kTile + 4];
+  __shared__ __align__(16) uint8_t ob[2 * kTile + 16];
   const int tid = threadIdx.x;
   const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int b = d.tn_blk[t];
@@ -238,7 +239,6 @@ __global__ void __launch_bounds__(256) k_rle1_write(Dev d) {
   const int len = min(kTile, D.n - start);
   const uint8_t* src = d.in + D.in_off + start;
   const int ca = d.tn_lb[t], ce = d.tn_fb[t];
-  seen[tid] = 0;
   uint8_t x[16];
   const int c = load16(src, len, tid, x);
   lastb[tid] = c > 0 ? x[c - 1] : 0;
@@ -260,6 +260,7 @@ __global__ void __launch_bounds__(256) k_rle1_write(Dev d) {
   __syncthreads();
   int a = tid > 0 ? xa[tid - 1] : ca;
   int cnt = 0;
+  bool ident = true;  // every position emits its own byte, nothing else
   int8_t rr[16];
   {
     // offset in the 259-piece, carried from position to position
@@ -271,11 +272,20 @@ __global__ void __launch_bounds__(256) k_rle1_write(Dev d) {
       const int rq = q < c ? r : 4;
       rr[q] = (int8_t)(rq < 4 ? rq : 4);
       cnt += rq < 3 ? 1 : (rq == 3 ? 2 : 0);
+      if (q < c && rq >= 3) ident = false;
     }
   }
   int total;
   int o = wg_excl_sum<256, int>(cnt, tmp, &total);
-  {
+  if (__syncthreads_and(ident)) {
+    // identity tile: thread i's bytes are output bytes 16i..16i+c
+    uint32_t w[4];
+#pragma unroll
+    for (int k2 = 0; k2 < 4; k2++)
+      w[k2] = (uint32_t)x[4 * k2] | (uint32_t)x[4 * k2 + 1] << 8 | (uint32_t)x[4 * k2 + 2] << 16 |
+              (uint32_t)x[4 * k2 + 3] << 24;
+    *reinterpret_cast<uint4*>(ob + 16 * tid) = make_uint4(w[0], w[1], w[2], w[3]);
+  } else {
     // run end of every position: next run start after it
     int e = tid < 255 ? xe[tid + 1] : ce;
     uint32_t ends[16];
@@ -290,14 +300,11 @@ __global__ void __launch_bounds__(256) k_rle1_write(Dev d) {
       if (bmask & (1u << q)) a = p0 + q;
       if (rr[q] < 4) {
         ob[o++] = x[q];
-        seen[x[q]] = 1;
         if (rr[q] == 3) {
           // j = 259*piece + 3: a full piece when 259*(piece+1) <= run length,
           // else the last piece, whose count is run length - (j - 3) - 4
           const int runlen = (int)ends[q] - a, j = p0 + q - a;
-          const uint8_t cb = j + 256 <= runlen ? 255 : (uint8_t)(runlen - j - 1);
-          ob[o++] = cb;
-          seen[cb] = 1;
+          ob[o++] = j + 256 <= runlen ? 255 : (uint8_t)(runlen - j - 1);
         }
       }
     }
@@ -320,12 +327,6 @@ __global__ void __launch_bounds__(256) k_rle1_write(Dev d) {
         if (oq >= 0 && oq < total) dst[oq] = ob[oq];
       }
     }
-  }
-  const uint64_t m = __ballot(seen[tid] != 0);
-  if (lane_id() == 0) {
-    const int w2 = (tid >> 6) * 2;
-    if ((uint32_t)m) atomicOr(&d.st[b].present[w2], (uint32_t)m);
-    if ((uint32_t)(m >> 32)) atomicOr(&d.st[b].present[w2 + 1], (uint32_t)(m >> 32));
   }
 }
 

@@ -220,18 +220,19 @@ __device__ inline int2 heap_up(int2* heap, int zz, int2 tmp, int lane, int2* las
   return c == L ? tmp : readlane2(e, L - 1);
 }
 
-// LDS of one table's code-length build.
+// LDS of one table's code-length build (the leaf weights live in the
+// table's symbol-count array, converted in place).
 struct HeapLds {
   __align__(16) int2 heap[kHeapCap];
   int16_t anc[520];  // pointer-jumping ancestors (node ids <= 2*258-1)
   int16_t dep[520];
-  int32_t leafw[260];
 };
 // code_too_long's two queues reuse heap/anc/dep: kQueueCap entries each
 constexpr int kQueueCap = 384;
-static_assert(offsetof(HeapLds, leafw) >= 2 * kQueueCap * 4, "queue scratch overlaps leafw");
+static_assert(sizeof(HeapLds) >= 2 * kQueueCap * 4, "queue scratch exceeds the heap arrays");
 
-// Whether the tree built from H.leafw[1..alpha] is deeper than 17 levels.
+// Whether the tree built from the leaf weights W[0..alpha) (leaf i+1 has
+// weight W[i]) is deeper than 17 levels.
 // Any min-priority queue pops the same weight values in the same order
 // (only which of two equal weights comes first depends on the queue), and
 // the depth byte of a merged weight is one more than its deeper input, so the
@@ -241,7 +242,7 @@ static_assert(offsetof(HeapLds, leafw) >= 2 * kQueueCap * 4, "queue scratch over
 // Each queue is read through a 64-entry register window (one entry per
 // lane) that moves on every 64 pops; merged nodes also enter the window of
 // their queue when it covers them.
-__device__ bool code_too_long(HeapLds& H, int alpha, int lane) {
+__device__ bool code_too_long(HeapLds& H, const int32_t* W, int alpha, int lane) {
   int32_t* S = reinterpret_cast<int32_t*>(&H);  // sorted leaf weights
   int32_t* C = S + kQueueCap;                   // merged weights, in order
   uint32_t key[5];
@@ -249,7 +250,7 @@ __device__ bool code_too_long(HeapLds& H, int alpha, int lane) {
 #pragma unroll
   for (int q = 0; q < 5; q++) {
     const int i = lane + 64 * q + 1;
-    key[q] = i <= alpha ? ((uint32_t)(H.leafw[i] >> 8) << 9) | (uint32_t)i : 0xFFFFFFFFu;
+    key[q] = i <= alpha ? ((uint32_t)(W[i - 1] >> 8) << 9) | (uint32_t)i : 0xFFFFFFFFu;
     rk[q] = 0;
   }
   // rank of every leaf key among all keys (keys are distinct)
@@ -303,29 +304,29 @@ __device__ bool code_too_long(HeapLds& H, int alpha, int lane) {
   return deep;
 }
 
-// Code lengths of one table from its symbol counts fr[0..alpha) (LDS), by
-// one wave; len[0..alpha) (LDS) receives them.  The weight halving rounds of
-// libbz2 (huffman.cu:110-123) are decided by code_too_long before the one
-// exact build.
-__device__ void build_lengths(HeapLds& H, const int32_t* fr, uint8_t* len, int alpha, int lane) {
+// Code lengths of one table from its symbol counts W[0..alpha) (LDS, turned
+// into the leaf weights in place), by one wave; len[0..alpha) (global)
+// receives them.  The weight halving rounds of libbz2 (huffman.cu:110-123)
+// are decided by code_too_long before the one exact build.
+__device__ void build_lengths(HeapLds& H, int32_t* W, uint8_t* len, int alpha, int lane) {
   const WinLane wl = window_lane(lane);
   for (int i = lane; i < alpha; i += 64) {
-    const int f = fr[i];
-    H.leafw[i + 1] = (f == 0 ? 1 : f) << 8;
+    const int f = W[i];
+    W[i] = (f == 0 ? 1 : f) << 8;
   }
   wave_lds_sync();
-  while (code_too_long(H, alpha, lane)) {
-    for (int i = lane + 1; i <= alpha; i += 64) {
-      int j = H.leafw[i] >> 8;
+  while (code_too_long(H, W, alpha, lane)) {
+    for (int i = lane; i < alpha; i += 64) {
+      int j = W[i] >> 8;
       j = 1 + (j / 2);
-      H.leafw[i] = j << 8;
+      W[i] = j << 8;
     }
     wave_lds_sync();
   }
   while (true) {
     int nNodes = alpha, nHeap = 0;
     for (int i0 = 1; i0 <= alpha; i0 += 64) {
-      const int lw = i0 + lane <= alpha ? H.leafw[i0 + lane] : 0;
+      const int lw = i0 + lane <= alpha ? W[i0 + lane - 1] : 0;
       const int n = min(64, alpha + 1 - i0);
       for (int q = 0; q < n; q++) {
         nHeap++;
@@ -381,10 +382,10 @@ __device__ void build_lengths(HeapLds& H, const int32_t* fr, uint8_t* len, int a
       if (j > 17) tooLong = true;
     }
     if (!__ballot(tooLong)) break;
-    for (int i = lane + 1; i <= alpha; i += 64) {
-      int j = H.leafw[i] >> 8;
+    for (int i = lane; i < alpha; i += 64) {
+      int j = W[i] >> 8;
       j = 1 + (j / 2);
-      H.leafw[i] = j << 8;
+      W[i] = j << 8;
     }
     wave_lds_sync();
   }
@@ -419,13 +420,13 @@ __global__ void __launch_bounds__(256) k_huff_order(Dev d) {
   for (int b = tid; b < d.nblk; b += 256) d.horder[atomicAdd(&cnt[259 - min(d.st[b].n_in_use + 2, 259)], 1)] = b;
 }
 
-// 6 waves: all assign groups, wave t builds table t.  39 KB of LDS and at
-// most 80 VGPRs let four blocks share a CU, so that the heap chains of
-// different blocks overlap.
+// 6 waves: all assign groups, wave t builds table t.  31 KB of LDS (the
+// current lengths stay in global memory, the counts become the weights in
+// place) and at most 64 VGPRs let five blocks share a CU, so that the heap
+// chains of different blocks overlap.
 constexpr int kHuffThreads = 384;
-__global__ void __launch_bounds__(kHuffThreads) __attribute__((amdgpu_waves_per_eu(6))) k_huff_build(Dev d) {
+__global__ void __launch_bounds__(kHuffThreads) __attribute__((amdgpu_waves_per_eu(8))) k_huff_build(Dev d) {
   __shared__ int32_t rf[6][260];
-  __shared__ uint8_t L[6][260];
   __shared__ HeapLds H[6];
   // the six lengths of a symbol packed in one word, used while the groups
   // are assigned; it shares table 0's heap, which is idle then
@@ -437,15 +438,15 @@ __global__ void __launch_bounds__(kHuffThreads) __attribute__((amdgpu_waves_per_
   const BlockState* st = d.st + b;
   const int ng = st->n_groups, nsel = st->n_sel, s = st->s;
   const int alpha = st->n_in_use + 2;
+  // the tables' current lengths, rewritten by every build (a workgroup
+  // barrier orders a wave's global stores before the other waves' loads)
   uint8_t* glen = d.len + (int64_t)b * 6 * 260;
-  for (int q = tid; q < 6 * 260; q += kHuffThreads) (&L[0][0])[q] = glen[q];
-  __syncthreads();
   const uint32_t* sym2 = reinterpret_cast<const uint32_t*>(d.SYM + D.off);  // 2 symbols per word
   for (int it = 0; it < 4; it++) {
     for (int v = tid; v < 260; v += kHuffThreads) {
       uint64_t w = 0;
 #pragma unroll
-      for (int tt = 0; tt < 6; tt++) w |= (uint64_t)L[tt][v] << (10 * tt);
+      for (int tt = 0; tt < 6; tt++) w |= (uint64_t)glen[tt * 260 + v] << (10 * tt);
       PL[v] = w;
     }
     for (int q = tid; q < 6 * 260; q += kHuffThreads) (&rf[0][0])[q] = 0;
@@ -488,10 +489,9 @@ __global__ void __launch_bounds__(kHuffThreads) __attribute__((amdgpu_waves_per_
       }
     }
     __syncthreads();
-    if (wv < ng) build_lengths(H[wv], rf[wv], L[wv], alpha, lane);
+    if (wv < ng) build_lengths(H[wv], rf[wv], glen + wv * 260, alpha, lane);
     __syncthreads();
   }
-  for (int q = tid; q < 6 * 260; q += kHuffThreads) glen[q] = (&L[0][0])[q];
 }
 
 // Codes (huffman.cu:135-146, :352-362), selector MTF (compress.cu:119-137)

@@ -63,19 +63,33 @@ __global__ void __launch_bounds__(256) k_mtf_gather(Dev d) {
   d.lo[(int64_t)t * 256 + tid] = max(max(lo[0][tid], lo[1][tid]), max(lo[2][tid], lo[3][tid]));
 }
 
-// Per block: entering list state of every segment (exclusive max-scan).
+// Per block: the set of byte values (those with a last occurrence in some
+// segment; the last column is a permutation of the RLE1 text) and the
+// entering list state of every segment (exclusive max-scan).
 __global__ void __launch_bounds__(256) k_mtf_scan(Dev d) {
   __shared__ int32_t tmp[8];
   const int b = blockIdx.x;
   const BlockDesc D = d.desc[b];
   const int m = d.st[b].m;
   const int s = threadIdx.x;
-  const bool present = (d.st[b].present[s >> 5] >> (s & 31)) & 1u;
+  const int nt = tiles_for(m);
+  bool present = false;
+  for (int k0 = 0; k0 < nt; k0 += 16) {
+    int v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = k0 + q < nt ? d.lo[(int64_t)(D.tile_c0 + k0 + q) * 256 + s] : -1;
+#pragma unroll
+    for (int q = 0; q < 16; q++) present |= v[q] >= 0;
+  }
+  const uint64_t pm = __ballot(present);
+  if (lane_id() == 0) {
+    d.st[b].present[2 * (s >> 6)] = (uint32_t)pm;
+    d.st[b].present[2 * (s >> 6) + 1] = (uint32_t)(pm >> 32);
+  }
   int tot;
   int rank = wg_excl_sum<256, int>(present ? 1 : 0, tmp, &tot);
   if (s == 0) d.st[b].n_in_use = tot;
   int E = present ? -1 - rank : (int)0x80000000;
-  const int nt = tiles_for(m);
   // 16 tiles' loads in flight per step (the scan itself is cheap)
   for (int k0 = 0; k0 < nt; k0 += 16) {
     int v[16];

@@ -160,6 +160,14 @@ bzg_status bzg_compress_segment(const uint8_t *d_in, size_t len,
  * sub-batches with bounded device memory (about 70 bytes per input byte). */
 bzg_status bzg_set_max_batch_bytes(uint64_t bytes);
 
+/* Overlap of consecutive sub-batches (default on; no counterpart in the
+ * reference): with two or more sub-batches, they alternate between two
+ * pipelines on two HIP streams driven by two host threads, and sub-batch
+ * p+1's CRC/RLE1 and suffix sort run while sub-batch p's MTF, Huffman,
+ * packing and assembly do (the sorts stay in sub-batch order).  Output is
+ * byte-identical either way; the second pipeline holds its own workspace. */
+bzg_status bzg_set_batch_overlap(int32_t on);
+
 /* Per-stage times of the last batched call on this thread's device (ms,
  * HIP events), the number of suffix-sort rounds, and the suffix-sort rows
  * keyed and sorted, summed over rounds. */

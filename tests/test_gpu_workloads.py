@@ -109,6 +109,38 @@ def test_batch_inputs_up_to_8mib_across_sub_batches(dev):
         _same(got, oracle.compress(d, lv, block_bytes=100000 * lv, threads=8), f"input {i} ({sizes[i]} B, L{lv})")
 
 
+@pytest.mark.parametrize("overlap", [True, False])
+def test_sub_batch_overlap_identical(dev, overlap):
+    """Sub-batches on two overlapped pipelines (and on one) give the oracle's
+    streams, with empty, tiny and multi-block inputs across 2 MiB
+    sub-batches."""
+    import torch
+    import bzip2gpu_amd as bz
+    import bzip2gpu_amd.datagen as dg
+    sizes = [1500000, 0, 700000, 2 * MiB + 5, 1, 900001, 300000, 1200000, 64, 2 * MiB - 7]
+    levels = np.array([9, 1, 2, 9, 5, 3, 1, 7, 9, 4], np.int32)
+    datas = [dg.mix(s, 1300 + i) if s else np.zeros(0, np.uint8) for i, s in enumerate(sizes)]
+    offs = np.cumsum([0] + [s + (-s) % 256 for s in sizes[:-1]]).astype(np.uint64)
+    pool = np.zeros(int(offs[-1]) + sizes[-1] + 256, np.uint8)
+    for i, d in enumerate(datas):
+        pool[int(offs[i]):int(offs[i]) + d.size] = d
+    t = torch.from_numpy(pool).to(dev)
+    bz.set_max_batch_bytes(2 * MiB)
+    bz.set_batch_overlap(overlap)
+    try:
+        _case(f"sub-batch overlap {overlap}: {len(sizes)} inputs, 2 MiB sub-batches")
+        for rep in range(2):
+            ptr, bo, by = bz.compress_batch_device(t, offs, sizes, levels, 0)
+            allout = _d2h(ptr, int(bo[-1] + by[-1]))
+            for i, d in enumerate(datas):
+                lv = int(levels[i])
+                _same(allout[int(bo[i]):int(bo[i] + by[i])],
+                      oracle.compress(d, lv, block_bytes=100000 * lv, threads=8), f"rep {rep} input {i}")
+    finally:
+        bz.set_max_batch_bytes(1 << 30)
+        bz.set_batch_overlap(True)
+
+
 def _requests(n, seed, median):
     rng = np.random.default_rng(seed)
     sizes = np.minimum(16 * MiB, np.ceil(np.exp(rng.normal(np.log(median), 1.5, n)))).astype(np.int64)
