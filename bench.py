@@ -48,6 +48,8 @@ def _args():
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--max-batch-mib", type=int, default=1024, help="batch: raw MiB per device sub-batch")
     p.add_argument("--no-overlap", action="store_true", help="batch: run sub-batches one after another")
+    p.add_argument("--tune-force", default="",
+                   help="pin autotuner variants instead of timing them, e.g. 0=0,1=0 (knob=variant)")
     p.add_argument("--plain-threads", type=int, default=16,
                    help="batch4096: host threads of the reference-style plain path (0: skip)")
     p.add_argument("--cpu-sample-mib", type=int, default=256,
@@ -212,7 +214,7 @@ def run_batch(a):
     def batched():
         return bz.compress_batch_device(d_pool, offs_u, sizes_u, levels, 0)
 
-    tune = _autotune(batched)
+    tune = _autotune(batched, forced=a.tune_force)
     for _ in range(max(0, a.warmup)):
         batched()
     torch.cuda.synchronize()
@@ -394,13 +396,20 @@ def run_serve(a):
     print(json.dumps(res), flush=True)
 
 
-def _autotune(run_once, max_runs=16):
+def _autotune(run_once, max_runs=16, forced=""):
     """Untimed set-up: lets the library's autotuner time each launch shape of
     the radix scatter (knob 0), then each launch policy of the suffix-sort
     rounds (knob 1), on this workload until every size class it met has
-    settled (each variant is timed twice per class)."""
+    settled (each variant is timed twice per class).  `forced` ("knob=variant,
+    ...") pins variants instead (profiling runs)."""
     import torch
     import bzip2gpu_amd as bz
+
+    if forced:
+        pins = dict(tuple(int(x) for x in kv.split("=")) for kv in forced.split(","))
+        for k, v in pins.items():
+            bz.tune_force(k, v)
+        return {"forced": {f"knob{k}": v for k, v in pins.items()}}
 
     def seen(knob):
         out = [(c, bz.tune_state(knob, c)) for c in range(4)]
@@ -560,7 +569,7 @@ def main():
     def batched():
         return bz.compress_batch_device(d_in, [0], [n], [level], bb)
 
-    tune = _autotune(batched)
+    tune = _autotune(batched, forced=a.tune_force)
     for _ in range(max(0, a.warmup)):
         batched()
     torch.cuda.synchronize()
