@@ -297,11 +297,13 @@ bzg_status bzg_compress_batch_device(const uint8_t* d_in, int32_t n_inputs, cons
   std::vector<StreamSpec> ss;
   std::vector<int64_t> region(parts.size() + 1, 0);  // result-buffer offset of each sub-batch
   int e;
+  bool tuning = false;  // some sub-batch would time autotuner variants
   if (parts.size() > 1) {
     for (size_t p = 0; p < parts.size(); p++) {
       specs(parts[p].first, parts[p].second, ss);
       if ((e = c->eng.plan(ss))) return hip_fail(e, "plan");
       region[p + 1] = region[p] + ((c->eng.planned_out_bytes() + 255) & ~255ll);
+      tuning = tuning || c->eng.will_tune();
     }
     if ((e = ensure_result(c, (size_t)region.back()))) return hip_fail(e, "result buffer");
   }
@@ -311,7 +313,9 @@ bzg_status bzg_compress_batch_device(const uint8_t* d_in, int32_t n_inputs, cons
   // engines take before their front end and leave when their sort has
   // ended), so sub-batch p's MTF, Huffman, pack and assembly overlap
   // sub-batch p+1's front end and sort.
-  const bool two = parts.size() > 1 && c->overlap;
+  // (runs that time autotuner variants take the sub-batches one at a time,
+  // so that the other pipeline does not distort the timings)
+  const bool two = parts.size() > 1 && c->overlap && !tuning;
   if (two && !c->eng2) {
     if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess)
       return fail(BZG_ERR_HIP, "hipStreamCreate");

@@ -106,6 +106,8 @@ class Engine {
   // return aborts the run) and with false once its suffix sort has ended on
   // the device; a caller that overlaps engines orders their sorts with it.
   void set_sort_gate(std::function<int(bool)> gate) { gate_ = std::move(gate); }
+  // Whether a run of the planned batch would time autotuner variants.
+  bool will_tune() const;
   // Stage groups (used by run and by the per-stage entry points).
   int run_front(hipStream_t s);
   int run_bwt(hipStream_t s, int* rounds_out = nullptr);
@@ -149,6 +151,8 @@ class Engine {
   int grow(size_t bytes, hipStream_t s);
   int run_stage(int idx, hipStream_t s, int (Engine::*fn)(hipStream_t), int variant = 0);
   // one suffix-sort round from device-side counts (round 0 / a later round)
+  bool shape_tuning_pending() const;
+  int sort_launch_class() const;
   int sort_round0(hipStream_t s);
   int sort_round(hipStream_t s);
   // Captured graphs of a stage group, one per recent device-side plan (a

@@ -406,6 +406,24 @@ int Engine::run_front(hipStream_t s) {
   return (int)hipGetLastError();
 }
 
+// The radix-scatter shape is still being timed for round 0, or for the
+// later rounds when the last timed run had large groups there.
+bool Engine::shape_tuning_pending() const {
+  Tuner& tu = Tuner::get();
+  return tu.exploring(TUNE_RADIX_SHAPE, TUNE_CLASS_ROUND0) ||
+         (late_large_seen_ && tu.exploring(TUNE_RADIX_SHAPE, TUNE_CLASS_LATER));
+}
+
+// Size class of the sort-launch knob: launch latency weighs on small
+// batches, exact grids on large ones.
+int Engine::sort_launch_class() const {
+  return total_cap_ < (8 << 20) ? 0 : total_cap_ < (128 << 20) ? 1 : 2;
+}
+
+bool Engine::will_tune() const {
+  return shape_tuning_pending() || Tuner::get().exploring(TUNE_SORT_LAUNCH, sort_launch_class());
+}
+
 int Engine::sort_round0(hipStream_t s) {
   launch_bwt_round(d_, grids_, true, late_bits_, shape0_, s, nullptr, nullptr);
   return (int)hipGetLastError();
@@ -451,11 +469,8 @@ int Engine::run_bwt(hipStream_t s, int* rounds_out) {
   const int v0 = tu.pick(TUNE_RADIX_SHAPE, TUNE_CLASS_ROUND0), v1 = tu.pick(TUNE_RADIX_SHAPE, TUNE_CLASS_LATER);
   shape0_ = tu.value(TUNE_RADIX_SHAPE, v0);
   shape1_ = tu.value(TUNE_RADIX_SHAPE, v1);
-  const bool shape_tuning = tu.state(TUNE_RADIX_SHAPE, TUNE_CLASS_ROUND0, nullptr, nullptr) < 0 ||
-                            (late_large_seen_ && tu.state(TUNE_RADIX_SHAPE, TUNE_CLASS_LATER, nullptr, nullptr) < 0);
-  // launch policy per batch size class: launch latency weighs on small
-  // batches, exact grids on large ones
-  const int mode_cls = total_cap_ < (8 << 20) ? 0 : total_cap_ < (128 << 20) ? 1 : 2;
+  const bool shape_tuning = shape_tuning_pending();
+  const int mode_cls = sort_launch_class();
   const int mode = shape_tuning ? 0 : tu.pick(TUNE_SORT_LAUNCH, mode_cls);
   // device-driven later rounds: persistent grids of 1 or 4 residencies
   late_grids_ = grids_;
@@ -481,7 +496,7 @@ int Engine::run_bwt(hipStream_t s, int* rounds_out) {
         const int64_t units = (int64_t)h_ctr_prev_[CTR_LELEMS] * timed_pass;
         if (units > 0 && hipEventElapsedTime(&ms, tev_[timed][0], tev_[timed][timed_pass]) == hipSuccess)
           tu.report(TUNE_RADIX_SHAPE, timed == 0 ? TUNE_CLASS_ROUND0 : TUNE_CLASS_LATER, timed == 0 ? v0 : v1,
-                    (double)ms * 1e6 / (double)units);
+                    (double)ms * 1e6, (double)units);
         timed = -1;
       }
       if (h_ctr_[CTR_ERR]) return (int)hipErrorIllegalState;
@@ -548,7 +563,7 @@ int Engine::run_bwt(hipStream_t s, int* rounds_out) {
   if (!shape_tuning && tu.state(TUNE_SORT_LAUNCH, mode_cls, nullptr, nullptr) < 0 && active_elems_ > 0) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ev_[7], ev_[8]) == hipSuccess)
-      tu.report(TUNE_SORT_LAUNCH, mode_cls, mode, (double)ms * 1e6 / (double)active_elems_);
+      tu.report(TUNE_SORT_LAUNCH, mode_cls, mode, (double)ms * 1e6, (double)active_elems_);
   }
   return 0;
 }

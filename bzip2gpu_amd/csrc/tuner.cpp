@@ -52,20 +52,26 @@ int Tuner::pick(int knob, int cls) {
   return best;
 }
 
-void Tuner::report(int knob, int cls, int variant, double ns_per_unit) {
+void Tuner::report(int knob, int cls, int variant, double ns, double units) {
   std::lock_guard<std::mutex> lk(mu_);
-  if (forced_[knob] >= 0 || variant < 0 || variant >= variants(knob) || !(ns_per_unit > 0)) return;
+  if (forced_[knob] >= 0 || variant < 0 || variant >= variants(knob) || !(ns > 0) || !(units > 0)) return;
   Slot& s = slot_[knob][cls];
   if (s.chosen >= 0) return;
-  s.sum[variant] += ns_per_unit;
+  s.ns[variant] += ns;
+  s.units[variant] += units;
   s.n[variant]++;
   for (int v = 0; v < variants(knob); v++)
     if (s.n[v] < kTuneTrials) return;
   int best = 0;
   for (int v = 1; v < variants(knob); v++)
-    if (s.sum[v] / s.n[v] < s.sum[best] / s.n[best]) best = v;
+    if (s.ns[v] / s.units[v] < s.ns[best] / s.units[best]) best = v;
   s.chosen = best;
   save_locked();
+}
+
+bool Tuner::exploring(int knob, int cls) {
+  std::lock_guard<std::mutex> lk(mu_);
+  return forced_[knob] < 0 && slot_[knob][cls].chosen < 0;
 }
 
 void Tuner::force(int knob, int variant) {
@@ -77,7 +83,7 @@ int Tuner::state(int knob, int cls, double* ns_per_unit, int* samples) {
   std::lock_guard<std::mutex> lk(mu_);
   const Slot& s = slot_[knob][cls];
   for (int v = 0; v < variants(knob); v++) {
-    if (ns_per_unit) ns_per_unit[v] = s.n[v] ? s.sum[v] / s.n[v] : 0.0;
+    if (ns_per_unit) ns_per_unit[v] = s.units[v] > 0 ? s.ns[v] / s.units[v] : 0.0;
     if (samples) samples[v] = s.n[v];
   }
   return forced_[knob] >= 0 ? forced_[knob] : s.chosen;

@@ -3,9 +3,10 @@
 // Some kernels come in several launch shapes that compute identical results
 // at different speeds on different inputs.  The tuner picks a shape per
 // knob and size class: while exploring it hands out every variant in turn,
-// timed by the caller with HIP events (nanoseconds per unit of work), and
-// after kTuneTrials samples of each it keeps the fastest for the rest of the
-// process.  Choices can be forced (tests), inspected (benchmarks) and kept
+// timed by the caller with HIP events (nanoseconds and units of work per
+// sample), and after kTuneTrials samples of each it keeps the one with the
+// least total time per total work (so that small samples, dominated by
+// fixed costs, weigh little) for the rest of the process.  Choices can be forced (tests), inspected (benchmarks) and kept
 // across processes in a text file named by the BZG_TUNE_CACHE environment
 // variable ("knob class variant" per line).
 #pragma once
@@ -32,7 +33,9 @@ class Tuner {
   int variants(int knob) const;
   int value(int knob, int variant) const;  // the launch parameter of a variant
   int pick(int knob, int cls);             // variant to run now
-  void report(int knob, int cls, int variant, double ns_per_unit);
+  void report(int knob, int cls, int variant, double ns, double units);
+  // whether a class of a knob has no choice yet
+  bool exploring(int knob, int cls);
   void force(int knob, int variant);       // -1: automatic
   // chosen variant (-1 while exploring); per-variant mean ns/unit and samples
   int state(int knob, int cls, double* ns_per_unit, int* samples);
@@ -42,7 +45,8 @@ class Tuner {
   Tuner();
   void save_locked();
   struct Slot {
-    double sum[kTuneMaxVariants] = {};
+    double ns[kTuneMaxVariants] = {};
+    double units[kTuneMaxVariants] = {};
     int n[kTuneMaxVariants] = {};
     int chosen = -1;
   };

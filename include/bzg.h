@@ -164,7 +164,8 @@ bzg_status bzg_set_max_batch_bytes(uint64_t bytes);
  * reference): with two or more sub-batches, they alternate between two
  * pipelines on two HIP streams driven by two host threads, and sub-batch
  * p+1's CRC/RLE1 and suffix sort run while sub-batch p's MTF, Huffman,
- * packing and assembly do (the sorts stay in sub-batch order).  Output is
+ * packing and assembly do (the sorts stay in sub-batch order).  Calls that
+ * time autotuner variants run their sub-batches one at a time.  Output is
  * byte-identical either way; the second pipeline holds its own workspace. */
 bzg_status bzg_set_batch_overlap(int32_t on);
 
@@ -203,12 +204,14 @@ int64_t bzg_graph_replays(void);
  * device), chosen per batch size class
  * (0: under 8 MiB of block capacity, 1: under 128 MiB, 2: larger) from the
  * sort stage's time per sorted row, once knob 0 has settled.
- * A choice is made after timing each variant twice and is kept in the file
- * named by BZG_TUNE_CACHE when that variable is set.
+ * A choice is made after timing each variant twice (the least total time
+ * per total work of its samples) and is kept in the file named by
+ * BZG_TUNE_CACHE when that variable is set.
  * bzg_tune_force pins a variant for every class (-1: automatic).
  * bzg_tune_state returns the chosen variant of a class (-1 while exploring,
- * -2 for a bad argument) and fills, per variant, its launch value, mean
- * nanoseconds per row and pass, and sample count. */
+ * -2 for a bad argument) and fills, per variant, its launch value, the
+ * nanoseconds per unit of work over its samples (knob 0: rows x radix
+ * passes; knob 1: sorted rows), and its sample count. */
 bzg_status bzg_tune_force(int32_t knob, int32_t variant);
 int32_t bzg_tune_state(int32_t knob, int32_t size_class, int32_t *h_values,
                        float *h_ns_per_unit, int32_t *h_samples,
