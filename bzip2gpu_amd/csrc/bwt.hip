@@ -37,13 +37,7 @@ namespace bzg {
 // Stable rank of this lane's digit among the items this wave has ranked so
 // far; cnt = this wave's 256 counters in LDS.
 __device__ inline uint32_t wave_rank_digit(uint32_t digit, bool valid, uint32_t* cnt) {
-  uint64_t peers = ballot(valid);
-#pragma unroll
-  for (int b = 0; b < 8; b++) {
-    bool bit = (digit >> b) & 1u;
-    uint64_t m = ballot(bit);
-    peers &= bit ? m : ~m;
-  }
+  const uint64_t peers = wave_match_bits<8>(digit, ballot(valid));
   uint32_t r = (uint32_t)mbcnt(peers);
   uint32_t base = 0;
   if (valid) {

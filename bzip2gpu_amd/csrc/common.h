@@ -87,6 +87,26 @@ __device__ inline int lane_id() { return __lane_id(); }
 
 __device__ inline uint64_t ballot(bool p) { return __ballot(p); }
 
+// The lanes among `lanes` whose low NBITS key bits equal this lane's.  Per
+// bit: nb is all ones where the bit is set and m the lanes where it is set,
+// so peers & ~(m ^ nb) keeps the lanes that agree on it, one three-input
+// logic op per 32-bit half (truth table 0x90 is src0 & ~(src1 ^ src2)).  The
+// empty asm keeps nb opaque, so that the ballot compares it instead of
+// testing the key's bit a second time.
+template <int NBITS>
+__device__ inline uint64_t wave_match_bits(uint32_t key, uint64_t lanes) {
+  uint32_t plo = (uint32_t)lanes, phi = (uint32_t)(lanes >> 32);
+#pragma unroll
+  for (int b = 0; b < NBITS; b++) {
+    uint32_t nb = (uint32_t)__builtin_amdgcn_sbfe((int)key, b, 1);
+    asm volatile("" : "+v"(nb));
+    const uint64_t m = __ballot(nb != 0u);
+    plo = __builtin_amdgcn_bitop3_b32(plo, (uint32_t)m, nb, 0x90);
+    phi = __builtin_amdgcn_bitop3_b32(phi, (uint32_t)(m >> 32), nb, 0x90);
+  }
+  return ((uint64_t)phi << 32) | plo;
+}
+
 // number of set bits of m in lanes below this lane
 __device__ inline int mbcnt(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),

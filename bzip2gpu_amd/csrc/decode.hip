@@ -375,13 +375,7 @@ __global__ void __launch_bounds__(64) k_dec_huff(const uint32_t* __restrict__ W,
         const int i = i0 + lane;
         const bool v = i < alpha;
         const int l = v ? H.lens[t * kMaxAlpha + i] : 0;
-        uint64_t peers = ballot(v);
-#pragma unroll
-        for (int b = 0; b < 5; b++) {
-          const bool on = (l >> b) & 1;
-          const uint64_t bb = ballot(v && on);
-          peers &= on ? bb : ~bb;
-        }
+        const uint64_t peers = wave_match_bits<5>((uint32_t)l, ballot(v));
         const int r = __popcll(peers & lanemask_lt());
         const int before = v ? H.run[l] : 0;
         __syncthreads();
@@ -820,13 +814,7 @@ __global__ void __launch_bounds__(256) k_dec_scatter(const DecBlk* __restrict__ 
     const int e = w * 1024 + r * 64 + lane;
     const bool v = e < n;
     const uint32_t c = v ? lt[e] : 0u;
-    uint64_t peers = ballot(v);
-#pragma unroll
-    for (int bit = 0; bit < 8; bit++) {
-      const bool on = (c >> bit) & 1u;
-      const uint64_t bb = ballot(v && on);
-      peers &= on ? bb : ~bb;
-    }
+    const uint64_t peers = wave_match_bits<8>(c, ballot(v));
     const uint32_t rank = (uint32_t)__popcll(peers & lanemask_lt());
     const uint32_t before = v ? wc[w][c] : 0u;
     __builtin_amdgcn_wave_barrier();
