@@ -153,6 +153,12 @@ __device__ inline T wave_sum(T v) {
   return v;
 }
 template <typename T>
+__device__ inline T wave_xor(T v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v ^= __shfl_xor(v, d, 64);
+  return v;
+}
+template <typename T>
 __device__ inline T wave_max(T v) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
