@@ -354,128 +354,227 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
 
 __device__ inline int ilog2(uint32_t v) { return 31 - __clz(v); }
 
-// RLE2 count / write over capacity tiles.
+// RLE2 count / write over capacity tiles, one wave per tile, four tiles per
+// 256-thread workgroup; lane l holds MTF bytes [64 l, 64 l + 64) of its tile
+// as 16 words.  Byte flags are words with 0x80 in each flagged byte.  A
+// nonzero byte v emits the symbol v + 1; the first zero of a zero run of
+// length L emits the ilog2(L + 1) bits of L + 1 below its leading one, low
+// bit first (0 = RUNA, 1 = RUNB; src/compression/rle2.cu:47-67); other
+// zeros emit nothing.
+constexpr uint32_t kHi8 = 0x80808080u;
+__device__ inline uint32_t nz80b(uint32_t x) { return (((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & kHi8; }
+__device__ inline uint32_t low_flags8(int r) { return r >= 4 ? kHi8 : (r <= 0 ? 0u : kHi8 & ((1u << (8 * r)) - 1u)); }
+
+// Zero-run length of the run starting at byte q of word i: up to the next
+// nonzero byte (NZ: nonzero flags; nx: lane-relative first nonzero after
+// the word).
+__device__ inline int zrun_len(uint32_t NZ, int nx, int i, int q) {
+  const uint32_t after = q < 3 ? NZ >> (8 * q + 8) : 0u;
+  const int e = after ? 4 * i + q + 1 + (__builtin_ctz(after) >> 3) : nx;
+  return e - (4 * i + q);
+}
+
+// A wave stages its tile's symbols (at most one per byte) in LDS with one
+// word of padding after every 64 symbols: lanes stage their own runs of
+// symbols side by side, and without it the lanes' stores at equal offsets
+// in their runs would fall on the same banks.
+constexpr int kRle2Stage = kTile + 2 * (kTile / 64) + 8;
+__device__ inline int spos(int j) { return j + 2 * (j >> 6); }
+
 template <bool WRITE>
-__device__ inline void rle2_tile(const Dev& d) {
-  __shared__ int32_t tmp[8];
-  __shared__ int32_t carry;
-  __shared__ int32_t xe[256];
-  __shared__ int32_t hist[4][260];
-  __shared__ uint16_t ob[WRITE ? 2 * kTile : 1];
-  const int tid = threadIdx.x, wv = tid >> 6;
-  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
+__device__ inline void rle2_wave(const Dev& d, uint16_t* ob, int32_t* hist) {
+  const int tid = threadIdx.x, lane = lane_id();
+  const int t = 4 * (int)xcd_remap(blockIdx.x, gridDim.x) + (tid >> 6);
+  if (t >= d.ntile_c) return;
   const int b = d.tc_blk[t];
   const BlockDesc D = d.desc[b];
   const int m = d.st[b].m;
   const int k = t - D.tile_c0;
   const int start = k * kTile;
   if (start >= m) {
-    if (!WRITE && tid == 0) d.tc_cnt[t] = 0;
+    if (!WRITE && lane == 0) d.tc_cnt[t] = 0;
     return;
   }
-  const int nt = tiles_for(m);
-  if (WRITE)
-    for (int q = tid; q < 4 * 260; q += 256) (&hist[0][0])[q] = 0;
-  // the tile's bytes (and the byte before each thread's) are loaded before
-  // the carry lookup so that the latencies overlap
   const int n = min(kTile, m - start);
-  const uint8_t* src = d.MTF + D.off + start;  // 256-byte aligned
-  const int q0 = tid * 16;
-  const int p0 = start + q0;
-  uint8_t x[16];
-  if (q0 + 16 <= n) {
-    const uint4 v = *reinterpret_cast<const uint4*>(src + q0);
-    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+  const int q0 = lane * 64;
+  const int c = max(0, min(64, n - q0));
+  // the tile's bytes (capacity slots are 256-byte aligned and readable past m)
+  const uint8_t* src = d.MTF + D.off + start;
+  uint32_t w[16];
+  if (c > 0) {
 #pragma unroll
-    for (int q = 0; q < 16; q++) x[q] = (uint8_t)(w[q >> 2] >> (8 * (q & 3)));
+    for (int j = 0; j < 4; j++) {
+      const uint4 v = *reinterpret_cast<const uint4*>(src + q0 + 16 * j);
+      w[4 * j] = v.x; w[4 * j + 1] = v.y; w[4 * j + 2] = v.z; w[4 * j + 3] = v.w;
+    }
   } else {
 #pragma unroll
-    for (int q = 0; q < 16; q++) x[q] = (q0 + q < n) ? src[q0 + q] : 1;
+    for (int i = 0; i < 16; i++) w[i] = 0u;
   }
-  const bool prev_nz = q0 > 0 ? src[q0 - 1] != 0 : (start > 0 ? src[-1] != 0 : true);
+  // first nonzero at or after the tile's end (block-relative), from the
+  // later tiles' first nonzero positions, else m
+  int carry = 0x7fffffff;
+  for (int j = k + 1 + lane; j < tiles_for(m); j += 64) {
+    const int v = d.tc_fnz[D.tile_c0 + j];
+    if (v != 0x7fffffff) { carry = v; break; }
+  }
+  carry = wave_min(carry);
+  if (carry == 0x7fffffff) carry = m;
+  // nonzero flags; the byte before the tile counts as nonzero at the block
+  // start, so that a zero there starts a run
+  uint32_t NZ[16], ZS[16];
+  bool prev_nz;
   {
-    int f = 0x7fffffff;
-    for (int j = k + 1 + tid; j < nt; j += 256) {
-      int v = d.tc_fnz[D.tile_c0 + j];
-      if (v != 0x7fffffff) { f = v; break; }
-    }
-    f = wave_min(f);
-    if (lane_id() == 0) tmp[tid >> 6] = f;
-    __syncthreads();
-    if (tid == 0) {
-      int ff = min(min(tmp[0], tmp[1]), min(tmp[2], tmp[3]));
-      carry = ff == 0x7fffffff ? m : ff;
-    }
-    __syncthreads();
+    const uint32_t last = (uint32_t)__shfl_up((int)(w[15] >> 24), 1, 64);
+    prev_nz = lane > 0 ? last != 0u : (start > 0 ? src[-1] != 0 : true);
   }
-  // nonzero positions (positions past the tile count as nonzero)
-  uint32_t nzm = 0;
+  uint32_t pf = prev_nz ? 0x80000000u : 0u;
 #pragma unroll
-  for (int q = 0; q < 16; q++)
-    if (x[q] != 0) nzm |= 1u << q;
-  const uint32_t valid = q0 >= n ? 0u : (q0 + 16 <= n ? 0xFFFFu : ((1u << (n - q0)) - 1u));
-  const int firstNZ = (nzm & valid) ? p0 + __builtin_ctz(nzm & valid) : 0x7fffffff;
-  xe[tid] = wg_rev_incl_min<256, int>(firstNZ, tmp, carry);
-  __syncthreads();
-  const int es = tid < 255 ? xe[tid + 1] : carry;
-  // symbols per position: nonzero v -> 1 (v+1); the first zero of a run of
-  // length L -> the bits of L+1 below its leading one
-  int e_of[16];
+  for (int i = 0; i < 16; i++) {
+    const uint32_t f = nz80b(w[i]);
+    const uint32_t vm = low_flags8(c - 4 * i);
+    // zero bytes whose previous byte is nonzero start a zero run
+    ZS[i] = ~f & __builtin_amdgcn_alignbyte(f, pf, 3u) & vm;
+    NZ[i] = f & vm;
+    pf = f;
+  }
+  // first nonzero after each word (lane-relative; beyond the lane from the
+  // later lanes, else the carry)
+  int fl = 64;
+#pragma unroll
+  for (int i = 15; i >= 0; i--)
+    if (NZ[i]) fl = 4 * i + (__builtin_ctz(NZ[i]) >> 3);
+  int nxl = __shfl_down(wave_incl_min_rev(fl < 64 ? start + q0 + fl : 0x7fffffff), 1, 64);
+  if (lane == 63) nxl = 0x7fffffff;
+  nxl = min(nxl, carry) - (start + q0);
+  int nx[16];
   {
-    int nb = es;
+    int nxt = nxl;
 #pragma unroll
-    for (int q = 15; q >= 0; q--) {
-      e_of[q] = nb;
-      if ((valid >> q) & (nzm >> q) & 1u) nb = p0 + q;
+    for (int i = 15; i >= 0; i--) {
+      nx[i] = nxt;
+      if (NZ[i]) nxt = 4 * i + (__builtin_ctz(NZ[i]) >> 3);
     }
   }
-  int cs[16];
+  // symbols of the lane (a word holds at most two zero-run starts)
   int cnt = 0;
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
-    int c = 0;
-    if ((valid >> q) & 1u) {
-      const bool pnz = q == 0 ? prev_nz : ((nzm >> (q - 1)) & 1u);
-      if ((nzm >> q) & 1u) c = 1;
-      else if (pnz || p0 + q == 0) c = ilog2((uint32_t)(e_of[q] - (p0 + q)) + 1);
-    }
-    cs[q] = c;
-    cnt += c;
-  }
-  int total;
-  int o = wg_excl_sum<256, int>(cnt, tmp, &total);
-  if (!WRITE) {
-    if (tid == 0) d.tc_cnt[t] = total;
-    return;
-  }
-  // symbols staged in LDS in order, then copied with coalesced stores
+  for (int i = 0; i < 16; i++) {
+    cnt += __popc(NZ[i]);
+    uint32_t z = ZS[i];
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
-    if (cs[q]) {
-      if ((nzm >> q) & 1u) {
-        const uint16_t v = (uint16_t)(x[q] + 1);
-        ob[o++] = v;
-        atomicAdd(&hist[wv][v], 1);
-      } else {
-        const uint32_t kk = (uint32_t)(e_of[q] - (p0 + q)) + 1;
-        const int n1 = __popc(kk & ((1u << cs[q]) - 1u));
-        for (int r = 0; r < cs[q]; r++) ob[o++] = (uint16_t)((kk >> r) & 1u);
-        atomicAdd(&hist[wv][1], n1);
-        atomicAdd(&hist[wv][0], cs[q] - n1);
+    for (int r = 0; r < 2; r++) {
+      if (z) {
+        const int q = __builtin_ctz(z) >> 3;
+        cnt += ilog2((uint32_t)zrun_len(NZ[i], nx[i], i, q) + 1u);
+        z &= z - 1u;
       }
     }
   }
-  __syncthreads();
+  if (!WRITE) {
+    cnt = wave_sum(cnt);
+    if (lane == 0) d.tc_cnt[t] = cnt;
+    return;
+  }
+  const int incl = wave_incl_sum(cnt);
+  int o = incl - cnt;
+  const int total = __builtin_amdgcn_readlane(incl, 63);
+  int n1 = 0, n0 = 0;  // RUNB / RUNA symbols of the lane
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const uint32_t x = w[i], nzf = NZ[i];
+    // the word's zero-run starts (at most two): byte, L + 1, symbol count
+    int qa = 4, qb = 4, ca = 0, cb = 0;
+    uint32_t ka = 0u, kb = 0u;
+    uint32_t z = ZS[i];
+    if (z) {
+      qa = __builtin_ctz(z) >> 3;
+      ka = (uint32_t)zrun_len(nzf, nx[i], i, qa) + 1u;
+      ca = ilog2(ka);
+      z &= z - 1u;
+    }
+    if (z) {
+      qb = __builtin_ctz(z) >> 3;
+      kb = (uint32_t)zrun_len(nzf, nx[i], i, qb) + 1u;
+      cb = ilog2(kb);
+    }
+    // output order within the word: byte by byte, a nonzero byte's symbol
+    // or a zero-run start's bits (start a precedes start b)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if ((nzf >> (8 * q + 7)) & 1u) {
+        const uint32_t v = ((x >> (8 * q)) & 255u) + 1u;
+        const int at = o + __popc(nzf & low_flags8(q)) + (qa < q ? ca : 0) + (qb < q ? cb : 0);
+        ob[spos(at)] = (uint16_t)v;
+        atomicAdd(&hist[v], 1);
+      }
+    }
+    // the run bits of both starts in one loop
+    {
+      const int pa = o + __popc(nzf & low_flags8(qa));
+      const int pb = o + __popc(nzf & low_flags8(qb)) + ca;
+#pragma unroll 1
+      for (int r = 0; r < ca + cb; r++) {
+        const bool in_a = r < ca;
+        const int at = in_a ? pa + r : pb + (r - ca);
+        const uint32_t bit = in_a ? (ka >> r) & 1u : (kb >> (r - ca)) & 1u;
+        ob[spos(at)] = (uint16_t)bit;
+      }
+    }
+    o += __popc(nzf) + ca + cb;
+    asm volatile("" : "+v"(o));  // one running offset (not a sum over earlier words at each use)
+    const int ones = (ca ? __popc(ka) - 1 : 0) + (cb ? __popc(kb) - 1 : 0);
+    n1 += ones;
+    n0 += ca + cb - ones;
+  }
+  n1 = wave_sum(n1);
+  n0 = wave_sum(n0);
+  if (lane == 0) {
+    if (n0) atomicAdd(&hist[0], n0);
+    if (n1) atomicAdd(&hist[1], n1);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // copy-out as 32-bit words of two symbols (single symbols at the edges)
   uint16_t* dst = d.SYM + D.off + d.tc_off[t];
-  for (int i = tid; i < total; i += 256) dst[i] = ob[i];
-  for (int q = tid; q < 260; q += 256) {
-    const int h = hist[0][q] + hist[1][q] + hist[2][q] + hist[3][q];
+  const int lead = (int)((reinterpret_cast<uintptr_t>(dst) >> 1) & 1u);
+  uint32_t* A0 = reinterpret_cast<uint32_t*>(reinterpret_cast<uintptr_t>(dst) & ~(uintptr_t)3);
+  const int nw = (lead + total + 1) >> 1;
+  const uint32_t* o32 = reinterpret_cast<const uint32_t*>(ob);
+  for (int wi = lane; wi < nw; wi += 64) {
+    const int oo = 2 * wi - lead;  // symbol index of the word's low half
+    if (oo >= 0 && oo + 2 <= total) {
+      // symbols oo, oo + 1: one staged word, or the halves of two (odd oo)
+      const uint32_t v = lead ? __builtin_amdgcn_alignbyte(o32[spos(oo + 1) >> 1], o32[spos(oo - 1) >> 1], 2u)
+                              : o32[spos(oo) >> 1];
+      A0[wi] = v;
+    } else {
+      if (oo >= 0 && oo < total) dst[oo] = ob[spos(oo)];
+      if (oo + 1 >= 0 && oo + 1 < total) dst[oo + 1] = ob[spos(oo + 1)];
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int q = lane; q < 260; q += 64) {
+    const int h = hist[q];
     if (h) atomicAdd(&d.freq[(int64_t)b * 260 + q], h);
   }
 }
 
-__global__ void __launch_bounds__(256) k_rle2_count(Dev d) { rle2_tile<false>(d); }
-__global__ void __launch_bounds__(256) k_rle2_write(Dev d) { rle2_tile<true>(d); }
+__global__ void __launch_bounds__(256) k_rle2_count(Dev d) { rle2_wave<false>(d, nullptr, nullptr); }
+
+__global__ void __launch_bounds__(256) k_rle2_write(Dev d) {
+  __shared__ __align__(16) uint16_t ob[4][kRle2Stage];
+  __shared__ int32_t hist[4][260];
+  const int wv = threadIdx.x >> 6;
+  for (int q = lane_id(); q < 260; q += 64) hist[wv][q] = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  rle2_wave<true>(d, ob[wv], hist[wv]);
+}
 
 void launch_mtf(const Dev& d, hipStream_t s) {
   k_mtf_gather<<<d.ntile_c, 256, 0, s>>>(d);
@@ -486,9 +585,10 @@ void launch_mtf(const Dev& d, hipStream_t s) {
 }
 
 void launch_rle2(const Dev& d, hipStream_t s) {
-  k_rle2_count<<<d.ntile_c, 256, 0, s>>>(d);
+  const int wgs = (d.ntile_c + 3) / 4;  // four tiles per workgroup, one per wave
+  k_rle2_count<<<wgs, 256, 0, s>>>(d);
   launch_tile_scan_c(d, s);
-  k_rle2_write<<<d.ntile_c, 256, 0, s>>>(d);
+  k_rle2_write<<<wgs, 256, 0, s>>>(d);
 }
 
 void launch_mtf_rle2(const Dev& d, hipStream_t s) {
