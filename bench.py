@@ -364,6 +364,7 @@ def run_serve(a):
         st = _serve_stats(arr, done, sz)
         st.update({"lambda": lam, "mean_batch": round(float(bs.mean()), 2), "max_batch": int(bs.max())})
         per.append(st)
+        print(f"[serve] lambda {lam}/s done: {st}", file=sys.stderr, flush=True)
     srv.close()
     # plain path at the lowest rate, shorter window (it cannot keep up)
     psrv = bz.Server(mode="plain", workers=1)
