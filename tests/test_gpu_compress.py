@@ -92,6 +92,13 @@ def test_stock_equal_blocks(dev):
     data = rng.integers(0, 256, 899000, dtype=np.uint8).tobytes()
     _case("stock-equal random block")
     _same(bz.compress(data, 9), bz2.compress(data, 9), "stock-equal")
+    # a block whose last-iteration tables need libbz2's weight halving
+    # (bytes with p(b) proportional to 0.7^b; tests/test_oracle.py checks the
+    # unlimited tree depth of this input is 18)
+    p = 0.7 ** np.arange(256)
+    data = np.random.default_rng(4).choice(256, size=880000, p=p / p.sum()).astype(np.uint8).tobytes()
+    _case("stock-equal block with weight halving")
+    _same(bz.compress(data, 9), bz2.compress(data, 9), "stock-equal, halving")
 
 
 def test_batch_device_many_inputs(dev):

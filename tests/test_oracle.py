@@ -247,6 +247,38 @@ def test_libbz2_byte_equality_random_and_runs():
     assert _stock_equal_case(runs[:99000], 1)
 
 
+def _deepest_unlimited_table(data: bytes) -> int:
+    """Depth of the deepest Huffman tree of the block's last iteration if
+    there were no 17-bit limit: the tables' symbol counts from the final
+    selectors, built with a limit no tree reaches."""
+    r1 = np.frombuffer(oracle.rle1(data), np.uint8)
+    sa, _ = oracle.bwt(r1)
+    mtfb, _, nuse = oracle.mtf(r1.tobytes(), sa)[:3]
+    sym = oracle.rle2(mtfb, nuse)
+    _, _, sel, ng = oracle.huff_build(sym, nuse)
+    alpha = nuse + 2
+    deepest = 0
+    for t in range(ng):
+        f = np.zeros(alpha, np.int32)
+        for g, tb in enumerate(sel):
+            if tb == t:
+                np.add.at(f, sym[g * 50:(g + 1) * 50], 1)
+        deepest = max(deepest, int(oracle.make_code_lengths(f, alpha, 255).max()))
+    return deepest
+
+
+def test_libbz2_byte_equality_with_weight_halving():
+    # i.i.d. bytes with p(b) proportional to 0.7^b: one of the last
+    # iteration's tables would be 18 levels deep, so libbz2 halves the weights
+    # and rebuilds it (huffman.cu:110-123); the stream still equals stock
+    # libbz2's, which pins that branch of the oracle
+    rng = np.random.default_rng(4)
+    p = 0.7 ** np.arange(256)
+    data = rng.choice(256, size=880000, p=p / p.sum()).astype(np.uint8).tobytes()
+    assert _deepest_unlimited_table(data) > 17
+    assert _stock_equal_case(data, 9)
+
+
 def test_multiblock_roundtrip_and_combined_crc():
     import bzip2gpu_amd.datagen as dg
     from tests import bz2dec
