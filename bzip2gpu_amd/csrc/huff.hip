@@ -395,10 +395,13 @@ __device__ void build_lengths(HeapLds& H, int32_t* W, uint8_t* len, int alpha, i
 }  // namespace
 
 // The four selection / code-length iterations of one block in one
-// workgroup of eight waves: all threads assign 50-symbol groups (costs from
+// workgroup of six waves: all threads assign 50-symbol groups (costs from
 // 64-bit packed lengths, first strict minimum, counts in LDS), then wave t
-// rebuilds table t's lengths.  A block's tables never wait for another
-// block's.
+// rebuilds table t's lengths if the set of groups that chose table t
+// changed (else its counts, and so its lengths, are the previous
+// iteration's).  An iteration in which no group changed its table is a
+// fixed point: the later ones would repeat it, and they are skipped.  A
+// block's tables never wait for another block's.
 // Launch order of the code-length builds: blocks by descending alphabet
 // size (the heap work grows with it).  More blocks than CUs means some CUs
 // run two builds at once; the builds launched first are paired with those
@@ -428,6 +431,7 @@ constexpr int kHuffThreads = 384;
 __global__ void __launch_bounds__(kHuffThreads) __attribute__((amdgpu_waves_per_eu(8))) k_huff_build(Dev d) {
   __shared__ int32_t rf[6][260];
   __shared__ HeapLds H[6];
+  __shared__ int32_t s_chg;  // tables whose set of groups changed this iteration
   // the six lengths of a symbol packed in one word, used while the groups
   // are assigned; it shares table 0's heap, which is idle then
   static_assert(sizeof(H[0].heap) >= 260 * sizeof(uint64_t), "packed lengths fit table 0's heap");
@@ -450,6 +454,7 @@ __global__ void __launch_bounds__(kHuffThreads) __attribute__((amdgpu_waves_per_
       PL[v] = w;
     }
     for (int q = tid; q < 6 * 260; q += kHuffThreads) (&rf[0][0])[q] = 0;
+    if (tid == 0) s_chg = it == 0 ? 0x3f : 0;
     __syncthreads();
     // every group but a block's last holds kGSize symbols: that path runs
     // without per-symbol guards
@@ -477,7 +482,14 @@ __global__ void __launch_bounds__(kHuffThreads) __attribute__((amdgpu_waves_per_
         const uint32_t c = (uint32_t)(acc >> (10 * tt)) & 1023u;
         if (c < bc) { bc = c; bt = tt; }
       }
-      if (it == 3) d.sel[D.sel_off + g] = (uint8_t)bt;
+      // a table whose groups are those of the previous iteration gets the
+      // same counts, hence the same lengths: its build is skipped
+      uint8_t* sp = d.sel + D.sel_off + g;
+      if (it > 0) {
+        const int old = *sp;
+        if (old != bt) atomicOr(&s_chg, (1 << old) | (1 << bt));
+      }
+      *sp = (uint8_t)bt;
       int32_t* rt = rf[bt];
       if (full) {
 #pragma unroll
@@ -489,7 +501,10 @@ __global__ void __launch_bounds__(kHuffThreads) __attribute__((amdgpu_waves_per_
       }
     }
     __syncthreads();
-    if (wv < ng) build_lengths(H[wv], rf[wv], glen + wv * 260, alpha, lane);
+    const int chg = s_chg;
+    // no table changed: every later iteration repeats this one
+    if (chg == 0) break;
+    if (wv < ng && ((chg >> wv) & 1)) build_lengths(H[wv], rf[wv], glen + wv * 260, alpha, lane);
     __syncthreads();
   }
 }
