@@ -194,6 +194,12 @@ __device__ inline void wave_sync() {
 // positions 0..K-1 by last occurrence, and every other present byte moves
 // from position p to K + p - rank[p], rank[p] = #(step bytes whose old
 // position is below p).
+//
+// A wave takes kMtfRun consecutive capacity tiles: a segment that follows
+// the previous one of the same block starts from the list state that one
+// ended with, so the state entering it is computed only for a run's first
+// segment of each block (256 x 256 compares).
+constexpr int kMtfRun = 4;
 __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
   __shared__ int32_t E[256];
   __shared__ __align__(16) uint8_t buf[kTile];
@@ -203,153 +209,164 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
   __shared__ __align__(4) uint8_t rank[256];
   __shared__ __align__(4) uint8_t newpos[256];
   __shared__ unsigned long long pm[256];  // per-byte lane masks of the current step
-  const int t = (int)xcd_remap(blockIdx.x, gridDim.x);
-  const int b = d.tc_blk[t];
-  const BlockDesc D = d.desc[b];
-  const int m = d.st[b].m;
-  const int k = t - D.tile_c0;
-  const int start = k * kTile;
   const int lane = lane_id();
-  if (start >= m) return;
-  const int n = min(kTile, m - start);
-  uint8_t* mtf = d.MTF + D.off + start;  // 256-byte aligned segment
-  for (int o = lane * 16; o < n; o += 1024) {
-    if (o + 16 <= n) {
-      *reinterpret_cast<uint4*>(buf + o) = *reinterpret_cast<const uint4*>(mtf + o);
-    } else {
-      for (int q = o; q < n; q++) buf[q] = mtf[q];
-    }
-  }
-  for (int q = lane; q < 256; q += 64) {
-    E[q] = d.lo[(int64_t)t * 256 + q];
-    pflag[q] = 0;
-    sflag[q] = 0;
-    pm[q] = 0ull;
-  }
-  wave_sync();
-  // list position of every present byte: # present bytes more recent
-  {
-    int e4[4], p4[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int r = 0; r < 4; r++) e4[r] = E[4 * lane + r];
-    for (int r = 0; r < 256; r++) {
-      const int er = E[r];
-#pragma unroll
-      for (int q = 0; q < 4; q++) p4[q] += er > e4[q] ? 1 : 0;
-    }
-    reinterpret_cast<uint32_t*>(s2p)[lane] =
-        (uint32_t)(p4[0] & 255) | ((uint32_t)(p4[1] & 255) << 8) | ((uint32_t)(p4[2] & 255) << 16) |
-        ((uint32_t)(p4[3] & 255) << 24);
-  }
-  wave_sync();
-  int fnz = 0x7fffffff;
   const uint64_t below = lanemask_lt();
-  for (int base = 0; base < n; base += 64) {
-    const int i = base + lane;
-    const bool valid = i < n;
-    const uint32_t c = valid ? buf[i] : 0u;
-    // lanes holding the same byte: each lane ORs its bit into its byte's
-    // LDS mask and reads the mask back (cleared again below)
-    if (valid) atomicOr(&pm[c], 1ull << lane);
-    wave_sync();
-    const uint64_t peers = valid ? pm[c] : 0ull;
-    const uint64_t pb = peers & below;
-    const bool has_prev = valid && pb != 0;
-    const int prev = has_prev ? 63 - __clzll(pb) : -1;
-    const uint64_t above = valid ? (peers & ~below & ~(1ull << lane)) : 0ull;
-    const int nxt = above ? __builtin_ctzll(above) : 64;
-    const int mypos = valid ? (int)s2p[c] : 0;
-    const bool first = valid && !has_prev;
-    // Repeated byte: distinct bytes strictly between its previous lane and
-    // this lane = lanes j in (prev, lane) whose next occurrence is >= lane.
-    // Row j of a 64x64 bit matrix marks the lanes i with j < i <= nxt_j;
-    // its transpose gives, for lane i, the set of such j.
-    uint64_t row = 0;
-    if (valid) {
-      uint64_t upto = nxt >= 63 ? ~0ull : ((2ull << nxt) - 1ull);
-      row = upto & ~((2ull << lane) - 1ull);
-      if (lane == 63) row = 0;
+  const int w = (int)xcd_remap(blockIdx.x, gridDim.x);
+  int held = -1;  // capacity tile whose exit list state s2p holds
+  for (int j = 0; j < kMtfRun; j++) {
+    const int t = w * kMtfRun + j;
+    if (t >= d.ntile_c) break;
+    const int b = d.tc_blk[t];
+    const BlockDesc D = d.desc[b];
+    const int m = d.st[b].m;
+    const int k = t - D.tile_c0;
+    const int start = k * kTile;
+    if (start >= m) continue;
+    const int n = min(kTile, m - start);
+    uint8_t* mtf = d.MTF + D.off + start;  // 256-byte aligned segment
+    wave_sync();  // the previous segment's copy-out has read buf
+    for (int o = lane * 16; o < n; o += 1024) {
+      if (o + 16 <= n) {
+        *reinterpret_cast<uint4*>(buf + o) = *reinterpret_cast<const uint4*>(mtf + o);
+      } else {
+        for (int q = o; q < n; q++) buf[q] = mtf[q];
+      }
     }
-    const uint64_t col = bit_transpose64(row);
-    // First occurrence: earlier first occurrences whose old position is
-    // behind this one, by a bit-serial compare of the positions (MSB first).
-    const uint64_t fm = ballot(first);
-    const int K = __popcll(fm);
-    // gt: earlier first lanes with a larger position, eq: those equal so far
-    // (MSB first); nm is all ones where this lane's bit is clear
-    uint32_t gl = 0, gh = 0;
-    uint32_t el = (uint32_t)(fm & below), eh = (uint32_t)((fm & below) >> 32);
-#pragma unroll
-    for (int bb = 7; bb >= 0; bb--) {
-      const uint32_t nm = (uint32_t)__builtin_amdgcn_sbfe(~mypos, bb, 1);
-      const uint64_t B = ballot(nm == 0) & fm;
-      const uint32_t bl = (uint32_t)B, bh = (uint32_t)(B >> 32);
-      gl |= el & bl & nm;
-      gh |= eh & bh & nm;
-      el &= bl ^ nm;
-      eh &= bh ^ nm;
-    }
-    int out;
-    if (has_prev) {
-      const uint64_t between = below & ~((2ull << prev) - 1ull);
-      out = __popcll(col & between);
-    } else {
-      out = mypos + __popc(gl) + __popc(gh);
-    }
-    if (valid) buf[i] = (uint8_t)out;
-    const uint64_t nz = ballot(valid && out != 0);
-    if (nz && fnz == 0x7fffffff) fnz = start + base + __builtin_ctzll(nz);
-    // list update
-    const bool last = valid && nxt == 64;
-    const uint64_t lastm = ballot(last);
-    if (first) {
-      pflag[mypos] = 1;
-      sflag[c] = 1;
-    }
-    if (last) newpos[c] = (uint8_t)(__popcll(lastm >> lane) - 1);
-    if (valid) pm[c] = 0ull;  // every lane has read its mask (in program order)
-    wave_sync();
-    {
-      // flags below this lane's four positions, from one ballot per byte slot
-      const uint32_t f4 = reinterpret_cast<const uint32_t*>(pflag)[lane];
-      const int ex = mbcnt(ballot(f4 & 1u)) + mbcnt(ballot((f4 >> 8) & 1u)) + mbcnt(ballot((f4 >> 16) & 1u)) +
-                     mbcnt(ballot((f4 >> 24) != 0u));
-      const int r1 = ex + (int)(f4 & 1u), r2 = r1 + (int)((f4 >> 8) & 1u), r3 = r2 + (int)((f4 >> 16) & 1u);
-      reinterpret_cast<uint32_t*>(rank)[lane] =
-          (uint32_t)ex | ((uint32_t)r1 << 8) | ((uint32_t)r2 << 16) | ((uint32_t)r3 << 24);
-    }
-    wave_sync();
-    {
-      const uint32_t sp4 = reinterpret_cast<const uint32_t*>(s2p)[lane];
-      const uint32_t sf4 = reinterpret_cast<const uint32_t*>(sflag)[lane];
-      const uint32_t nw4 = reinterpret_cast<const uint32_t*>(newpos)[lane];
-      int rk[4];
-#pragma unroll
-      for (int r = 0; r < 4; r++) rk[r] = rank[(sp4 >> (8 * r)) & 255u];
-      uint32_t np4 = 0;
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int op = (int)((sp4 >> (8 * r)) & 255u);
-        const int np = ((sf4 >> (8 * r)) & 1u) ? (int)((nw4 >> (8 * r)) & 255u) : K + op - rk[r];
-        np4 |= (uint32_t)(np & 255) << (8 * r);
+    // the previous segment of the same block left the list state this one
+    // enters with (and the step flags cleared)
+    if (held != t - 1 || k == 0) {
+      for (int q = lane; q < 256; q += 64) {
+        E[q] = d.lo[(int64_t)t * 256 + q];
+        pflag[q] = 0;
+        sflag[q] = 0;
+        pm[q] = 0ull;
       }
       wave_sync();
-      reinterpret_cast<uint32_t*>(s2p)[lane] = np4;
-      if (first) {
-        pflag[mypos] = 0;
-        sflag[c] = 0;
+      // list position of every present byte: # present bytes more recent
+      {
+        int e4[4], p4[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < 4; r++) e4[r] = E[4 * lane + r];
+        for (int r = 0; r < 256; r++) {
+          const int er = E[r];
+#pragma unroll
+          for (int q = 0; q < 4; q++) p4[q] += er > e4[q] ? 1 : 0;
+        }
+        reinterpret_cast<uint32_t*>(s2p)[lane] =
+            (uint32_t)(p4[0] & 255) | ((uint32_t)(p4[1] & 255) << 8) | ((uint32_t)(p4[2] & 255) << 16) |
+            ((uint32_t)(p4[3] & 255) << 24);
       }
     }
     wave_sync();
-  }
-  for (int o = lane * 16; o < n; o += 1024) {
-    if (o + 16 <= n) {
-      *reinterpret_cast<uint4*>(mtf + o) = *reinterpret_cast<const uint4*>(buf + o);
-    } else {
-      for (int q = o; q < n; q++) mtf[q] = buf[q];
+    int fnz = 0x7fffffff;
+    for (int base = 0; base < n; base += 64) {
+      const int i = base + lane;
+      const bool valid = i < n;
+      const uint32_t c = valid ? buf[i] : 0u;
+      // lanes holding the same byte: each lane ORs its bit into its byte's
+      // LDS mask and reads the mask back (cleared again below)
+      if (valid) atomicOr(&pm[c], 1ull << lane);
+      wave_sync();
+      const uint64_t peers = valid ? pm[c] : 0ull;
+      const uint64_t pb = peers & below;
+      const bool has_prev = valid && pb != 0;
+      const int prev = has_prev ? 63 - __clzll(pb) : -1;
+      const uint64_t above = valid ? (peers & ~below & ~(1ull << lane)) : 0ull;
+      const int nxt = above ? __builtin_ctzll(above) : 64;
+      const int mypos = valid ? (int)s2p[c] : 0;
+      const bool first = valid && !has_prev;
+      // Repeated byte: distinct bytes strictly between its previous lane and
+      // this lane = lanes j in (prev, lane) whose next occurrence is >= lane.
+      // Row j of a 64x64 bit matrix marks the lanes i with j < i <= nxt_j;
+      // its transpose gives, for lane i, the set of such j.
+      uint64_t row = 0;
+      if (valid) {
+        uint64_t upto = nxt >= 63 ? ~0ull : ((2ull << nxt) - 1ull);
+        row = upto & ~((2ull << lane) - 1ull);
+        if (lane == 63) row = 0;
+      }
+      const uint64_t col = bit_transpose64(row);
+      // First occurrence: earlier first occurrences whose old position is
+      // behind this one, by a bit-serial compare of the positions (MSB first).
+      const uint64_t fm = ballot(first);
+      const int K = __popcll(fm);
+      // gt: earlier first lanes with a larger position, eq: those equal so far
+      // (MSB first); nm is all ones where this lane's bit is clear
+      uint32_t gl = 0, gh = 0;
+      uint32_t el = (uint32_t)(fm & below), eh = (uint32_t)((fm & below) >> 32);
+#pragma unroll
+      for (int bb = 7; bb >= 0; bb--) {
+        const uint32_t nm = (uint32_t)__builtin_amdgcn_sbfe(~mypos, bb, 1);
+        const uint64_t B = ballot(nm == 0) & fm;
+        const uint32_t bl = (uint32_t)B, bh = (uint32_t)(B >> 32);
+        gl |= el & bl & nm;
+        gh |= eh & bh & nm;
+        el &= bl ^ nm;
+        eh &= bh ^ nm;
+      }
+      int out;
+      if (has_prev) {
+        const uint64_t between = below & ~((2ull << prev) - 1ull);
+        out = __popcll(col & between);
+      } else {
+        out = mypos + __popc(gl) + __popc(gh);
+      }
+      if (valid) buf[i] = (uint8_t)out;
+      const uint64_t nz = ballot(valid && out != 0);
+      if (nz && fnz == 0x7fffffff) fnz = start + base + __builtin_ctzll(nz);
+      // list update
+      const bool last = valid && nxt == 64;
+      const uint64_t lastm = ballot(last);
+      if (first) {
+        pflag[mypos] = 1;
+        sflag[c] = 1;
+      }
+      if (last) newpos[c] = (uint8_t)(__popcll(lastm >> lane) - 1);
+      if (valid) pm[c] = 0ull;  // every lane has read its mask (in program order)
+      wave_sync();
+      {
+        // flags below this lane's four positions, from one ballot per byte slot
+        const uint32_t f4 = reinterpret_cast<const uint32_t*>(pflag)[lane];
+        const int ex = mbcnt(ballot(f4 & 1u)) + mbcnt(ballot((f4 >> 8) & 1u)) + mbcnt(ballot((f4 >> 16) & 1u)) +
+                       mbcnt(ballot((f4 >> 24) != 0u));
+        const int r1 = ex + (int)(f4 & 1u), r2 = r1 + (int)((f4 >> 8) & 1u), r3 = r2 + (int)((f4 >> 16) & 1u);
+        reinterpret_cast<uint32_t*>(rank)[lane] =
+            (uint32_t)ex | ((uint32_t)r1 << 8) | ((uint32_t)r2 << 16) | ((uint32_t)r3 << 24);
+      }
+      wave_sync();
+      {
+        const uint32_t sp4 = reinterpret_cast<const uint32_t*>(s2p)[lane];
+        const uint32_t sf4 = reinterpret_cast<const uint32_t*>(sflag)[lane];
+        const uint32_t nw4 = reinterpret_cast<const uint32_t*>(newpos)[lane];
+        int rk[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) rk[r] = rank[(sp4 >> (8 * r)) & 255u];
+        uint32_t np4 = 0;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int op = (int)((sp4 >> (8 * r)) & 255u);
+          const int np = ((sf4 >> (8 * r)) & 1u) ? (int)((nw4 >> (8 * r)) & 255u) : K + op - rk[r];
+          np4 |= (uint32_t)(np & 255) << (8 * r);
+        }
+        wave_sync();
+        reinterpret_cast<uint32_t*>(s2p)[lane] = np4;
+        if (first) {
+          pflag[mypos] = 0;
+          sflag[c] = 0;
+        }
+      }
+      wave_sync();
     }
+    for (int o = lane * 16; o < n; o += 1024) {
+      if (o + 16 <= n) {
+        *reinterpret_cast<uint4*>(mtf + o) = *reinterpret_cast<const uint4*>(buf + o);
+      } else {
+        for (int q = o; q < n; q++) mtf[q] = buf[q];
+      }
+    }
+    if (lane == 0) d.tc_fnz[t] = fnz;
+    held = t;
   }
-  if (lane == 0) d.tc_fnz[t] = fnz;
 }
 
 __device__ inline int ilog2(uint32_t v) { return 31 - __clz(v); }
@@ -580,7 +597,7 @@ void launch_mtf(const Dev& d, hipStream_t s) {
   k_mtf_gather<<<d.ntile_c, 256, 0, s>>>(d);
   k_mtf_scan<<<d.nblk, 256, 0, s>>>(d);
   probe_begin("k_mtf_apply", s);
-  k_mtf_apply<<<d.ntile_c, 64, 0, s>>>(d);
+  k_mtf_apply<<<(d.ntile_c + kMtfRun - 1) / kMtfRun, 64, 0, s>>>(d);
   probe_end("k_mtf_apply", s);
 }
 
