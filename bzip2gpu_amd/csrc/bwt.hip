@@ -506,8 +506,12 @@ __device__ inline void emit_groups(const Dev& d, int nxt, bool has, Seg g, int32
 constexpr int kCountMax = 64;
 // One workgroup sorts a pack of <= kPack rows in LDS.  KT holds the
 // composite key (group in pack << keybits | key): 32 bits whenever they fit.
+// Every row of a group already has the group's first row as its rank (from
+// the round that formed the group), and the rows of its first subgroup keep
+// it: after round 0 (which ranks every row first) only the other rows'
+// ranks are written (`keep_first`).
 template <typename KT>
-__device__ inline void sort_small_body(const Dev& d, int cur, int keybits, int k) {
+__device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, int keybits, int k) {
   const int tid_ = loop_tid();
   __shared__ PackLds L;
   __shared__ KT K0[kPack], K1[kPack];
@@ -611,7 +615,7 @@ __device__ inline void sort_small_body(const Dev& d, int cur, int keybits, int k
       const int st0 = L.start[sg];
       const uint32_t v = V0[e];
       d.SA[coff + st0 + lt + eqb] = v;
-      d.RK[coff + (v & kSAMask)] = (uint32_t)(st0 + lt);
+      if (lt != 0 || !keep_first) d.RK[coff + (v & kSAMask)] = (uint32_t)(st0 + lt);
       V1[e] = (eq >= 2 && eqb == 0) ? (((uint32_t)lt << 16) | (uint32_t)eq) : 0u;
     }
     __syncthreads();
@@ -729,7 +733,7 @@ __device__ inline void sort_small_body(const Dev& d, int cur, int keybits, int k
       int hj = L.start[sg] + (head_of[q] - L.lofs[sg]);
       uint32_t v = Vs[e];
       d.SA[off + j] = v;
-      d.RK[off + (v & kSAMask)] = (uint32_t)hj;
+      if (head_of[q] != L.lofs[sg] || !keep_first) d.RK[off + (v & kSAMask)] = (uint32_t)hj;
       // a group of >= 2 rows ends here
       if (((e == total - 1) || (kk[q + 1] != kk[q])) && e > head_of[q]) {
         endm |= 1u << q;
@@ -758,10 +762,10 @@ __device__ inline void sort_small_body(const Dev& d, int cur, int keybits, int k
 
 template <typename KT>
 __device__ inline void sort_small_loop(const Dev& d, int keybits) {
-  const int cur = ctr_get(d, CTR_ROUND) & 1;
+  const int round = ctr_get(d, CTR_ROUND), cur = round & 1;
   for (ItemLoop it(round_packs(d)); it.i < it.end; it.i += it.step) {
     __syncthreads();  // the previous pack's LDS is no longer read
-    sort_small_body<KT>(d, cur, keybits, it.i);
+    sort_small_body<KT>(d, cur, round > 0, keybits, it.i);
   }
 }
 
