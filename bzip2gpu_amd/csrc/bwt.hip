@@ -1068,13 +1068,18 @@ __global__ void __launch_bounds__(256) k_lsum(Dev d, int fin) {
   }
 }
 
+// A workgroup of NT threads per tile; thread t owns rows [RPT t, RPT t + RPT).
+template <int NT>
 __device__ inline void lheads_item(const Dev& d, int cur, int fin, int t) {
+  constexpr int RPT = kTile / NT;     // rows per thread
+  constexpr int KL = kTile / NT + 1;  // key loads per thread (rows -1 .. kTile)
+  constexpr int NW = NT / 64;
   const int tid_ = loop_tid();
-  __shared__ int32_t tmp[8];
+  __shared__ int32_t tmp[NW + 1];
   __shared__ int32_t s_carry, s_base;
-  __shared__ int32_t xs[256];
-  __shared__ uint32_t sk[kTile + 2 + (kTile + 2) / 16 + 1];
-  __shared__ uint32_t ss[kTile + kTile / 16];
+  __shared__ int32_t xs[NT];
+  __shared__ uint32_t sk[kTile + 2 + (kTile + 2) / RPT + 1];
+  __shared__ uint32_t ss[kTile + kTile / RPT];
   // per-bucket counters reuse the key staging area once the keys are read
   int32_t* bcnt = reinterpret_cast<int32_t*>(sk);
   static_assert((kStageMax + kRkBucket - 1) / kRkBucket <= kTile, "bucket counters fit the key area");
@@ -1093,80 +1098,83 @@ __device__ inline void lheads_item(const Dev& d, int cur, int fin, int t) {
   // carry: last boundary before this tile (largest lb of earlier tiles)
   {
     int lbv = -1;
-    for (int j = kt - 1 - tid; j >= 0; j -= 256) {
+    for (int j = kt - 1 - tid; j >= 0; j -= NT) {
       int v = d.lsum[2 * (t0 + j) + 1];
       if (v >= 0) { lbv = v; break; }
     }
     lbv = wave_max(lbv);
     if ((tid_ & 63) == 0) tmp[tid >> 6] = lbv;
     __syncthreads();
-    if (tid == 0) s_carry = max(max(tmp[0], tmp[1]), max(tmp[2], tmp[3]));
+    if (tid == 0) {
+      int c = tmp[0];
+      for (int w = 1; w < NW; w++) c = max(c, tmp[w]);
+      s_carry = c;
+    }
     __syncthreads();
   }
   // stage the tile's keys (with the row before and after) and SA values in
-  // LDS; row e sits at e + e/16 (one word of skew per 16 rows), so the
-  // 16-row runs the threads own below are read without bank conflicts
+  // LDS; row e sits at e + e/RPT (one word of skew per RPT rows), so the
+  // RPT-row runs the threads own below are read without bank conflicts
   // (all loads are issued before the LDS writes so that their latencies overlap)
   {
-    uint32_t kv[17], sv[16];
+    uint32_t kv[KL], sv[RPT];
 #pragma unroll
-    for (int q = 0; q < 17; q++) {
-      const int e = tid - 1 + 256 * q;
+    for (int q = 0; q < KL; q++) {
+      const int e = tid - 1 + NT * q;
       const int p = pos + e;
       kv[q] = (e <= n && p >= 0 && p < g.len) ? K[base + p] : 0u;
     }
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int e = tid + 256 * q;
+    for (int q = 0; q < RPT; q++) {
+      const int e = tid + NT * q;
       sv[q] = e < n ? S[base + pos + e] : 0u;
     }
 #pragma unroll
-    for (int q = 0; q < 17; q++) {
-      const int e = tid - 1 + 256 * q;
+    for (int q = 0; q < KL; q++) {
+      const int e = tid - 1 + NT * q;
       if (e <= n) {
         const int ix = e + 1;
-        sk[ix + (ix >> 4)] = kv[q];
+        sk[ix + ix / RPT] = kv[q];
       }
     }
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int e = tid + 256 * q;
+    for (int q = 0; q < RPT; q++) {
+      const int e = tid + NT * q;
       if (e < n) {
-        ss[e + (e >> 4)] = sv[q];
+        ss[e + e / RPT] = sv[q];
         if (fin) d.SA[base + pos + e] = sv[q];
       }
     }
   }
   __syncthreads();
-  // thread owns rows [16*tid, 16*tid+16) of the tile
-  const int e0 = tid * 16;
-  uint32_t kk[17];
+  const int e0 = tid * RPT;
+  uint32_t kk[RPT + 1];
   uint32_t bm = 0;
   int lastB = -1;
   {
     const int ix = e0;  // row e0 - 1
-    uint32_t prev = sk[ix + (ix >> 4)];
+    uint32_t prev = sk[ix + ix / RPT];
 #pragma unroll
-    for (int q = 0; q < 17; q++) {
+    for (int q = 0; q <= RPT; q++) {
       const int jx = e0 + q + 1;
-      kk[q] = sk[jx + (jx >> 4)];
+      kk[q] = sk[jx + jx / RPT];
     }
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
+    for (int q = 0; q < RPT; q++) {
       const int e = e0 + q;
       const int p = pos + e;
       if (e < n && (p == 0 || kk[q] != prev)) { bm |= 1u << q; lastB = p; }
       prev = kk[q];
     }
   }
-  int incl = wg_incl_max<256, int>(lastB, tmp, s_carry);
+  int incl = wg_incl_max<NT, int>(lastB, tmp, s_carry);
   xs[tid] = incl;
   __syncthreads();
   int run = tid > 0 ? xs[tid - 1] : s_carry;
   run = max(run, s_carry);
-  int head_of[16];
+  int head_of[RPT];
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
+  for (int q = 0; q < RPT; q++) {
     if (bm & (1u << q)) run = pos + e0 + q;
     head_of[q] = run;
   }
@@ -1176,25 +1184,25 @@ __device__ inline void lheads_item(const Dev& d, int cur, int fin, int t) {
   {
     const int tile_c0 = u32(d.desc[g.blk].tile_c0);
     const int nbk = (u32(d.st[g.blk].m) + kRkBucket - 1) / kRkBucket;
-    for (int j = tid; j < nbk; j += 256) bcnt[j] = 0;
+    for (int j = tid; j < nbk; j += NT) bcnt[j] = 0;
     __syncthreads();
-    uint32_t lr[16];
+    uint32_t lr[RPT];
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
+    for (int q = 0; q < RPT; q++) {
       const int e = e0 + q;
-      lr[q] = e < n ? (uint32_t)atomicAdd(&bcnt[(ss[e + (e >> 4)] & kSAMask) / kRkBucket], 1) : 0u;
+      lr[q] = e < n ? (uint32_t)atomicAdd(&bcnt[(ss[e + e / RPT] & kSAMask) / kRkBucket], 1) : 0u;
     }
     __syncthreads();
-    for (int j = tid; j < nbk; j += 256) {
+    for (int j = tid; j < nbk; j += NT) {
       const int c = bcnt[j];
       if (c) bcnt[j] = atomicAdd(&d.rkb_cnt[tile_c0 + j * (kRkBucket / kTile)], c);
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
+    for (int q = 0; q < RPT; q++) {
       const int e = e0 + q;
       if (e < n) {
-        const uint32_t v = ss[e + (e >> 4)] & kSAMask;
+        const uint32_t v = ss[e + e / RPT] & kSAMask;
         const int bk = (int)(v / kRkBucket);
         d.RKB[off + (uint64_t)bk * kRkBucket + bcnt[bk] + lr[q]] =
             ((uint64_t)(uint32_t)(g.start + head_of[q]) << 32) | v;
@@ -1205,7 +1213,7 @@ __device__ inline void lheads_item(const Dev& d, int cur, int fin, int t) {
   uint32_t endm = 0;
   int nout = 0;
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
+  for (int q = 0; q < RPT; q++) {
     const int e = e0 + q;
     const int p = pos + e;
     if (e < n && ((p == g.len - 1) || (kk[q + 1] != kk[q])) && p > head_of[q]) {
@@ -1214,13 +1222,13 @@ __device__ inline void lheads_item(const Dev& d, int cur, int fin, int t) {
     }
   }
   int tot;
-  const int ex = wg_excl_sum<256, int>(nout, tmp, &tot);
+  const int ex = wg_excl_sum<NT, int>(nout, tmp, &tot);
   if (tid == 0 && tot > 0) s_base = atomicAdd(&d.ctr[nxt], tot);
   __syncthreads();
   if (endm) {
     int idx = s_base + ex;
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
+    for (int q = 0; q < RPT; q++) {
       if ((endm >> q) & 1u) {
         if (idx < d.seg_cap) d.seg[nxt][idx] = Seg{g.blk, g.start + head_of[q], pos + e0 + q - head_of[q] + 1, 0};
         else d.ctr[CTR_ERR] = 1;
@@ -1230,11 +1238,13 @@ __device__ inline void lheads_item(const Dev& d, int cur, int fin, int t) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_lheads(Dev d, int fin) {
+constexpr int kHeadsNT = 512;  // threads per group-head workgroup
+template <int NT>
+__global__ void __launch_bounds__(NT) k_lheads(Dev d, int fin) {
   const int cur = ctr_get(d, CTR_ROUND) & 1;
   for (ItemLoop it(ctr_get(d, CTR_LTILES)); it.i < it.end; it.i += it.step) {
     __syncthreads();  // the previous tile's LDS is no longer read
-    lheads_item(d, cur, fin, it.i);
+    lheads_item<NT>(d, cur, fin, it.i);
   }
 }
 
@@ -1315,7 +1325,7 @@ SortGrids sort_grids(int device) {
   g.scatter[0] = grid_for(k_lscatter<256, false>, 256, cus);
   g.scatter[1] = grid_for(k_lscatter<512, false>, 512, cus);
   g.scatter[2] = grid_for(k_lscatter<1024, false>, 1024, cus);
-  g.heads = grid_for(k_lheads, 256, cus);
+  g.heads = grid_for(k_lheads<kHeadsNT>, kHeadsNT, cus);
   g.apply = grid_for(k_rk_apply, 256, cus);
   return g;
 }
@@ -1393,7 +1403,7 @@ void launch_bwt_round(const Dev& d, const SortGrids& g, bool first, int late_bit
   k_rk_zero<<<sized ? std::max(1, (d.ntile_c + 255) / 256) : g.light, 256, 0, s>>>(d);
   k_lsum<<<grid(g.light, ntiles), 256, 0, s>>>(d, fin);
   probe_begin("k_lheads", s);
-  k_lheads<<<grid(g.heads, ntiles), 256, 0, s>>>(d, fin);
+  k_lheads<kHeadsNT><<<grid(g.heads, ntiles), kHeadsNT, 0, s>>>(d, fin);
   probe_end("k_lheads", s);
   k_rk_apply<<<grid(g.apply, d.ntile_c), 256, 0, s>>>(d);
 }
