@@ -400,7 +400,8 @@ def run_serve(a):
 def _autotune(run_once, max_runs=16, forced=""):
     """Untimed set-up: lets the library's autotuner time each launch shape of
     the radix scatter (knob 0), then each launch policy of the suffix-sort
-    rounds (knob 1), on this workload until every size class it met has
+    rounds (knob 1), and each MTF segment run length (knob 2), on this
+    workload until every size class it met has
     settled (each variant is timed twice per class).  `forced` ("knob=variant,
     ...") pins variants instead (profiling runs)."""
     import torch
@@ -417,7 +418,7 @@ def _autotune(run_once, max_runs=16, forced=""):
         return [(c, st) for c, st in out if any(n for _, _, n in st[1]) or st[0] >= 0]
 
     def unsettled():
-        return any(not seen(k) or any(st[0] < 0 for _, st in seen(k)) for k in (0, 1))
+        return any(not seen(k) or any(st[0] < 0 for _, st in seen(k)) for k in (0, 1, 2))
 
     runs = 0
     while runs < max_runs and unsettled():
@@ -425,8 +426,9 @@ def _autotune(run_once, max_runs=16, forced=""):
         runs += 1
     torch.cuda.synchronize()
     out = {"setup_runs": runs}
-    names = {0: ("radix_scatter_threads", "ns_per_row_pass"), 1: ("sort_launch", "ns_per_sorted_row")}
-    for k in (0, 1):
+    names = {0: ("radix_scatter_threads", "ns_per_row_pass"), 1: ("sort_launch", "ns_per_sorted_row"),
+             2: ("mtf_segments_per_wave", "ns_per_capacity_tile")}
+    for k in (0, 1, 2):
         for c, (chosen, rows) in seen(k):
             key, unit = names[k]
             val = rows[chosen][0] if chosen >= 0 else None

@@ -373,6 +373,8 @@ int Engine::setup(const uint8_t* d_in, hipStream_t s, uint8_t* out) {
   d.nstream = (int32_t)S;
   d.ntile_n = (int32_t)NTn;
   d.ntile_c = (int32_t)NTc;
+  d.mtf_run = 4;
+  d.mtf_pad = 0;
   d.total_cap = CAP;
   d.hdr_stride = hdr_stride;
   d.seg_cap = seg_cap;
@@ -421,7 +423,9 @@ int Engine::sort_launch_class() const {
 }
 
 bool Engine::will_tune() const {
-  return shape_tuning_pending() || Tuner::get().exploring(TUNE_SORT_LAUNCH, sort_launch_class());
+  Tuner& tu = Tuner::get();
+  return shape_tuning_pending() || tu.exploring(TUNE_SORT_LAUNCH, sort_launch_class()) ||
+         tu.exploring(TUNE_MTF_RUN, 0);
 }
 
 int Engine::sort_round0(hipStream_t s) {
@@ -658,11 +662,17 @@ int Engine::run(hipStream_t s, bool assemble, StageTimes* times) {
   if (gate_) (void)gate_(false);
   if (rc) return rc;
   trace(s, "bwt done", rounds);
-  if (times) BZG_TRY(hipEventRecord(ev_[2], s));
+  // MTF segments per wave: an autotuner knob, timed over the MTF + RLE2
+  // stage (unit: capacity tiles) while it is explored
+  Tuner& tu = Tuner::get();
+  const int mv = tu.pick(TUNE_MTF_RUN, 0);
+  const bool mtf_timing = tu.exploring(TUNE_MTF_RUN, 0);
+  d_.mtf_run = tu.value(TUNE_MTF_RUN, mv);
+  if (times || mtf_timing) BZG_TRY(hipEventRecord(ev_[2], s));
   rc = run_stage(1, s, &Engine::run_mtf_rle2);
   if (rc) return rc;
   trace(s, "mtf+rle2 done");
-  if (times) BZG_TRY(hipEventRecord(ev_[3], s));
+  if (times || mtf_timing) BZG_TRY(hipEventRecord(ev_[3], s));
   rc = run_stage(2, s, &Engine::run_huffman);
   if (rc) return rc;
   trace(s, "huffman done");
@@ -687,6 +697,12 @@ int Engine::run(hipStream_t s, bool assemble, StageTimes* times) {
     times->small_rows = small_rows_;
     times->large_rows = large_rows_;
     times->large_pass_rows = large_pass_rows_;
+  }
+  if (mtf_timing) {
+    float ms = 0.f;
+    BZG_TRY(hipEventSynchronize(ev_[3]));
+    if (hipEventElapsedTime(&ms, ev_[2], ev_[3]) == hipSuccess)
+      tu.report(TUNE_MTF_RUN, 0, mv, (double)ms * 1e6, (double)d_.ntile_c);
   }
   return 0;
 }

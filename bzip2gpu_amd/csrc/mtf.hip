@@ -195,11 +195,10 @@ __device__ inline void wave_sync() {
 // from position p to K + p - rank[p], rank[p] = #(step bytes whose old
 // position is below p).
 //
-// A wave takes kMtfRun consecutive capacity tiles: a segment that follows
+// A wave takes d.mtf_run consecutive capacity tiles: a segment that follows
 // the previous one of the same block starts from the list state that one
 // ended with, so the state entering it is computed only for a run's first
 // segment of each block (256 x 256 compares).
-constexpr int kMtfRun = 4;
 __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
   __shared__ int32_t E[256];
   __shared__ __align__(16) uint8_t buf[kTile];
@@ -213,8 +212,9 @@ __global__ void __launch_bounds__(64) k_mtf_apply(Dev d) {
   const uint64_t below = lanemask_lt();
   const int w = (int)xcd_remap(blockIdx.x, gridDim.x);
   int held = -1;  // capacity tile whose exit list state s2p holds
-  for (int j = 0; j < kMtfRun; j++) {
-    const int t = w * kMtfRun + j;
+  const int run = d.mtf_run;
+  for (int j = 0; j < run; j++) {
+    const int t = w * run + j;
     if (t >= d.ntile_c) break;
     const int b = d.tc_blk[t];
     const BlockDesc D = d.desc[b];
@@ -597,7 +597,8 @@ void launch_mtf(const Dev& d, hipStream_t s) {
   k_mtf_gather<<<d.ntile_c, 256, 0, s>>>(d);
   k_mtf_scan<<<d.nblk, 256, 0, s>>>(d);
   probe_begin("k_mtf_apply", s);
-  k_mtf_apply<<<(d.ntile_c + kMtfRun - 1) / kMtfRun, 64, 0, s>>>(d);
+  // d.mtf_run is 1, 2, 4 or 8 (set by the engine)
+  k_mtf_apply<<<(d.ntile_c + d.mtf_run - 1) / d.mtf_run, 64, 0, s>>>(d);
   probe_end("k_mtf_apply", s);
 }
 

@@ -38,6 +38,7 @@ struct Dev {
   const uint8_t* in;
   int32_t nblk, nstream;
   int32_t ntile_n, ntile_c;
+  int32_t mtf_run, mtf_pad;  // MTF segments per wave (1, 2, 4 or 8); padding
   int64_t total_cap;
   // tile -> block maps
   const int32_t* tn_blk;

@@ -9,6 +9,9 @@ namespace bzg {
 namespace {
 // TUNE_RADIX_SHAPE: threads per 4096-row tile (16, 8 or 4 rows per thread)
 const int kRadixShapes[] = {256, 512, 1024};
+// TUNE_MTF_RUN: segments per MTF wave (the list state entering a run's first
+// segment is computed, the later ones continue from the previous segment)
+const int kMtfRuns[] = {1, 2, 4, 8};
 }  // namespace
 
 Tuner& Tuner::get() {
@@ -32,12 +35,14 @@ Tuner::Tuner() {
 int Tuner::variants(int knob) const {
   if (knob == TUNE_RADIX_SHAPE) return (int)(sizeof(kRadixShapes) / sizeof(kRadixShapes[0]));
   if (knob == TUNE_SORT_LAUNCH) return 3;
+  if (knob == TUNE_MTF_RUN) return (int)(sizeof(kMtfRuns) / sizeof(kMtfRuns[0]));
   return 0;
 }
 
 int Tuner::value(int knob, int variant) const {
   if (variant < 0 || variant >= variants(knob)) return 0;
   if (knob == TUNE_RADIX_SHAPE) return kRadixShapes[variant];
+  if (knob == TUNE_MTF_RUN) return kMtfRuns[variant];
   return variant;  // TUNE_SORT_LAUNCH: 0 per round, 1/2 device-driven (x1/x4 grids)
 }
 

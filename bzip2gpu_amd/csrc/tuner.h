@@ -21,7 +21,9 @@ enum TuneKnob : int {
   TUNE_SORT_LAUNCH = 1,  // suffix-sort rounds: per-round readback with exact
                          // grids (0) or device-driven, later rounds on
                          // persistent grids of one (1) or four (2) residencies
-  TUNE_KNOBS = 2
+  TUNE_MTF_RUN = 2,      // consecutive 4096-symbol MTF segments per wave
+                         // (1, 2, 4, 8), one size class
+  TUNE_KNOBS = 3
 };
 constexpr int kTuneMaxVariants = 4;
 constexpr int kTuneClasses = 4;

@@ -204,6 +204,13 @@ int64_t bzg_graph_replays(void);
  * device), chosen per batch size class
  * (0: under 8 MiB of block capacity, 1: under 128 MiB, 2: larger) from the
  * sort stage's time per sorted row, once knob 0 has settled.
+ * Knob 2 is the number of consecutive 4096-symbol MTF segments one wave
+ * takes, variants 0/1/2/3 = 1/2/4/8 (the list state entering the first
+ * segment of a run is computed from the segment summaries, later segments
+ * continue from the previous one), one size class, chosen from the MTF +
+ * RLE2 stage's time per capacity tile.
+ * Batched calls take their sub-batches one at a time while any knob is
+ * being timed.
  * A choice is made after timing each variant twice (the least total time
  * per total work of its samples) and is kept in the file named by
  * BZG_TUNE_CACHE when that variable is set.
@@ -211,7 +218,8 @@ int64_t bzg_graph_replays(void);
  * bzg_tune_state returns the chosen variant of a class (-1 while exploring,
  * -2 for a bad argument) and fills, per variant, its launch value, the
  * nanoseconds per unit of work over its samples (knob 0: rows x radix
- * passes; knob 1: sorted rows), and its sample count. */
+ * passes; knob 1: sorted rows; knob 2: capacity tiles), and its sample
+ * count. */
 bzg_status bzg_tune_force(int32_t knob, int32_t variant);
 int32_t bzg_tune_state(int32_t knob, int32_t size_class, int32_t *h_values,
                        float *h_ns_per_unit, int32_t *h_samples,

@@ -145,6 +145,27 @@ def test_radix_shapes_identical(dev, variant):
         bz.tune_force(0, -1)
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_mtf_segment_runs_identical(dev, variant):
+    """Every MTF run length (1, 2, 4 or 8 segments per wave) gives the
+    oracle's stream, with runs that cross block ends (100000-byte blocks
+    span 25 tiles) and a block shorter than a run."""
+    import bzip2gpu_amd as bz
+    import bzip2gpu_amd.datagen as dg
+    data = dg.mix(3 << 20, 81).tobytes()
+    try:
+        bz.tune_force(2, variant)
+        for level in (9, 1):
+            _case(f"mtf segment run variant {variant}, level {level}")
+            bb = 100000 * level
+            _same(bz.compress(data, level, block_bytes=bb), oracle.compress(data, level, block_bytes=bb),
+                  f"variant {variant} level {level}")
+        short = data[:30000]
+        _same(bz.compress(short, 9), oracle.compress(short, 9), f"variant {variant} short")
+    finally:
+        bz.tune_force(2, -1)
+
+
 @pytest.mark.parametrize("policy", [0, 1, 2])
 def test_sort_launch_policies_identical(dev, policy):
     """Both launch policies of the suffix-sort rounds (per round with exact
