@@ -119,6 +119,7 @@ class Decoder {
   size_t out_len_ = 0;
   int64_t nbits_ = 0;
   Buf w_, cand_, res_, sym_, slot_, idx_, small_, blk_, bres_, stblk_, f_, p_, tlen_, toff_;
+  Buf sel_;  // per candidate: raw selectors, then their MTF-decoded tables (2 x kDecSelCap bytes)
   Buf l_, tt_, mtblk_, hist_, cb_, nodeblk_, nx_, ln_, pos_, t_, tf_, tent_, too_, treg_, out_, stg_, rrow_;
   int32_t* h_small_ = nullptr;  // pinned
   std::vector<DecCand> cands_;

@@ -59,7 +59,6 @@ namespace {
 
 constexpr uint64_t kBlockMagic = 0x314159265359ull;
 constexpr uint64_t kEosMagic = 0x177245385090ull;
-constexpr int kLutBits = 12;
 constexpr int kMaxLen = 20;
 constexpr int kPadWords = 16;
 constexpr int kHdrWords = 2048;  // input words of a block header kept in LDS
@@ -153,10 +152,14 @@ __global__ void __launch_bounds__(256) k_dec_magic(const uint32_t* __restrict__ 
 
 // ---------------------------------------------------------------- stage 2
 
+// Lookup tables of LB bits (codes up to LB bits in one read): LB = 12 takes
+// about 56 KB (two blocks' decodes per CU), LB = 10 about 20 KB (eight per
+// CU; the selectors live in global memory).
+template <int LB>
 struct HuffLds {
-  uint8_t sel[kDecSelCap];
+  static constexpr int kLong = kMaxLen - LB;  // code lengths past the table
   union {
-    uint16_t lut[kMaxGroups * (1 << kLutBits)];  // (symbol << 5) | length, 0 = none
+    uint16_t lut[kMaxGroups * (1 << LB)];  // (symbol << 5) | length, 0 = none
     uint32_t hw[kHdrWords];                      // header words (before the tables exist)
   };
   uint32_t ring[kRing];
@@ -164,10 +167,11 @@ struct HuffLds {
   uint8_t lens[kMaxGroups * kMaxAlpha];
   int32_t first[kMaxGroups * 21], cnt[kMaxGroups * 21], base[kMaxGroups * 21];
   int32_t maxlen[kMaxGroups];
-  // codes longer than the lookup table (lengths 13..20): left-aligned 20-bit
-  // limits (first code + count) and perm index minus first code, per length
-  uint32_t lim[kMaxGroups][8] __attribute__((aligned(16)));
-  int32_t dlt[kMaxGroups][8];
+  // codes longer than the lookup table (lengths LB+1..20): left-aligned
+  // 20-bit limits (first code + count) and perm index minus first code, per
+  // length (padded to whole 16-byte loads)
+  uint32_t lim[kMaxGroups][(kLong + 3) / 4 * 4] __attribute__((aligned(16)));
+  int32_t dlt[kMaxGroups][kLong];
   int32_t run[32];
   uint8_t seq[256];
 };
@@ -175,35 +179,38 @@ struct HuffLds {
 // Codes longer than the lookup table: the length is the smallest l whose
 // left-aligned limit exceeds the 20 peeked bits (empty lengths repeat the
 // previous limit, so the smallest such l is a used length).
-__device__ inline uint32_t long_code(const HuffLds& H, int t, uint32_t p20) {
-  static_assert(kLutBits == 12, "long codes are lengths 13..20");
-  const uint4 a = *reinterpret_cast<const uint4*>(&H.lim[t][0]);
-  const uint4 b = *reinterpret_cast<const uint4*>(&H.lim[t][4]);
+template <int LB>
+__device__ inline uint32_t long_code(const HuffLds<LB>& H, int t, uint32_t p20) {
+  constexpr int NL = HuffLds<LB>::kLong;
+  uint32_t lim[(NL + 3) / 4 * 4];
+#pragma unroll
+  for (int q = 0; q < (NL + 3) / 4; q++)
+    *reinterpret_cast<uint4*>(&lim[4 * q]) = *reinterpret_cast<const uint4*>(&H.lim[t][4 * q]);
   int l = 0;
-  l = p20 < b.w ? 20 : l;
-  l = p20 < b.z ? 19 : l;
-  l = p20 < b.y ? 18 : l;
-  l = p20 < b.x ? 17 : l;
-  l = p20 < a.w ? 16 : l;
-  l = p20 < a.z ? 15 : l;
-  l = p20 < a.y ? 14 : l;
-  l = p20 < a.x ? 13 : l;
+#pragma unroll
+  for (int q = NL - 1; q >= 0; q--) l = p20 < lim[q] ? LB + 1 + q : l;
   if (l == 0) return 0u;
-  const int i = (int)(p20 >> (kMaxLen - l)) + H.dlt[t][l - 13];
+  const int i = (int)(p20 >> (kMaxLen - l)) + H.dlt[t][l - LB - 1];
   return ((uint32_t)H.perm[t * kMaxAlpha + i] << 5) | (uint32_t)l;
 }
 
 __device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // One 64-thread workgroup per candidate (idx[blockIdx.x] indexes cands).
+template <int kLutBits>
 __global__ void __launch_bounds__(64) k_dec_huff(const uint32_t* __restrict__ W, int64_t nbits, int64_t nwords,
                                                  const DecCand* __restrict__ cands,
                                                  const int32_t* __restrict__ idx, DecRes* __restrict__ res,
-                                                 uint16_t* __restrict__ sym) {
-  __shared__ HuffLds H;
+                                                 uint16_t* __restrict__ sym, uint8_t* __restrict__ selbuf) {
+  __shared__ HuffLds<kLutBits> H;
   const int lane = threadIdx.x;
   const int ci = idx[blockIdx.x];
   const DecCand c = cands[ci];
+  // this candidate's selectors: raw table indexes, then the MTF-decoded ones
+  // (a separate area, so no lane reads a line it read before another lane
+  // wrote it)
+  uint8_t* sel_raw = selbuf + (int64_t)ci * (2 * kDecSelCap);
+  uint8_t* sel = sel_raw + kDecSelCap;
   DecRes* R = res + ci;
   const uint64_t wlast = (uint64_t)nwords - 1;  // a zero pad word
   auto gw = [&](uint64_t i) -> uint32_t { return W[i < wlast ? i : wlast]; };
@@ -271,7 +278,7 @@ __global__ void __launch_bounds__(64) k_dec_huff(const uint32_t* __restrict__ W,
         const uint64_t below = zm & lanemask_lt();
         const int j = below ? lane - (63 - __clzll(below)) - 1 : run + lane;
         if (j >= ng) bad = true;
-        if (k + r < kDecSelCap) H.sel[k + r] = (uint8_t)j;
+        if (k + r < kDecSelCap) sel_raw[k + r] = (uint8_t)j;
       }
       if (ballot(bad)) {
         status = kDecBad;
@@ -292,12 +299,13 @@ __global__ void __launch_bounds__(64) k_dec_huff(const uint32_t* __restrict__ W,
     }
   }
   // move-to-front over the table ids, 64 selectors per batch
+  __syncthreads();  // the raw selectors are in memory
   if (status == kDecOk) {
     const int nk = min(nsel, kDecSelCap);
     uint32_t mtf = 0x543210u;
     for (int k0 = 0; k0 < nk; k0 += 64) {
       const int i = k0 + lane;
-      const uint32_t raw = i < nk ? H.sel[i] : 0u;
+      const uint32_t raw = i < nk ? sel_raw[i] : 0u;
       uint32_t res = 0;
       const int cnt = min(64, nk - k0);
       for (int q = 0; q < cnt; q++) {
@@ -307,7 +315,7 @@ __global__ void __launch_bounds__(64) k_dec_huff(const uint32_t* __restrict__ W,
         mtf = (mtf & ~((low << 4) | 15u)) | ((mtf & low) << 4) | v;
         res = lane == q ? v : res;
       }
-      if (i < nk) H.sel[i] = (uint8_t)res;
+      if (i < nk) sel[i] = (uint8_t)res;
     }
   }
   // code lengths: 5-bit start, then per symbol "1x" steps (x=0: +1, 1: -1) ending in "0"
@@ -405,8 +413,8 @@ __global__ void __launch_bounds__(64) k_dec_huff(const uint32_t* __restrict__ W,
     const int cap = c.sym_cap;
     uint16_t* out = sym + c.sym_off;
     int gi = 0, left = kGSize;
-    int tbl = uni(H.sel[0]);
-    uint32_t tnext = H.sel[min(1, nsel_kept - 1)];  // next group's table (read ahead)
+    int tbl = uni(sel[0]);
+    uint32_t tnext = sel[min(1, nsel_kept - 1)];  // next group's table (read ahead)
     // input words in an LDS ring of 4 chunks of 64 words: chunks cb and
     // cb + 64 are read, cb + 128 is resident and cb + 192 is in flight (pf)
     const int64_t cb0 = pos >> 5;
@@ -532,7 +540,7 @@ __global__ void __launch_bounds__(64) k_dec_huff(const uint32_t* __restrict__ W,
           break;
         }
         tbl = uni((int)tnext);
-        tnext = H.sel[min(gi + 1, nsel_kept - 1)];
+        tnext = sel[min(gi + 1, nsel_kept - 1)];
         left = kGSize;
       }
     }
@@ -1364,8 +1372,19 @@ int Decoder::decode_cands(const std::vector<int32_t>& idx, hipStream_t s) {
   if (idx.empty()) return 0;
   DEC_NEED(idx_, idx.size() * sizeof(int32_t));
   DEC_CK(hipMemcpyAsync(idx_.p, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
-  DEC_LAUNCH(k_dec_huff, dim3((unsigned)idx.size()), dim3(64), 0, s, (const uint32_t*)w_.p, nbits_,
-                     (nbits_ + 31) / 32 + kPadWords, (const DecCand*)cand_.p, (const int32_t*)idx_.p, (DecRes*)res_.p, (uint16_t*)sym_.p);
+  // 12-bit tables while every block's decode fits on the CUs at two per
+  // CU, 10-bit tables (eight per CU) for more blocks
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_) != hipSuccess || cus < 1) cus = 256;
+  if (idx.size() <= (size_t)(2 * cus)) {
+    DEC_LAUNCH(k_dec_huff<12>, dim3((unsigned)idx.size()), dim3(64), 0, s, (const uint32_t*)w_.p, nbits_,
+               (nbits_ + 31) / 32 + kPadWords, (const DecCand*)cand_.p, (const int32_t*)idx_.p, (DecRes*)res_.p,
+               (uint16_t*)sym_.p, (uint8_t*)sel_.p);
+  } else {
+    DEC_LAUNCH(k_dec_huff<10>, dim3((unsigned)idx.size()), dim3(64), 0, s, (const uint32_t*)w_.p, nbits_,
+               (nbits_ + 31) / 32 + kPadWords, (const DecCand*)cand_.p, (const int32_t*)idx_.p, (DecRes*)res_.p,
+               (uint16_t*)sym_.p, (uint8_t*)sel_.p);
+  }
   return (int)hipGetLastError();
 }
 
@@ -1438,6 +1457,7 @@ int Decoder::run(const uint8_t* d_in, size_t len, hipStream_t s, DecodeTimes* tm
     DEC_NEED(sym_, (size_t)sym_tot_ * 2);
     DEC_NEED(slot_, (size_t)sym_tot_);
     DEC_NEED(res_, (size_t)std::max(ncand, 1) * sizeof(DecRes));
+    DEC_NEED(sel_, (size_t)std::max(ncand, 1) * 2 * kDecSelCap);
     if (ncand) DEC_CK(hipMemcpyAsync(cand_.p, cands_.data(), (size_t)ncand * sizeof(DecCand), hipMemcpyHostToDevice, s));
     if (int r = decode_cands(all, s)) return r;
     if (ncand) DEC_CK(hipMemcpyAsync(res_h_.data(), res_.p, (size_t)ncand * sizeof(DecRes), hipMemcpyDeviceToHost, s));

@@ -162,3 +162,20 @@ def test_device_input_large(dev):
     assert nb == (data.size + 899999) // 900000 and ns == 1
     out = bz.decompress(c)
     _same(out, data.tobytes(), "64 MiB mix")
+
+
+def test_many_streams_small_tables(dev):
+    """More blocks than two per CU (here 700 streams, 1400 candidates): the
+    decoder takes its 10-bit lookup tables (codes of 11-20 bits through the
+    length limits).  Stock streams and this library's, levels 1-9."""
+    import bzip2gpu_amd as bz
+    import bzip2gpu_amd.datagen as dg
+    rng = np.random.default_rng(31)
+    parts, streams = [], []
+    for i in range(700):
+        p = dg.mix(int(rng.integers(200, 6000)), 300 + i).tobytes()
+        lv = int(rng.integers(1, 10))
+        parts.append(p)
+        streams.append(bz2.compress(p, lv) if i % 2 else bz.compress(p, lv))
+    _case("decompress 700 concatenated streams")
+    _same(bz.decompress(b"".join(streams)), b"".join(parts), "700 streams")
