@@ -615,11 +615,14 @@ __global__ void __launch_bounds__(64) k_dec_mtf1(const DecBlk* __restrict__ blk,
       const uint32_t xi = (uint32_t)__builtin_amdgcn_readlane((int)x, j) - 1u;  // 1..255
       const uint32_t sw = (uint32_t)__builtin_amdgcn_readlane((int)wl, (int)(xi >> 2));
       const uint32_t sl = (sw >> ((xi & 3u) * 8u)) & 255u;
-      // list positions 0..xi shift up by one, position 0 takes the slot
-      const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)wl, 0x138, 0xf, 0xf, false);
-      const uint32_t sh = (wl << 8) | (lane == 0 ? sl : (prev >> 24));
-      const int dd = (int)xi - 4 * lane;
-      const uint32_t mask = dd >= 3 ? 0xFFFFFFFFu : (dd < 0 ? 0u : (0xFFFFFFFFu >> (8 * (3 - dd))));
+      // list positions 0..xi shift up by one, position 0 takes the slot;
+      // the lane's bytes at positions <= xi (the low c of its 4, c clamped
+      // to 0..4) take the shifted word: mask = high word of 2^32-1 << 8c
+      // (lane 0, which has no lane below, takes sl in the shifted-in byte)
+      const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp((int)(sl << 24), (int)wl, 0x138, 0xf, 0xf, false);
+      const uint32_t sh = (wl << 8) | (prev >> 24);
+      const int c = min(max((int)xi - 4 * lane + 1, 0), 4);
+      const uint32_t mask = (uint32_t)((0xFFFFFFFFull << (8 * c)) >> 32);
       wl = (sh & mask) | (wl & ~mask);
       myslot = lane == j ? sl : myslot;
     }
