@@ -61,7 +61,6 @@ constexpr uint64_t kBlockMagic = 0x314159265359ull;
 constexpr uint64_t kEosMagic = 0x177245385090ull;
 constexpr int kMaxLen = 20;
 constexpr int kPadWords = 16;
-constexpr int kHdrWords = 2048;  // input words of a block header kept in LDS
 constexpr int kRing = 256;       // LDS ring of input words for the symbol loop
 
 // ---------------------------------------------------------------- bit input
@@ -153,14 +152,16 @@ __global__ void __launch_bounds__(256) k_dec_magic(const uint32_t* __restrict__ 
 // ---------------------------------------------------------------- stage 2
 
 // Lookup tables of LB bits (codes up to LB bits in one read): LB = 12 takes
-// about 56 KB (two blocks' decodes per CU), LB = 10 about 20 KB (eight per
-// CU; the selectors live in global memory).
+// about 56 KB of LDS (two blocks' decodes per CU), LB = 9 about 14 KB
+// (eleven per CU; the selectors live in global memory).  The first kHdr
+// words of a block (its header) are read from LDS, later ones from memory.
 template <int LB>
 struct HuffLds {
   static constexpr int kLong = kMaxLen - LB;  // code lengths past the table
+  static constexpr int kHdr = LB >= 12 ? 2048 : 1024;
   union {
     uint16_t lut[kMaxGroups * (1 << LB)];  // (symbol << 5) | length, 0 = none
-    uint32_t hw[kHdrWords];                      // header words (before the tables exist)
+    uint32_t hw[kHdr];                     // header words (before the tables exist)
   };
   uint32_t ring[kRing];
   uint16_t perm[kMaxGroups * kMaxAlpha];       // symbols by (length, symbol)
@@ -234,6 +235,7 @@ __global__ void __launch_bounds__(64) k_dec_huff(const uint32_t* __restrict__ W,
   }
   // the words of the block header, read from LDS by the bit reader
   const uint64_t hb = (uint64_t)(c.bit + 48) >> 5;
+  constexpr int kHdrWords = HuffLds<kLutBits>::kHdr;
   for (int i = lane; i < kHdrWords; i += 64) H.hw[i] = gw(hb + i);
   __syncthreads();
   auto hwf = [&](uint64_t i) -> uint32_t {
@@ -1376,7 +1378,7 @@ int Decoder::decode_cands(const std::vector<int32_t>& idx, hipStream_t s) {
   DEC_NEED(idx_, idx.size() * sizeof(int32_t));
   DEC_CK(hipMemcpyAsync(idx_.p, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
   // 12-bit tables while every block's decode fits on the CUs at two per
-  // CU, 10-bit tables (eight per CU) for more blocks
+  // CU, 9-bit tables (eleven per CU) for more blocks
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_) != hipSuccess || cus < 1) cus = 256;
   if (idx.size() <= (size_t)(2 * cus)) {
@@ -1384,7 +1386,7 @@ int Decoder::decode_cands(const std::vector<int32_t>& idx, hipStream_t s) {
                (nbits_ + 31) / 32 + kPadWords, (const DecCand*)cand_.p, (const int32_t*)idx_.p, (DecRes*)res_.p,
                (uint16_t*)sym_.p, (uint8_t*)sel_.p);
   } else {
-    DEC_LAUNCH(k_dec_huff<10>, dim3((unsigned)idx.size()), dim3(64), 0, s, (const uint32_t*)w_.p, nbits_,
+    DEC_LAUNCH(k_dec_huff<9>, dim3((unsigned)idx.size()), dim3(64), 0, s, (const uint32_t*)w_.p, nbits_,
                (nbits_ + 31) / 32 + kPadWords, (const DecCand*)cand_.p, (const int32_t*)idx_.p, (DecRes*)res_.p,
                (uint16_t*)sym_.p, (uint8_t*)sel_.p);
   }

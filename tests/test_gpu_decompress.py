@@ -166,7 +166,7 @@ def test_device_input_large(dev):
 
 def test_many_streams_small_tables(dev):
     """More blocks than two per CU (here 700 streams, 1400 candidates): the
-    decoder takes its 10-bit lookup tables (codes of 11-20 bits through the
+    decoder takes its 9-bit lookup tables (codes of 10-20 bits through the
     length limits).  Stock streams and this library's, levels 1-9."""
     import bzip2gpu_amd as bz
     import bzip2gpu_amd.datagen as dg
