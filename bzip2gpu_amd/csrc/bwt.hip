@@ -322,8 +322,12 @@ struct PackLds {
   int32_t count;
   int32_t total;
   int32_t last_len;
+  int32_t max_len;  // longest group
   int32_t tmp[8];
 };
+
+// packs whose groups are all at most this long are sorted by counting
+constexpr int kCountMax = 64;
 
 // Loads the small groups of pack k into LDS (a pack holds <= 1023 groups:
 // each has >= 2 rows and the pack <= kPack-1 rows).
@@ -331,6 +335,7 @@ __device__ inline void load_pack(const Dev& d, int cur, int k, PackLds& P) {
   const int tid_ = loop_tid();
   const int i0 = u32(d.packs[k]), i1 = u32(d.packs[k + 1]);
   int count = 0;
+  if (tid_ == 0) P.max_len = 0;
   for (int base = i0; base < i1; base += 256) {
     int i = base + tid_;
     Seg g{0, 0, 0, 0};
@@ -345,6 +350,8 @@ __device__ inline void load_pack(const Dev& d, int cur, int k, PackLds& P) {
       P.lofs[idx] = d.pa[i];
       if (ex == tot - 1) P.last_len = g.len;
     }
+    const int ml = wave_max(small ? g.len : 0);
+    if ((tid_ & 63) == 0 && ml > 0) atomicMax(&P.max_len, ml);
     count += tot;
     __syncthreads();
   }
@@ -370,9 +377,34 @@ __device__ inline int find_seg(const PackLds& P, int e) {  // last j with lofs[j
 }
 
 // ------------------------------------------------------------------- keys
+// The rotation's 4 bytes at start q (cyclic, big-endian: the round-0 key of
+// q), for a block of m > 4 bytes at T (4-byte aligned).
+__device__ inline uint32_t text_key(const uint8_t* T, uint32_t q, uint32_t m) {
+  const uint32_t w = q >> 2;
+  uint32_t x;
+  if (4 * w + 8 <= m) {  // both words inside the block
+    const uint32_t* T32 = reinterpret_cast<const uint32_t*>(T);
+    x = __builtin_amdgcn_alignbyte(T32[w + 1], T32[w], q & 3u);
+  } else {
+    x = 0;
+    uint32_t p = q;
+    for (int k = 0; k < 4; k++) {
+      x |= (uint32_t)T[p] << (8 * k);
+      p = p + 1 == m ? 0 : p + 1;
+    }
+  }
+  return __builtin_bswap32(x);
+}
+
 // Rounds r >= 1: key(i) = RK[(SA[i] + P) mod m] for every row of every
 // listed group (P >= m: identical rotations, ordered by start index).
 // Item w < npacks is a small-group pack, the others large-group tiles.
+// In round 1 (P = 4) the ranks are those of round 0, which order the
+// rotations by their first 4 bytes, so the 4 text bytes at SA + 4 give the
+// same order and the same ties; packs sorted by counting (groups of at most
+// kCountMax rows, compared only within a group) take those: a block's text
+// is a quarter of its rank array, so the gathers of its rows share cache
+// lines four times as often.
 __device__ inline void keys_item(const Dev& d, int cur, int round, int P, int npacks, int w, PackLds& L) {
   const int tid_ = loop_tid();
   if (w < npacks) {
@@ -381,6 +413,7 @@ __device__ inline void keys_item(const Dev& d, int cur, int round, int P, int np
     const int total = L.total;
     for (int j = tid_; j < L.count; j += 256)
       if (j == 0 || L.blk[j] != L.blk[j - 1]) atomicMax(&d.st[L.blk[j]].rounds, round + 1);
+    const bool by_text = P == 4 && L.max_len <= kCountMax;
     // rows' segment and block first (block data reloaded only when the block
     // changes), then all SA loads, then all rank loads, then the stores
     int jj[kPack / 256], mm[kPack / 256];
@@ -414,7 +447,7 @@ __device__ inline void keys_item(const Dev& d, int cur, int round, int P, int np
       if (jj[q] >= 0 && P < mm[q]) {
         uint32_t p = v[q] + (uint32_t)P;
         if (p >= (uint32_t)mm[q]) p -= (uint32_t)mm[q];
-        v[q] = d.RK[oo[q] + p];
+        v[q] = by_text ? text_key(d.T + oo[q], p, (uint32_t)mm[q]) : d.RK[oo[q] + p];
       }
     }
 #pragma unroll
@@ -502,8 +535,6 @@ __device__ inline void emit_groups(const Dev& d, int nxt, bool has, Seg g, int32
 }
 
 // ----------------------------------------------------------- small groups
-// packs whose groups are all at most this long are sorted by counting
-constexpr int kCountMax = 64;
 // One workgroup sorts a pack of <= kPack rows in LDS.  KT holds the
 // composite key (group in pack << keybits | key): 32 bits whenever they fit.
 // Every row of a group already has the group's first row as its rank (from
@@ -525,6 +556,7 @@ __device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, i
   load_pack(d, cur, k, L);
   if (L.count == 0) return;
   const int total = L.total;
+  const bool counting = L.max_len <= kCountMax;
   // group of every row: mark group starts, then a running max over rows
   uint16_t* sid = reinterpret_cast<uint16_t*>(K1);  // free until the first pass
   for (int e = tid; e < kPack; e += 256) sid[e] = 0;
@@ -571,24 +603,19 @@ __device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, i
         sv[q] = d.SA[coff + j];
       }
     }
+    // (the counting sort compares keys within a group only: the key alone,
+    // which in round 1 may be 32 text bits)
 #pragma unroll
     for (int q = 0; q < kPack / 256; q++) {
       const int e = tid + 256 * q;
       if (e < total) {
-        K0[e] = ((KT)sid[e] << keybits) | (KT)kv[q];
+        K0[e] = counting ? (KT)kv[q] : (((KT)sid[e] << keybits) | (KT)kv[q]);
         V0[e] = sv[q];
       }
     }
   }
-  // largest group of the pack
-  {
-    int mxl = 0;
-    for (int j = tid; j < L.count; j += 256) mxl = max(mxl, (j + 1 < L.count ? L.lofs[j + 1] : total) - L.lofs[j]);
-    mxl = wave_max(mxl);
-    if (lane == 0) xs[wv] = mxl;
-  }
   __syncthreads();
-  if (max(max(xs[0], xs[1]), max(xs[2], xs[3])) <= kCountMax) {
+  if (counting) {
     // Small groups only: every row's place in its group by counting (rows
     // with a smaller key, and equal keys before it); equal keys stay together
     // in input order and form the next round's subgroup.
