@@ -401,10 +401,13 @@ __device__ inline uint32_t text_key(const uint8_t* T, uint32_t q, uint32_t m) {
 // Item w < npacks is a small-group pack, the others large-group tiles.
 // In round 1 (P = 4) the ranks are those of round 0, which order the
 // rotations by their first 4 bytes, so the 4 text bytes at SA + 4 give the
-// same order and the same ties; packs sorted by counting (groups of at most
-// kCountMax rows, compared only within a group) take those: a block's text
-// is a quarter of its rank array, so the gathers of its rows share cache
-// lines four times as often.
+// same order and the same ties.  Packs sorted by counting (groups of at most
+// kCountMax rows, compared only within a group) take those, and since the
+// text does not change during the round, the sort gathers them itself
+// (`text_keys`): such packs have nothing to do here but mark their blocks'
+// round count.  A block's text is a quarter of its rank array, so the
+// gathers of its rows share cache lines four times as often.
+__device__ inline bool text_keys(int P, const PackLds& L) { return P == 4 && L.max_len <= kCountMax; }
 __device__ inline void keys_item(const Dev& d, int cur, int round, int P, int npacks, int w, PackLds& L) {
   const int tid_ = loop_tid();
   if (w < npacks) {
@@ -413,7 +416,7 @@ __device__ inline void keys_item(const Dev& d, int cur, int round, int P, int np
     const int total = L.total;
     for (int j = tid_; j < L.count; j += 256)
       if (j == 0 || L.blk[j] != L.blk[j - 1]) atomicMax(&d.st[L.blk[j]].rounds, round + 1);
-    const bool by_text = P == 4 && L.max_len <= kCountMax;
+    if (text_keys(P, L)) return;
     // rows' segment and block first (block data reloaded only when the block
     // changes), then all SA loads, then all rank loads, then the stores
     int jj[kPack / 256], mm[kPack / 256];
@@ -447,7 +450,7 @@ __device__ inline void keys_item(const Dev& d, int cur, int round, int P, int np
       if (jj[q] >= 0 && P < mm[q]) {
         uint32_t p = v[q] + (uint32_t)P;
         if (p >= (uint32_t)mm[q]) p -= (uint32_t)mm[q];
-        v[q] = by_text ? text_key(d.T + oo[q], p, (uint32_t)mm[q]) : d.RK[oo[q] + p];
+        v[q] = d.RK[oo[q] + p];
       }
     }
 #pragma unroll
@@ -542,15 +545,31 @@ __device__ inline void emit_groups(const Dev& d, int nxt, bool has, Seg g, int32
 // it: after round 0 (which ranks every row first) only the other rows'
 // ranks are written (`keep_first`).
 template <typename KT>
-__device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, int keybits, int k) {
+struct SortLds {
+  PackLds L;
+  KT K0[kPack], K1[kPack];
+  uint32_t V0[kPack], V1[kPack];
+  uint32_t cnt[4][256];
+  int32_t xs[256];
+  int32_t s_base;
+  int32_t tmp[8];
+};
+
+// TEXT: the round may take keys from the text (round 1); the other rounds
+// run a copy without that path (both copies share the LDS in S).
+template <typename KT, bool TEXT>
+__device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, int P, int keybits, int k,
+                                       SortLds<KT>& S) {
   const int tid_ = loop_tid();
-  __shared__ PackLds L;
-  __shared__ KT K0[kPack], K1[kPack];
-  __shared__ uint32_t V0[kPack], V1[kPack];
-  __shared__ uint32_t cnt[4][256];
-  __shared__ int32_t xs[256];
-  __shared__ int32_t s_base;
-  __shared__ int32_t tmp[8];
+  PackLds& L = S.L;
+  KT* K0 = S.K0;
+  KT* K1 = S.K1;
+  uint32_t* V0 = S.V0;
+  uint32_t* V1 = S.V1;
+  uint32_t (*cnt)[256] = S.cnt;
+  int32_t* xs = S.xs;
+  int32_t& s_base = S.s_base;
+  int32_t* tmp = S.tmp;
   const int tid = tid_, wv = tid >> 6, lane = (tid_ & 63);
   const int nxt = cur ^ 1;
   load_pack(d, cur, k, L);
@@ -583,24 +602,45 @@ __device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, i
   // block offset is reloaded only when the block changes, and the rows'
   // loads are all in flight before the LDS writes)
   {
-    int cb = -1;
+    // round 1's counting-sorted packs gather their keys from the text here
+    const bool tk = TEXT && text_keys(P, L);
+    int cb = -1, cm = 0;
     uint64_t coff = 0;
     uint32_t kv[kPack / 256], sv[kPack / 256];
+    uint64_t oo[kPack / 256];
+    int mm[kPack / 256];
 #pragma unroll
     for (int q = 0; q < kPack / 256; q++) {
       const int e = tid + 256 * q;
       kv[q] = 0;
       sv[q] = 0;
+      oo[q] = 0;
+      mm[q] = 0;
       if (e < total) {
         const int sg = sid[e];
         const int b = L.blk[sg];
         if (b != cb) {
           cb = b;
           coff = d.desc[b].off;
+          cm = tk ? d.st[b].m : 0;
         }
         const int j = L.start[sg] + (e - L.lofs[sg]);
-        kv[q] = d.KEY[coff + j];
+        if (!tk) kv[q] = d.KEY[coff + j];
         sv[q] = d.SA[coff + j];
+        oo[q] = coff;
+        mm[q] = cm;
+      }
+    }
+    if (tk) {
+#pragma unroll
+      for (int q = 0; q < kPack / 256; q++) {
+        if (tid + 256 * q < total) {
+          // key(i) = the 4 bytes at SA + 4 (the start itself when the
+          // prefix covers the rotation: identical rotations by start)
+          uint32_t p = (sv[q] & kSAMask) + 4u;
+          if (p >= (uint32_t)mm[q]) p -= (uint32_t)mm[q];
+          kv[q] = 4 < mm[q] ? text_key(d.T + oo[q], p, (uint32_t)mm[q]) : (sv[q] & kSAMask);
+        }
       }
     }
     // (the counting sort compares keys within a group only: the key alone,
@@ -789,10 +829,12 @@ __device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, i
 
 template <typename KT>
 __device__ inline void sort_small_loop(const Dev& d, int keybits) {
+  __shared__ SortLds<KT> S;
   const int round = ctr_get(d, CTR_ROUND), cur = round & 1;
   for (ItemLoop it(round_packs(d)); it.i < it.end; it.i += it.step) {
     __syncthreads();  // the previous pack's LDS is no longer read
-    sort_small_body<KT>(d, cur, round > 0, keybits, it.i);
+    if (round == 1) sort_small_body<KT, true>(d, cur, true, round_prefix(1), keybits, it.i, S);
+    else sort_small_body<KT, false>(d, cur, round > 0, 0, keybits, it.i, S);
   }
 }
 
