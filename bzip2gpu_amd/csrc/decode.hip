@@ -1356,12 +1356,13 @@ int Decoder::fail(const std::string& msg) {
 }
 
 // launch with the per-kernel event probe (benchmarking aid) around it
-#define DEC_LAUNCH(k, grid, block, shm, s, ...)             \
+#define DEC_LAUNCH_AS(name, k, grid, block, shm, s, ...)  \
   do {                                                     \
-    probe_begin(#k, s);                                    \
+    probe_begin(name, s);                                  \
     hipLaunchKernelGGL(k, grid, block, shm, s, __VA_ARGS__); \
-    probe_end(#k, s);                                      \
+    probe_end(name, s);                                    \
   } while (0)
+#define DEC_LAUNCH(k, grid, block, shm, s, ...) DEC_LAUNCH_AS(#k, k, grid, block, shm, s, __VA_ARGS__)
 #define DEC_CK(x)                         \
   do {                                    \
     hipError_t e_ = (x);                  \
@@ -1382,11 +1383,11 @@ int Decoder::decode_cands(const std::vector<int32_t>& idx, hipStream_t s) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_) != hipSuccess || cus < 1) cus = 256;
   if (idx.size() <= (size_t)(2 * cus)) {
-    DEC_LAUNCH(k_dec_huff<12>, dim3((unsigned)idx.size()), dim3(64), 0, s, (const uint32_t*)w_.p, nbits_,
+    DEC_LAUNCH_AS("k_dec_huff", k_dec_huff<12>, dim3((unsigned)idx.size()), dim3(64), 0, s, (const uint32_t*)w_.p, nbits_,
                (nbits_ + 31) / 32 + kPadWords, (const DecCand*)cand_.p, (const int32_t*)idx_.p, (DecRes*)res_.p,
                (uint16_t*)sym_.p, (uint8_t*)sel_.p);
   } else {
-    DEC_LAUNCH(k_dec_huff<9>, dim3((unsigned)idx.size()), dim3(64), 0, s, (const uint32_t*)w_.p, nbits_,
+    DEC_LAUNCH_AS("k_dec_huff", k_dec_huff<9>, dim3((unsigned)idx.size()), dim3(64), 0, s, (const uint32_t*)w_.p, nbits_,
                (nbits_ + 31) / 32 + kPadWords, (const DecCand*)cand_.p, (const int32_t*)idx_.p, (DecRes*)res_.p,
                (uint16_t*)sym_.p, (uint8_t*)sel_.p);
   }
