@@ -108,6 +108,7 @@ class Engine {
   void set_sort_gate(std::function<int(bool)> gate) { gate_ = std::move(gate); }
   // Whether a run of the planned batch would time autotuner variants.
   bool will_tune() const;
+  int mtf_run_class() const;
   // Stage groups (used by run and by the per-stage entry points).
   int run_front(hipStream_t s);
   int run_bwt(hipStream_t s, int* rounds_out = nullptr);

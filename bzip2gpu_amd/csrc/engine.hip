@@ -422,10 +422,14 @@ int Engine::sort_launch_class() const {
   return total_cap_ < (8 << 20) ? 0 : total_cap_ < (128 << 20) ? 1 : 2;
 }
 
+// Size class of the MTF-run knob: small batches need every segment's wave
+// to fill the device, large ones can take longer runs.
+int Engine::mtf_run_class() const { return total_cap_ < (32 << 20) ? 0 : 1; }
+
 bool Engine::will_tune() const {
   Tuner& tu = Tuner::get();
   return shape_tuning_pending() || tu.exploring(TUNE_SORT_LAUNCH, sort_launch_class()) ||
-         tu.exploring(TUNE_MTF_RUN, 0);
+         tu.exploring(TUNE_MTF_RUN, mtf_run_class());
 }
 
 int Engine::sort_round0(hipStream_t s) {
@@ -665,8 +669,9 @@ int Engine::run(hipStream_t s, bool assemble, StageTimes* times) {
   // MTF segments per wave: an autotuner knob, timed over the MTF + RLE2
   // stage (unit: capacity tiles) while it is explored
   Tuner& tu = Tuner::get();
-  const int mv = tu.pick(TUNE_MTF_RUN, 0);
-  const bool mtf_timing = tu.exploring(TUNE_MTF_RUN, 0);
+  const int mcls = mtf_run_class();
+  const int mv = tu.pick(TUNE_MTF_RUN, mcls);
+  const bool mtf_timing = tu.exploring(TUNE_MTF_RUN, mcls);
   d_.mtf_run = tu.value(TUNE_MTF_RUN, mv);
   if (times || mtf_timing) BZG_TRY(hipEventRecord(ev_[2], s));
   rc = run_stage(1, s, &Engine::run_mtf_rle2);
@@ -702,7 +707,7 @@ int Engine::run(hipStream_t s, bool assemble, StageTimes* times) {
     float ms = 0.f;
     BZG_TRY(hipEventSynchronize(ev_[3]));
     if (hipEventElapsedTime(&ms, ev_[2], ev_[3]) == hipSuccess)
-      tu.report(TUNE_MTF_RUN, 0, mv, (double)ms * 1e6, (double)d_.ntile_c);
+      tu.report(TUNE_MTF_RUN, mcls, mv, (double)ms * 1e6, (double)d_.ntile_c);
   }
   return 0;
 }

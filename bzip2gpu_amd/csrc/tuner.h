@@ -22,7 +22,8 @@ enum TuneKnob : int {
                          // grids (0) or device-driven, later rounds on
                          // persistent grids of one (1) or four (2) residencies
   TUNE_MTF_RUN = 2,      // consecutive 4096-symbol MTF segments per wave
-                         // (1, 2, 4, 8), one size class
+                         // (1, 2, 4, 8); classes: under 32 MiB of block
+                         // capacity (0), larger (1)
   TUNE_KNOBS = 3
 };
 constexpr int kTuneMaxVariants = 4;

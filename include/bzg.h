@@ -207,8 +207,9 @@ int64_t bzg_graph_replays(void);
  * Knob 2 is the number of consecutive 4096-symbol MTF segments one wave
  * takes, variants 0/1/2/3 = 1/2/4/8 (the list state entering the first
  * segment of a run is computed from the segment summaries, later segments
- * continue from the previous one), one size class, chosen from the MTF +
- * RLE2 stage's time per capacity tile.
+ * continue from the previous one), per size class (0: under 32 MiB of
+ * block capacity, 1: larger), chosen from the MTF + RLE2 stage's time per
+ * capacity tile.
  * Batched calls take their sub-batches one at a time while any knob is
  * being timed.
  * A choice is made after timing each variant twice (the least total time
