@@ -208,6 +208,7 @@ __global__ void __launch_bounds__(kScanWG) k_plan_scan2(Dev d) {
     d.ctr[CTR_LELEMS] = tots[3];
     d.ctr[CTR_NSEG] = nseg;
     d.ctr[CTR_ROUND] = r;
+    d.ctr[CTR_RKN] = 0;  // blocks flagged for rank building count again
     if (nseg > 0) d.ctr[CTR_RAN] = r + 1;
     if (r < kMaxSortRounds) {  // rows in small groups, then in large groups
       d.round_rows[r] = nseg > 0 ? tots[0] : 0;
@@ -281,6 +282,22 @@ void launch_bwt_plan(const Dev& d, hipStream_t s) {
 
 __device__ inline int round_packs(const Dev& d) { return (ctr_get(d, CTR_SMALL) + kHalf - 1) / kHalf; }
 
+// packs whose groups are all at most this long are sorted by counting
+constexpr int kCountMax = 64;
+
+// Ranks are kept only for blocks that need them.  Round 0 sorts by the text
+// and rounds 1-2 sort the packs of small groups (at most kCountMax rows per
+// group) by text keys too, so a block needs its ranks only when one of its
+// groups takes the radix paths (more than kCountMax rows: a pack sorted by
+// composite keys, or the large-group tiles), when it shares such a pack, or
+// from round 3 on.  Such a block's ranks are then built from its suffix
+// array and the round's group list (rank = the row's group start, or the row
+// itself outside every group) and kept up to date by the later rounds.
+__device__ inline void rk_need(const Dev& d, int b) {
+  if (d.st[b].rk == 0 && atomicCAS(&d.st[b].rk, 0, 1) == 0) atomicAdd(&d.ctr[CTR_RKN], 1);
+}
+__device__ inline bool rk_kept(const Dev& d, int b) { return d.st[b].rk == 2; }
+
 // pack k = small groups whose pa lies in [k*kHalf, (k+1)*kHalf)
 __global__ void __launch_bounds__(256) k_mk_packs(Dev d) {
   const int npacks = round_packs(d), nseg = ctr_get(d, CTR_NSEG);
@@ -300,10 +317,11 @@ __global__ void __launch_bounds__(256) k_mk_packs(Dev d) {
 }
 
 __global__ void __launch_bounds__(256) k_mk_large(Dev d) {
-  const int cur = ctr_get(d, CTR_ROUND) & 1, nseg = ctr_get(d, CTR_NSEG);
+  const int round = ctr_get(d, CTR_ROUND), cur = round & 1, nseg = ctr_get(d, CTR_NSEG);
   if (blockIdx.x == 0 && threadIdx.x == 0) d.ctr[cur ^ 1] = 0;  // next list starts empty
   for (int i = blockIdx.x * 256 + threadIdx.x; i < nseg; i += gridDim.x * 256) {
     Seg g = d.seg[cur][i];
+    if (round >= 1 && (round >= 3 || g.len > kCountMax)) rk_need(d, g.blk);
     if (g.len <= kHalf) continue;
     d.llist[d.pc[i]] = i;
     int nt = tiles_for(g.len), t0 = d.pb[i];
@@ -326,8 +344,6 @@ struct PackLds {
   int32_t tmp[8];
 };
 
-// packs whose groups are all at most this long are sorted by counting
-constexpr int kCountMax = 64;
 
 // Loads the small groups of pack k into LDS (a pack holds <= 1023 groups:
 // each has >= 2 rows and the pack <= kPack-1 rows).
@@ -396,18 +412,42 @@ __device__ inline uint32_t text_key(const uint8_t* T, uint32_t q, uint32_t m) {
   return __builtin_bswap32(x);
 }
 
+// The rotation's 8 bytes at start q, big-endian, for a block of m > 8 bytes.
+__device__ inline uint64_t text_key8(const uint8_t* T, uint32_t q, uint32_t m) {
+  const uint32_t w = q >> 2;
+  uint32_t lo, hi;  // bytes q..q+3 and q+4..q+7, little-endian
+  if (4 * w + 12 <= m) {  // the three words inside the block
+    const uint32_t* T32 = reinterpret_cast<const uint32_t*>(T);
+    const uint32_t a = T32[w], b = T32[w + 1], c = T32[w + 2];
+    lo = __builtin_amdgcn_alignbyte(b, a, q & 3u);
+    hi = __builtin_amdgcn_alignbyte(c, b, q & 3u);
+  } else {
+    lo = hi = 0;
+    uint32_t p = q;
+    for (int k = 0; k < 8; k++) {
+      const uint32_t v = (uint32_t)T[p] << (8 * (k & 3));
+      if (k < 4) lo |= v; else hi |= v;
+      p = p + 1 == m ? 0 : p + 1;
+    }
+  }
+  return ((uint64_t)__builtin_bswap32(lo) << 32) | __builtin_bswap32(hi);
+}
+
 // Rounds r >= 1: key(i) = RK[(SA[i] + P) mod m] for every row of every
 // listed group (P >= m: identical rotations, ordered by start index).
 // Item w < npacks is a small-group pack, the others large-group tiles.
-// In round 1 (P = 4) the ranks are those of round 0, which order the
+// In round 1 (P = 4) the ranks would be those of round 0, which order the
 // rotations by their first 4 bytes, so the 4 text bytes at SA + 4 give the
-// same order and the same ties.  Packs sorted by counting (groups of at most
-// kCountMax rows, compared only within a group) take those, and since the
-// text does not change during the round, the sort gathers them itself
-// (`text_keys`): such packs have nothing to do here but mark their blocks'
-// round count.  A block's text is a quarter of its rank array, so the
-// gathers of its rows share cache lines four times as often.
-__device__ inline bool text_keys(int P, const PackLds& L) { return P == 4 && L.max_len <= kCountMax; }
+// same order and the same ties; in round 2 (P = 8) the 8 text bytes at
+// SA + 8 stand for the ranks of round 1 the same way.  Packs sorted by
+// counting (groups of at most kCountMax rows, compared only within a group)
+// take those, and since the text does not change during the round, the
+// sort gathers them itself (`text_keys`): such packs have nothing to do
+// here but mark their blocks' round count.  A block's text is a quarter of
+// its rank array, so the gathers of its rows also share cache lines four
+// times as often, and blocks whose groups all stay that small need no ranks
+// at all.
+__device__ inline bool text_keys(int P, const PackLds& L) { return (P == 4 || P == 8) && L.max_len <= kCountMax; }
 __device__ inline void keys_item(const Dev& d, int cur, int round, int P, int npacks, int w, PackLds& L) {
   const int tid_ = loop_tid();
   if (w < npacks) {
@@ -576,6 +616,12 @@ __device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, i
   if (L.count == 0) return;
   const int total = L.total;
   const bool counting = L.max_len <= kCountMax;
+  // rounds 1-2: counting-sorted packs gather their keys from the text (in
+  // the key fill below), 4 bytes at SA + 4 in round 1, 8 bytes at SA + 8 in
+  // round 2 (the high half in V1, which the counting sort fills only after
+  // its compares; K1 holds the rows' group indices)
+  const bool tk = TEXT && text_keys(P, L);
+  const bool wide = tk && P == 8;
   // group of every row: mark group starts, then a running max over rows
   uint16_t* sid = reinterpret_cast<uint16_t*>(K1);  // free until the first pass
   for (int e = tid; e < kPack; e += 256) sid[e] = 0;
@@ -602,8 +648,6 @@ __device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, i
   // block offset is reloaded only when the block changes, and the rows'
   // loads are all in flight before the LDS writes)
   {
-    // round 1's counting-sorted packs gather their keys from the text here
-    const bool tk = TEXT && text_keys(P, L);
     int cb = -1, cm = 0;
     uint64_t coff = 0;
     uint32_t kv[kPack / 256], sv[kPack / 256];
@@ -631,60 +675,101 @@ __device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, i
         mm[q] = cm;
       }
     }
+    uint32_t kh[kPack / 256];
+#pragma unroll
+    for (int q = 0; q < kPack / 256; q++) kh[q] = 0;
     if (tk) {
 #pragma unroll
       for (int q = 0; q < kPack / 256; q++) {
         if (tid + 256 * q < total) {
-          // key(i) = the 4 bytes at SA + 4 (the start itself when the
+          // key(i) = the P bytes at SA + P (the start itself when the
           // prefix covers the rotation: identical rotations by start)
-          uint32_t p = (sv[q] & kSAMask) + 4u;
-          if (p >= (uint32_t)mm[q]) p -= (uint32_t)mm[q];
-          kv[q] = 4 < mm[q] ? text_key(d.T + oo[q], p, (uint32_t)mm[q]) : (sv[q] & kSAMask);
+          const uint32_t a = sv[q] & kSAMask, cmq = (uint32_t)mm[q];
+          uint32_t p = a + (uint32_t)P;
+          if (p >= cmq) p -= cmq;
+          if (!wide) {
+            kv[q] = 4 < mm[q] ? text_key(d.T + oo[q], p, cmq) : a;
+          } else if (8 < mm[q]) {
+            const uint64_t k8 = text_key8(d.T + oo[q], p, cmq);
+            kv[q] = (uint32_t)k8;
+            kh[q] = (uint32_t)(k8 >> 32);
+          } else {
+            kv[q] = a;
+          }
         }
       }
     }
     // (the counting sort compares keys within a group only: the key alone,
-    // which in round 1 may be 32 text bits)
+    // which in rounds 1-2 may be 32 or 64 text bits)
 #pragma unroll
     for (int q = 0; q < kPack / 256; q++) {
       const int e = tid + 256 * q;
       if (e < total) {
-        K0[e] = counting ? (KT)kv[q] : (((KT)sid[e] << keybits) | (KT)kv[q]);
+        if (counting && sizeof(KT) == 8) K0[e] = (KT)(((uint64_t)kh[q] << 32) | kv[q]);
+        else K0[e] = counting ? (KT)kv[q] : (((KT)sid[e] << keybits) | (KT)kv[q]);
+        if (wide && sizeof(KT) == 4) V1[e] = kh[q];
         V0[e] = sv[q];
       }
     }
   }
+  // 64-bit keys of the counting sort: K0 alone, or V1 (high) and K0 (low)
+  const bool split = wide && sizeof(KT) == 4;
+  auto key_at = [&](int i) -> uint64_t {
+    return split ? (((uint64_t)V1[i] << 32) | (uint32_t)K0[i]) : (uint64_t)K0[i];
+  };
   __syncthreads();
   if (counting) {
     // Small groups only: every row's place in its group by counting (rows
     // with a smaller key, and equal keys before it); equal keys stay together
     // in input order and form the next round's subgroup.
     int cb = -1;
+    bool ck = false;  // the row's block keeps ranks
     uint64_t coff = 0;
-    for (int e = tid; e < total; e += 256) {
+    uint32_t vres[kPack / 256];  // subgroup of each row, to V1 after the compares
+#pragma unroll
+    for (int r = 0; r < kPack / 256; r++) {
+      const int e = tid + 256 * r;
+      vres[r] = 0;
+      if (e >= total) continue;
       const int sg = sid[e];
       const int gs = L.lofs[sg];
       const int ge = sg + 1 < L.count ? L.lofs[sg + 1] : total;
-      const KT my = K0[e];
       int lt = 0, eq = 0, eqb = 0;
-      for (int j = gs; j < ge; j++) {
-        const KT kj = K0[j];
-        lt += kj < my ? 1 : 0;
-        const int same = kj == my ? 1 : 0;
-        eq += same;
-        eqb += (j < e) ? same : 0;
+      if (split) {
+        const uint64_t my = key_at(e);
+        for (int j = gs; j < ge; j++) {
+          const uint64_t kj = key_at(j);
+          lt += kj < my ? 1 : 0;
+          const int same = kj == my ? 1 : 0;
+          eq += same;
+          eqb += (j < e) ? same : 0;
+        }
+      } else {
+        const KT my = K0[e];
+        for (int j = gs; j < ge; j++) {
+          const KT kj = K0[j];
+          lt += kj < my ? 1 : 0;
+          const int same = kj == my ? 1 : 0;
+          eq += same;
+          eqb += (j < e) ? same : 0;
+        }
       }
       const int b = L.blk[sg];
       if (b != cb) {
         cb = b;
         coff = d.desc[b].off;
+        ck = rk_kept(d, b);
       }
       const int st0 = L.start[sg];
       const uint32_t v = V0[e];
       d.SA[coff + st0 + lt + eqb] = v;
-      if (lt != 0 || !keep_first) d.RK[coff + (v & kSAMask)] = (uint32_t)(st0 + lt);
-      V1[e] = (eq >= 2 && eqb == 0) ? (((uint32_t)lt << 16) | (uint32_t)eq) : 0u;
+      if (ck && (lt != 0 || !keep_first)) d.RK[coff + (v & kSAMask)] = (uint32_t)(st0 + lt);
+      vres[r] = (eq >= 2 && eqb == 0) ? (((uint32_t)lt << 16) | (uint32_t)eq) : 0u;
     }
+    __syncthreads();  // every row's compares are done (V1 may hold key halves)
+#pragma unroll
+    for (int r = 0; r < kPack / 256; r++)
+      if (tid + 256 * r < total) V1[tid + 256 * r] = vres[r];
     __syncthreads();
     // subgroups of >= 2 rows, emitted by their first row, in row order
     const int e0 = tid * (kPack / 256);
@@ -784,6 +869,7 @@ __device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, i
   uint32_t endm = 0;
   int nout = 0;
   int cb = -1;
+  bool ck = false;  // the row's block keeps ranks
   uint64_t coff = 0;
 #pragma unroll
   for (int q = 0; q < 8; q++) {
@@ -794,13 +880,14 @@ __device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, i
       if (b != cb) {
         cb = b;
         coff = d.desc[b].off;
+        ck = rk_kept(d, b);
       }
       const uint64_t off = coff;
       int j = L.start[sg] + (e - L.lofs[sg]);
       int hj = L.start[sg] + (head_of[q] - L.lofs[sg]);
       uint32_t v = Vs[e];
       d.SA[off + j] = v;
-      if (head_of[q] != L.lofs[sg] || !keep_first) d.RK[off + (v & kSAMask)] = (uint32_t)hj;
+      if (ck && (head_of[q] != L.lofs[sg] || !keep_first)) d.RK[off + (v & kSAMask)] = (uint32_t)hj;
       // a group of >= 2 rows ends here
       if (((e == total - 1) || (kk[q + 1] != kk[q])) && e > head_of[q]) {
         endm |= 1u << q;
@@ -833,7 +920,7 @@ __device__ inline void sort_small_loop(const Dev& d, int keybits) {
   const int round = ctr_get(d, CTR_ROUND), cur = round & 1;
   for (ItemLoop it(round_packs(d)); it.i < it.end; it.i += it.step) {
     __syncthreads();  // the previous pack's LDS is no longer read
-    if (round == 1) sort_small_body<KT, true>(d, cur, true, round_prefix(1), keybits, it.i, S);
+    if (round == 1 || round == 2) sort_small_body<KT, true>(d, cur, true, round_prefix(round), keybits, it.i, S);
     else sort_small_body<KT, false>(d, cur, round > 0, 0, keybits, it.i, S);
   }
 }
@@ -1126,6 +1213,11 @@ __device__ inline void lsum_item(const Dev& d, int cur, int fin, int t) {
     for (int w = 1; w < 4; w++) { f = min(f, tmp[w]); l2 = max(l2, tmp[4 + w]); }
     d.lsum[2 * t] = f;
     d.lsum[2 * t + 1] = l2;
+    // Round 0: a whole tile without a group start is a group of more than
+    // kTile rows, which round 1 sorts by ranks (the tile path), so the block
+    // keeps its ranks from round 0 on (the heads below write them); shorter
+    // long groups are found when round 1 is planned.
+    if (f == 0x7fffffff && n == kTile && ctr_get(d, CTR_ROUND) == 0) d.st[g.blk].rk = 2;
   }
 }
 
@@ -1249,8 +1341,9 @@ __device__ inline void lheads_item(const Dev& d, int cur, int fin, int t) {
   }
   // Rank updates RK[start] = head go to the block's buckets of kRkBucket
   // starts (k_rk_apply writes them), so that a bucket's updates land in one
-  // 32 KiB window that is staged in LDS.
-  {
+  // 32 KiB window that is staged in LDS; only blocks that keep ranks (none
+  // in round 0, whose rows are ranked later only if their block needs it).
+  if (u32(d.st[g.blk].rk) == 2) {
     const int tile_c0 = u32(d.desc[g.blk].tile_c0);
     const int nbk = (u32(d.st[g.blk].m) + kRkBucket - 1) / kRkBucket;
     for (int j = tid; j < nbk; j += NT) bcnt[j] = 0;
@@ -1326,11 +1419,13 @@ __global__ void __launch_bounds__(256) k_rk_zero(Dev d) {
 
 // A bucket's updates are applied to an LDS copy of its window of RK, then
 // the touched entries are written out in order.
-__device__ inline void rk_apply_item(const Dev& d, int t) {
+// (build: only the buckets of the blocks whose ranks are being built)
+__device__ inline void rk_apply_item(const Dev& d, int t, bool build) {
   const int tid_ = loop_tid();
   __shared__ uint32_t win[kRkBucket];
   __shared__ uint8_t touched[kRkBucket];
   const int b = u32(d.tc_blk[t]);
+  if (build && u32(d.st[b].rk) != 1) return;
   const int k = t - u32(d.desc[b].tile_c0);
   if (k % (kRkBucket / kTile)) return;
   const int cnt = u32(d.rkb_cnt[t]);
@@ -1359,13 +1454,151 @@ __device__ inline void rk_apply_item(const Dev& d, int t) {
     if (touched[i]) d.RK[boff + base + i] = win[i];
 }
 
-__global__ void __launch_bounds__(256) k_rk_apply(Dev d) {
-  if (ctr_get(d, CTR_LTILES) == 0) return;
+// build = 0: the round's large-group heads; 1: blocks whose ranks are built
+__global__ void __launch_bounds__(256) k_rk_apply(Dev d, int build) {
+  if (build ? ctr_get(d, CTR_RKN) == 0 : ctr_get(d, CTR_LTILES) == 0) return;
   // a bucket covers kRkBucket / kTile capacity tiles; its first tile applies it
   for (ItemLoop it(d.ntile_c); it.i < it.end; it.i += it.step) {
     __syncthreads();
-    rk_apply_item(d, it.i);
+    rk_apply_item(d, it.i, build != 0);
   }
+}
+
+// ---------------------------------------------------------- rank building
+// Rounds 1-2: a pack with a group of more than kCountMax rows is sorted by
+// composite rank keys, so every block with a group in it needs its ranks
+// (from round 3 on, k_mk_large flags every listed block).
+__global__ void __launch_bounds__(256) k_pack_flags(Dev d) {
+  __shared__ int32_t smax[4];
+  const int round = ctr_get(d, CTR_ROUND), cur = round & 1;
+  if (round < 1 || round >= 3) return;
+  const int tid = loop_tid();
+  for (ItemLoop it(round_packs(d)); it.i < it.end; it.i += it.step) {
+    const int i0 = u32(d.packs[it.i]), i1 = u32(d.packs[it.i + 1]);
+    int mx = 0;
+    for (int i = i0 + tid; i < i1; i += 256) {
+      const int len = d.seg[cur][i].len;
+      if (len <= kHalf) mx = max(mx, len);
+    }
+    mx = wave_max(mx);
+    __syncthreads();  // the previous item's maxima are read
+    if ((tid & 63) == 0) smax[tid >> 6] = mx;
+    __syncthreads();
+    if (max(max(smax[0], smax[1]), max(smax[2], smax[3])) > kCountMax) {
+      for (int i = i0 + tid; i < i1; i += 256) {
+        const Seg g = d.seg[cur][i];
+        if (g.len <= kHalf) rk_need(d, g.blk);
+      }
+    }
+  }
+}
+
+// Bucket counters of the blocks whose ranks are built.
+__global__ void __launch_bounds__(256) k_rkb_zero(Dev d) {
+  if (ctr_get(d, CTR_RKN) == 0) return;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < d.ntile_c; i += gridDim.x * 256)
+    if (d.st[d.tc_blk[i]].rk == 1) d.rkb_cnt[i] = 0;
+}
+
+// A block's ranks are built in three steps: every row's rank in a row-order
+// array H (KEY2, free between rounds) is the row itself (k_rk_self), then
+// the group start for the rows of the round's groups (k_rk_groups, writes
+// contiguous per group), and RK[SA[i]] = H[i] goes through the rank buckets
+// as the group heads' updates do (k_rk_rows, then k_rk_apply).
+__global__ void __launch_bounds__(256) k_rk_self(Dev d) {
+  if (ctr_get(d, CTR_RKN) == 0) return;
+  for (ItemLoop it(d.ntile_c); it.i < it.end; it.i += it.step) {
+    const int b = u32(d.tc_blk[it.i]);
+    if (u32(d.st[b].rk) != 1) continue;
+    const uint64_t off = u64(d.desc[b].off);
+    const int m = u32(d.st[b].m);
+    const int start = (it.i - u32(d.desc[b].tile_c0)) * kTile;
+    for (int e = (int)threadIdx.x; e < kTile && start + e < m; e += 256) d.KEY2[off + start + e] = (uint32_t)(start + e);
+  }
+}
+
+__device__ inline void rk_rows_item(const Dev& d, int t) {
+  const int tid = loop_tid();
+  __shared__ int32_t bcnt[(kStageMax + kRkBucket - 1) / kRkBucket];
+  const int b = u32(d.tc_blk[t]);
+  if (u32(d.st[b].rk) != 1) return;
+  const uint64_t off = u64(d.desc[b].off);
+  const int tile_c0 = u32(d.desc[b].tile_c0);
+  const int m = u32(d.st[b].m);
+  const int start = (t - tile_c0) * kTile;
+  if (start >= m) return;
+  const int n = min(kTile, m - start);
+  const int nbk = (m + kRkBucket - 1) / kRkBucket;
+  for (int j = tid; j < nbk; j += 256) bcnt[j] = 0;
+  uint32_t sv[kTile / 256], hv[kTile / 256];
+#pragma unroll
+  for (int q = 0; q < kTile / 256; q++) {
+    const int e = tid + 256 * q;
+    sv[q] = e < n ? d.SA[off + start + e] & kSAMask : 0u;
+    hv[q] = e < n ? d.KEY2[off + start + e] : 0u;
+  }
+  __syncthreads();
+  uint32_t lr[kTile / 256];
+#pragma unroll
+  for (int q = 0; q < kTile / 256; q++)
+    lr[q] = tid + 256 * q < n ? (uint32_t)atomicAdd(&bcnt[sv[q] / kRkBucket], 1) : 0u;
+  __syncthreads();
+  for (int j = tid; j < nbk; j += 256) {
+    const int c = bcnt[j];
+    if (c) bcnt[j] = atomicAdd(&d.rkb_cnt[tile_c0 + j * (kRkBucket / kTile)], c);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kTile / 256; q++) {
+    const int e = tid + 256 * q;
+    if (e < n) {
+      const int bk = (int)(sv[q] / kRkBucket);
+      d.RKB[off + (uint64_t)bk * kRkBucket + bcnt[bk] + lr[q]] = ((uint64_t)hv[q] << 32) | sv[q];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_rk_rows(Dev d) {
+  if (ctr_get(d, CTR_RKN) == 0) return;
+  for (ItemLoop it(d.ntile_c); it.i < it.end; it.i += it.step) {
+    __syncthreads();  // the previous tile's counters are no longer read
+    rk_rows_item(d, it.i);
+  }
+}
+
+// The rows of the round's groups in blocks being built take their group's
+// start in H (small groups one per thread, large groups by tile).
+__global__ void __launch_bounds__(256) k_rk_groups(Dev d) {
+  if (ctr_get(d, CTR_RKN) == 0) return;
+  const int cur = ctr_get(d, CTR_ROUND) & 1, nseg = ctr_get(d, CTR_NSEG), ntl = ctr_get(d, CTR_LTILES);
+  const int nch = (nseg + 255) / 256;
+  for (ItemLoop it(nch + ntl); it.i < it.end; it.i += it.step) {
+    if (it.i < nch) {
+      const int i = it.i * 256 + (int)threadIdx.x;
+      if (i < nseg) {
+        const Seg g = d.seg[cur][i];
+        if (g.len <= kHalf && d.st[g.blk].rk == 1) {
+          const uint64_t off = d.desc[g.blk].off;
+          for (int j = 0; j < g.len; j++) d.KEY2[off + g.start + j] = (uint32_t)g.start;
+        }
+      }
+    } else {
+      const int t = it.i - nch;
+      const Seg g = seg_get(&d.seg[cur][u32(d.ltile_seg[t])]);
+      if (u32(d.st[g.blk].rk) != 1) continue;
+      const int pos = u32(d.ltile_pos[t]);
+      const uint64_t off = u64(d.desc[g.blk].off);
+      const int n = min(kTile, g.len - pos);
+      for (int e = (int)threadIdx.x; e < n; e += 256) d.KEY2[off + g.start + pos + e] = (uint32_t)g.start;
+    }
+  }
+}
+
+// then those blocks keep their ranks
+__global__ void __launch_bounds__(256) k_rk_kept(Dev d) {
+  if (ctr_get(d, CTR_RKN) == 0) return;
+  for (int b = blockIdx.x * 256 + threadIdx.x; b < d.nblk; b += gridDim.x * 256)
+    if (d.st[b].rk == 1) d.st[b].rk = 2;
 }
 
 void launch_bwt_init(const Dev& d, hipStream_t s) {
@@ -1396,6 +1629,7 @@ SortGrids sort_grids(int device) {
   g.scatter[2] = grid_for(k_lscatter<1024, false>, 1024, cus);
   g.heads = grid_for(k_lheads<kHeadsNT>, kHeadsNT, cus);
   g.apply = grid_for(k_rk_apply, 256, cus);
+  g.apply = std::min(g.apply, grid_for(k_rk_rows, 256, cus));
   return g;
 }
 
@@ -1429,6 +1663,17 @@ void launch_bwt_round(const Dev& d, const SortGrids& g, bool first, int late_bit
   auto grid = [&](int persistent, int items) { return sized ? round8(items) : persistent; };
   k_mk_packs<<<sized ? (npacks + 1 + 255) / 256 : g.light, 256, 0, s>>>(d);
   k_mk_large<<<sized ? std::max(1, (nseg + 255) / 256) : g.light, 256, 0, s>>>(d);
+  if (!first) {
+    // ranks of the blocks that need them from this round on (the kernels
+    // return at once when no block is flagged)
+    k_pack_flags<<<g.keys, 256, 0, s>>>(d);
+    k_rkb_zero<<<g.light, 256, 0, s>>>(d);
+    k_rk_self<<<g.keys, 256, 0, s>>>(d);
+    k_rk_groups<<<g.keys, 256, 0, s>>>(d);
+    k_rk_rows<<<g.apply, 256, 0, s>>>(d);
+    k_rk_apply<<<g.apply, 256, 0, s>>>(d, 1);
+    k_rk_kept<<<std::max(1, std::min(g.light, (d.nblk + 255) / 256)), 256, 0, s>>>(d);
+  }
   if (!first && (!exact || npacks + ntiles > 0)) {  // round-0 keys come from the text
     probe_begin("k_keys", s);
     k_keys<<<grid(g.keys, npacks + ntiles), 256, 0, s>>>(d);
@@ -1474,7 +1719,7 @@ void launch_bwt_round(const Dev& d, const SortGrids& g, bool first, int late_bit
   probe_begin("k_lheads", s);
   k_lheads<kHeadsNT><<<grid(g.heads, ntiles), kHeadsNT, 0, s>>>(d, fin);
   probe_end("k_lheads", s);
-  k_rk_apply<<<grid(g.apply, d.ntile_c), 256, 0, s>>>(d);
+  k_rk_apply<<<grid(g.apply, d.ntile_c), 256, 0, s>>>(d, 0);
 }
 
 }  // namespace bzg

@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(256) k_block_init(Dev d) {
   if (threadIdx.x == 0) {
     st->crc = 0; st->m = 0; st->s = 0; st->orig_ptr = 0; st->n_in_use = 0;
     st->n_groups = 0; st->n_sel = 0; st->rounds = 0; st->bits = 0;
-    st->hdr_bits = 0; st->bit_off = 0; st->err = 0; st->pad_ = 0;
+    st->hdr_bits = 0; st->bit_off = 0; st->err = 0; st->rk = 0;
   }
   if (threadIdx.x < 8) st->present[threadIdx.x] = 0;
   for (int i = threadIdx.x; i < 260; i += blockDim.x) d.freq[(int64_t)b * 260 + i] = 0;

@@ -116,6 +116,7 @@ enum : int {
   CTR_LELEMS = 7,  // total elements in large groups
   CTR_ROUND = 8,   // index of the current suffix-sort round (-1 before round 0)
   CTR_RAN = 9,     // rounds that had groups to sort
+  CTR_RKN = 10,    // blocks whose ranks are built this round
   CTR_N = 16
 };
 
