@@ -166,6 +166,29 @@ def test_mtf_segment_runs_identical(dev, variant):
         bz.tune_force(2, -1)
 
 
+@pytest.mark.parametrize("policy", [0, 1])
+def test_rank_building_paths(dev, policy):
+    """Blocks whose suffix-sort ranks are never built (text-like, groups of
+    at most 64 rows after round 0), built in round 1 (groups of about 1000
+    rows from a 1000-byte period), kept from round 0 (tile-long groups from
+    a 2-byte period), built in round 3 or later (a 65536-byte period with
+    mutations), and small blocks with long groups, in one batch: the oracle's
+    stream with both launch policies."""
+    import bzip2gpu_amd as bz
+    import bzip2gpu_amd.datagen as dg
+    parts = [dg.mix(1 << 20, 82), dg.periodic(900000, 83, 1000, 0.001), dg.periodic(900000, 84, 2, 0.001),
+             dg.periodic(900000, 85, 65536, 0.001), np.zeros(3000, np.uint8), dg.periodic(1000, 86, 3, 0.0),
+             dg.mix(700000, 87)]
+    data = np.concatenate(parts).tobytes()
+    want = oracle.compress(data, 9, threads=8)
+    try:
+        bz.tune_force(1, policy)
+        _case(f"rank building paths, sort policy {policy}")
+        _same(bz.compress(data, 9), want, f"policy {policy}")
+    finally:
+        bz.tune_force(1, -1)
+
+
 @pytest.mark.parametrize("policy", [0, 1, 2])
 def test_sort_launch_policies_identical(dev, policy):
     """Both launch policies of the suffix-sort rounds (per round with exact
