@@ -294,7 +294,15 @@ constexpr int kCountMax = 64;
 // array and the round's group list (rank = the row's group start, or the row
 // itself outside every group) and kept up to date by the later rounds.
 __device__ inline void rk_need(const Dev& d, int b) {
-  if (d.st[b].rk == 0 && atomicCAS(&d.st[b].rk, 0, 1) == 0) atomicAdd(&d.ctr[CTR_RKN], 1);
+  if (d.st[b].rk == 0 && atomicCAS(&d.st[b].rk, 0, 1) == 0) d.rk_list[atomicAdd(&d.ctr[CTR_RKN], 1)] = b;
+}
+// Work items of the blocks being built: (list entry, capacity tile of that
+// block); items past a block's last tile return -1.
+__device__ inline int rk_items(const Dev& d) { return ctr_get(d, CTR_RKN) * d.tiles_max; }
+__device__ inline int rk_item_tile(const Dev& d, int item) {
+  const int b = u32(d.rk_list[item / d.tiles_max]), k = item % d.tiles_max;
+  const BlockDesc& D = d.desc[b];
+  return k < tiles_for(u32(D.cap)) ? u32(D.tile_c0) + k : -1;
 }
 __device__ inline bool rk_kept(const Dev& d, int b) { return d.st[b].rk == 2; }
 
@@ -1458,9 +1466,10 @@ __device__ inline void rk_apply_item(const Dev& d, int t, bool build) {
 __global__ void __launch_bounds__(256) k_rk_apply(Dev d, int build) {
   if (build ? ctr_get(d, CTR_RKN) == 0 : ctr_get(d, CTR_LTILES) == 0) return;
   // a bucket covers kRkBucket / kTile capacity tiles; its first tile applies it
-  for (ItemLoop it(d.ntile_c); it.i < it.end; it.i += it.step) {
+  for (ItemLoop it(build ? rk_items(d) : d.ntile_c); it.i < it.end; it.i += it.step) {
     __syncthreads();
-    rk_apply_item(d, it.i, build != 0);
+    const int t = build ? rk_item_tile(d, it.i) : it.i;
+    if (t >= 0) rk_apply_item(d, t, build != 0);
   }
 }
 
@@ -1495,9 +1504,11 @@ __global__ void __launch_bounds__(256) k_pack_flags(Dev d) {
 
 // Bucket counters of the blocks whose ranks are built.
 __global__ void __launch_bounds__(256) k_rkb_zero(Dev d) {
-  if (ctr_get(d, CTR_RKN) == 0) return;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < d.ntile_c; i += gridDim.x * 256)
-    if (d.st[d.tc_blk[i]].rk == 1) d.rkb_cnt[i] = 0;
+  const int n = ctr_get(d, CTR_RKN);
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n * d.tiles_max; i += gridDim.x * 256) {
+    const int b = d.rk_list[i / d.tiles_max], k = i % d.tiles_max;
+    if (k < tiles_for(d.desc[b].cap)) d.rkb_cnt[d.desc[b].tile_c0 + k] = 0;
+  }
 }
 
 // A block's ranks are built in three steps: every row's rank in a row-order
@@ -1507,12 +1518,13 @@ __global__ void __launch_bounds__(256) k_rkb_zero(Dev d) {
 // as the group heads' updates do (k_rk_rows, then k_rk_apply).
 __global__ void __launch_bounds__(256) k_rk_self(Dev d) {
   if (ctr_get(d, CTR_RKN) == 0) return;
-  for (ItemLoop it(d.ntile_c); it.i < it.end; it.i += it.step) {
-    const int b = u32(d.tc_blk[it.i]);
-    if (u32(d.st[b].rk) != 1) continue;
+  for (ItemLoop it(rk_items(d)); it.i < it.end; it.i += it.step) {
+    const int t = rk_item_tile(d, it.i);
+    if (t < 0) continue;
+    const int b = u32(d.tc_blk[t]);
     const uint64_t off = u64(d.desc[b].off);
     const int m = u32(d.st[b].m);
-    const int start = (it.i - u32(d.desc[b].tile_c0)) * kTile;
+    const int start = (t - u32(d.desc[b].tile_c0)) * kTile;
     for (int e = (int)threadIdx.x; e < kTile && start + e < m; e += 256) d.KEY2[off + start + e] = (uint32_t)(start + e);
   }
 }
@@ -1560,9 +1572,10 @@ __device__ inline void rk_rows_item(const Dev& d, int t) {
 
 __global__ void __launch_bounds__(256) k_rk_rows(Dev d) {
   if (ctr_get(d, CTR_RKN) == 0) return;
-  for (ItemLoop it(d.ntile_c); it.i < it.end; it.i += it.step) {
+  for (ItemLoop it(rk_items(d)); it.i < it.end; it.i += it.step) {
     __syncthreads();  // the previous tile's counters are no longer read
-    rk_rows_item(d, it.i);
+    const int t = rk_item_tile(d, it.i);
+    if (t >= 0) rk_rows_item(d, t);
   }
 }
 
@@ -1596,9 +1609,8 @@ __global__ void __launch_bounds__(256) k_rk_groups(Dev d) {
 
 // then those blocks keep their ranks
 __global__ void __launch_bounds__(256) k_rk_kept(Dev d) {
-  if (ctr_get(d, CTR_RKN) == 0) return;
-  for (int b = blockIdx.x * 256 + threadIdx.x; b < d.nblk; b += gridDim.x * 256)
-    if (d.st[b].rk == 1) d.st[b].rk = 2;
+  const int n = ctr_get(d, CTR_RKN);
+  for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) d.st[d.rk_list[j]].rk = 2;
 }
 
 void launch_bwt_init(const Dev& d, hipStream_t s) {

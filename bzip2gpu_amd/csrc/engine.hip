@@ -349,6 +349,7 @@ int Engine::setup(const uint8_t* d_in, hipStream_t s, uint8_t* out) {
     d.pc = c.take<int32_t>(seg_cap);
     d.packs = c.take<int32_t>(pack_cap + 2);
     d.horder = c.take<int32_t>(B + 2);
+    d.rk_list = c.take<int32_t>(B + 2);
     d.ltile_seg = c.take<int32_t>(ltile_cap);
     d.ltile_pos = c.take<int32_t>(ltile_cap);
     d.llist = c.take<int32_t>(ltile_cap);
@@ -374,7 +375,7 @@ int Engine::setup(const uint8_t* d_in, hipStream_t s, uint8_t* out) {
   d.ntile_n = (int32_t)NTn;
   d.ntile_c = (int32_t)NTc;
   d.mtf_run = 4;
-  d.mtf_pad = 0;
+  d.tiles_max = tiles_for(max_cap_);
   d.total_cap = CAP;
   d.hdr_stride = hdr_stride;
   d.seg_cap = seg_cap;

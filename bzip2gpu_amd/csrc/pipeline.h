@@ -38,7 +38,8 @@ struct Dev {
   const uint8_t* in;
   int32_t nblk, nstream;
   int32_t ntile_n, ntile_c;
-  int32_t mtf_run, mtf_pad;  // MTF segments per wave (1, 2, 4 or 8); padding
+  int32_t mtf_run;   // MTF segments per wave (1, 2, 4 or 8)
+  int32_t tiles_max; // capacity tiles of the largest block
   int64_t total_cap;
   // tile -> block maps
   const int32_t* tn_blk;
@@ -83,6 +84,7 @@ struct Dev {
   int32_t* pc;         // per-seg exclusive prefix of large-group count
   int32_t* packs;      // pack -> first seg index
   int32_t* horder;     // blocks in code-length build order (largest alphabet first)
+  int32_t* rk_list;    // blocks whose suffix-sort ranks are built this round
   int32_t* ltile_seg;  // large tile -> seg index
   int32_t* ltile_pos;  // large tile -> first element within its seg
   int32_t* llist;      // compact list of large seg indices
