@@ -134,9 +134,12 @@ __global__ void __launch_bounds__(256) k_bwt_init(Dev d) {
   }
   if (t == 0 && threadIdx.x == 0) {
     d.ctr[CTR_ROUND] = -1;
+    d.ctr[CTR_KEPT] = 0;
     d.ctr[CTR_RAN] = 0;
   }
   if (k == 0 && threadIdx.x == 0) {
+    d.st[b].rk = 0;
+    d.st[b].rk_rows = 0;
     if (m > 1) {
       int idx = atomicAdd(&d.ctr[CTR_LIST0], 1);
       d.seg[0][idx] = Seg{b, 0, m, 0};
@@ -284,6 +287,9 @@ __device__ inline int round_packs(const Dev& d) { return (ctr_get(d, CTR_SMALL) 
 
 // packs whose groups are all at most this long are sorted by counting
 constexpr int kCountMax = 64;
+// round 1 builds the ranks of a block with more than 1/kRkShare of its rows
+// in packs sorted by composite keys (k_pack_flags)
+constexpr int kRkShare = 8;
 
 // Ranks are kept only for blocks that need them.  Round 0 sorts by the text
 // and rounds 1-2 sort the packs of small groups (at most kCountMax rows per
@@ -329,7 +335,10 @@ __global__ void __launch_bounds__(256) k_mk_large(Dev d) {
   if (blockIdx.x == 0 && threadIdx.x == 0) d.ctr[cur ^ 1] = 0;  // next list starts empty
   for (int i = blockIdx.x * 256 + threadIdx.x; i < nseg; i += gridDim.x * 256) {
     Seg g = d.seg[cur][i];
-    if (round >= 1 && (round >= 3 || g.len > kCountMax)) rk_need(d, g.blk);
+    // (round 1 sorts every small group by text keys, round 2 the counting
+    // ones, so ranks are needed for: round 1's large groups, round 2's
+    // groups of more than kCountMax rows, every group from round 3 on)
+    if (round >= 1 && (round >= 3 || g.len > (round == 1 ? kHalf : kCountMax))) rk_need(d, g.blk);
     if (g.len <= kHalf) continue;
     d.llist[d.pc[i]] = i;
     int nt = tiles_for(g.len), t0 = d.pb[i];
@@ -349,17 +358,21 @@ struct PackLds {
   int32_t total;
   int32_t last_len;
   int32_t max_len;  // longest group
+  int32_t no_rank;  // (with rank_state) a group's block keeps no ranks
   int32_t tmp[8];
 };
 
 
 // Loads the small groups of pack k into LDS (a pack holds <= 1023 groups:
 // each has >= 2 rows and the pack <= kPack-1 rows).
-__device__ inline void load_pack(const Dev& d, int cur, int k, PackLds& P) {
+__device__ inline void load_pack(const Dev& d, int cur, int k, PackLds& P, bool rank_state = false) {
   const int tid_ = loop_tid();
   const int i0 = u32(d.packs[k]), i1 = u32(d.packs[k + 1]);
   int count = 0;
-  if (tid_ == 0) P.max_len = 0;
+  if (tid_ == 0) {
+    P.max_len = 0;
+    P.no_rank = 0;
+  }
   for (int base = i0; base < i1; base += 256) {
     int i = base + tid_;
     Seg g{0, 0, 0, 0};
@@ -376,6 +389,10 @@ __device__ inline void load_pack(const Dev& d, int cur, int k, PackLds& P) {
     }
     const int ml = wave_max(small ? g.len : 0);
     if ((tid_ & 63) == 0 && ml > 0) atomicMax(&P.max_len, ml);
+    if (rank_state) {
+      const bool nr = __ballot(small && d.st[g.blk].rk != 2) != 0;
+      if ((tid_ & 63) == 0 && nr) atomicOr(&P.no_rank, 1);
+    }
     count += tot;
     __syncthreads();
   }
@@ -454,12 +471,16 @@ __device__ inline uint64_t text_key8(const uint8_t* T, uint32_t q, uint32_t m) {
 // here but mark their blocks' round count.  A block's text is a quarter of
 // its rank array, so the gathers of its rows also share cache lines four
 // times as often, and blocks whose groups all stay that small need no ranks
-// at all.
-__device__ inline bool text_keys(int P, const PackLds& L) { return (P == 4 || P == 8) && L.max_len <= kCountMax; }
+// at all.  In round 1 the packs sorted by composite keys take text keys
+// too when a block of theirs keeps no ranks (32 key bits instead of the
+// rank bits, in a 64-bit sort of their own).
+__device__ inline bool text_keys(int P, const PackLds& L) {
+  return L.max_len <= kCountMax ? (P == 4 || P == 8) : (P == 4 && L.no_rank);
+}
 __device__ inline void keys_item(const Dev& d, int cur, int round, int P, int npacks, int w, PackLds& L) {
   const int tid_ = loop_tid();
   if (w < npacks) {
-    load_pack(d, cur, w, L);
+    load_pack(d, cur, w, L, P == 4);
     if (L.count == 0) return;
     const int total = L.total;
     for (int j = tid_; j < L.count; j += 256)
@@ -606,8 +627,10 @@ struct SortLds {
 // TEXT: the round may take keys from the text (round 1); the other rounds
 // run a copy without that path (both copies share the LDS in S).
 template <typename KT, bool TEXT>
+// only: 0 every pack, 2 the packs sorted by composite text keys, 1 the
+// others (round 1 splits them between two launches)
 __device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, int P, int keybits, int k,
-                                       SortLds<KT>& S) {
+                                       SortLds<KT>& S, int only) {
   const int tid_ = loop_tid();
   PackLds& L = S.L;
   KT* K0 = S.K0;
@@ -620,7 +643,7 @@ __device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, i
   int32_t* tmp = S.tmp;
   const int tid = tid_, wv = tid >> 6, lane = (tid_ & 63);
   const int nxt = cur ^ 1;
-  load_pack(d, cur, k, L);
+  load_pack(d, cur, k, L, TEXT && P == 4);
   if (L.count == 0) return;
   const int total = L.total;
   const bool counting = L.max_len <= kCountMax;
@@ -629,6 +652,7 @@ __device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, i
   // round 2 (the high half in V1, which the counting sort fills only after
   // its compares; K1 holds the rows' group indices)
   const bool tk = TEXT && text_keys(P, L);
+  if (only != 0 && only != (tk && !counting ? 2 : 1)) return;
   const bool wide = tk && P == 8;
   // group of every row: mark group starts, then a running max over rows
   uint16_t* sid = reinterpret_cast<uint16_t*>(K1);  // free until the first pass
@@ -922,20 +946,31 @@ __device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, i
   }
 }
 
+// Round 1 sorts the packs sorted by composite text keys (32 text bits under
+// the group index) in a second, 64-bit launch (radix1), the others in the
+// round's usual one, so that no block needs ranks in round 1 for its small
+// groups.
 template <typename KT>
-__device__ inline void sort_small_loop(const Dev& d, int keybits) {
+__device__ inline void sort_small_loop(const Dev& d, int keybits, int radix1) {
   __shared__ SortLds<KT> S;
   const int round = ctr_get(d, CTR_ROUND), cur = round & 1;
+  if (radix1 && round != 1) return;
   for (ItemLoop it(round_packs(d)); it.i < it.end; it.i += it.step) {
     __syncthreads();  // the previous pack's LDS is no longer read
-    if (round == 1 || round == 2) sort_small_body<KT, true>(d, cur, true, round_prefix(round), keybits, it.i, S);
-    else sort_small_body<KT, false>(d, cur, round > 0, 0, keybits, it.i, S);
+    if (round == 1)
+      sort_small_body<KT, true>(d, cur, true, 4, radix1 ? 32 : keybits, it.i, S, radix1 ? 2 : 1);
+    else if (round == 2)
+      sort_small_body<KT, true>(d, cur, true, 8, keybits, it.i, S, 0);
+    else
+      sort_small_body<KT, false>(d, cur, round > 0, 0, keybits, it.i, S, 0);
   }
 }
 
-__global__ void __launch_bounds__(256) k_sort_small(Dev d, int keybits) { sort_small_loop<uint32_t>(d, keybits); }
+__global__ void __launch_bounds__(256) k_sort_small(Dev d, int keybits) { sort_small_loop<uint32_t>(d, keybits, 0); }
 
-__global__ void __launch_bounds__(256) k_sort_small64(Dev d, int keybits) { sort_small_loop<uint64_t>(d, keybits); }
+__global__ void __launch_bounds__(256) k_sort_small64(Dev d, int keybits, int radix1) {
+  sort_small_loop<uint64_t>(d, keybits, radix1);
+}
 
 // ----------------------------------------------------------- large groups
 // Digit histogram of a large-group tile (per-wave counters, so that skewed
@@ -1225,7 +1260,8 @@ __device__ inline void lsum_item(const Dev& d, int cur, int fin, int t) {
     // kTile rows, which round 1 sorts by ranks (the tile path), so the block
     // keeps its ranks from round 0 on (the heads below write them); shorter
     // long groups are found when round 1 is planned.
-    if (f == 0x7fffffff && n == kTile && ctr_get(d, CTR_ROUND) == 0) d.st[g.blk].rk = 2;
+    if (f == 0x7fffffff && n == kTile && ctr_get(d, CTR_ROUND) == 0 && atomicExch(&d.st[g.blk].rk, 2) != 2)
+      atomicAdd(&d.ctr[CTR_KEPT], 1);
   }
 }
 
@@ -1462,9 +1498,10 @@ __device__ inline void rk_apply_item(const Dev& d, int t, bool build) {
     if (touched[i]) d.RK[boff + base + i] = win[i];
 }
 
-// build = 0: the round's large-group heads; 1: blocks whose ranks are built
+// build = 0: the round's large-group heads (none to apply while no block
+// keeps ranks); 1: blocks whose ranks are built
 __global__ void __launch_bounds__(256) k_rk_apply(Dev d, int build) {
-  if (build ? ctr_get(d, CTR_RKN) == 0 : ctr_get(d, CTR_LTILES) == 0) return;
+  if (build ? ctr_get(d, CTR_RKN) == 0 : (ctr_get(d, CTR_LTILES) == 0 || ctr_get(d, CTR_KEPT) == 0)) return;
   // a bucket covers kRkBucket / kTile capacity tiles; its first tile applies it
   for (ItemLoop it(build ? rk_items(d) : d.ntile_c); it.i < it.end; it.i += it.step) {
     __syncthreads();
@@ -1474,13 +1511,18 @@ __global__ void __launch_bounds__(256) k_rk_apply(Dev d, int build) {
 }
 
 // ---------------------------------------------------------- rank building
-// Rounds 1-2: a pack with a group of more than kCountMax rows is sorted by
+// Round 2: a pack with a group of more than kCountMax rows is sorted by
 // composite rank keys, so every block with a group in it needs its ranks
-// (from round 3 on, k_mk_large flags every listed block).
+// (from round 3 on, k_mk_large flags every listed block).  Round 1 sorts
+// such packs by text keys, in a 64-bit sort about three times as costly per
+// row as building a block's ranks per block row, and a block with many rows
+// there most likely needs its ranks in a later round: here round 1 counts
+// those rows per block, and k_rk_pick flags the blocks with more than
+// m / kRkShare of them.
 __global__ void __launch_bounds__(256) k_pack_flags(Dev d) {
   __shared__ int32_t smax[4];
   const int round = ctr_get(d, CTR_ROUND), cur = round & 1;
-  if (round < 1 || round >= 3) return;
+  if (round != 1 && round != 2) return;
   const int tid = loop_tid();
   for (ItemLoop it(round_packs(d)); it.i < it.end; it.i += it.step) {
     const int i0 = u32(d.packs[it.i]), i1 = u32(d.packs[it.i + 1]);
@@ -1494,12 +1536,34 @@ __global__ void __launch_bounds__(256) k_pack_flags(Dev d) {
     if ((tid & 63) == 0) smax[tid >> 6] = mx;
     __syncthreads();
     if (max(max(smax[0], smax[1]), max(smax[2], smax[3])) > kCountMax) {
-      for (int i = i0 + tid; i < i1; i += 256) {
-        const Seg g = d.seg[cur][i];
-        if (g.len <= kHalf) rk_need(d, g.blk);
+      for (int base = i0; base < i1; base += 256) {  // whole waves (the sums)
+        const int i = base + tid;
+        Seg g{0, 0, 0, 0};
+        if (i < i1) g = d.seg[cur][i];
+        const int len = g.len <= kHalf ? g.len : 0;
+        if (round == 2) {
+          if (len > 0) rk_need(d, g.blk);
+          continue;
+        }
+        // a wave's groups mostly share a block: one add for the wave then
+        const int b0 = __shfl(g.blk, 0, 64);
+        if (__ballot(len > 0 && g.blk != b0) == 0) {
+          const int sum = wave_sum(len);
+          if ((tid & 63) == 0 && sum > 0) atomicAdd(&d.st[b0].rk_rows, sum);
+        } else if (len > 0) {
+          atomicAdd(&d.st[g.blk].rk_rows, len);
+        }
       }
     }
   }
+}
+
+// Round 1: blocks with many rows in packs sorted by composite keys build
+// their ranks (k_pack_flags).
+__global__ void __launch_bounds__(256) k_rk_pick(Dev d) {
+  if (ctr_get(d, CTR_ROUND) != 1) return;
+  for (int b = blockIdx.x * 256 + threadIdx.x; b < d.nblk; b += gridDim.x * 256)
+    if ((int64_t)d.st[b].rk_rows * kRkShare > d.st[b].m) rk_need(d, b);
 }
 
 // Bucket counters of the blocks whose ranks are built.
@@ -1611,6 +1675,7 @@ __global__ void __launch_bounds__(256) k_rk_groups(Dev d) {
 __global__ void __launch_bounds__(256) k_rk_kept(Dev d) {
   const int n = ctr_get(d, CTR_RKN);
   for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) d.st[d.rk_list[j]].rk = 2;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&d.ctr[CTR_KEPT], n);
 }
 
 void launch_bwt_init(const Dev& d, hipStream_t s) {
@@ -1679,6 +1744,7 @@ void launch_bwt_round(const Dev& d, const SortGrids& g, bool first, int late_bit
     // ranks of the blocks that need them from this round on (the kernels
     // return at once when no block is flagged)
     k_pack_flags<<<g.keys, 256, 0, s>>>(d);
+    k_rk_pick<<<std::max(1, std::min(g.light, (d.nblk + 255) / 256)), 256, 0, s>>>(d);
     k_rkb_zero<<<g.light, 256, 0, s>>>(d);
     k_rk_self<<<g.keys, 256, 0, s>>>(d);
     k_rk_groups<<<g.keys, 256, 0, s>>>(d);
@@ -1697,7 +1763,9 @@ void launch_bwt_round(const Dev& d, const SortGrids& g, bool first, int late_bit
   if (!exact || npacks > 0) {
     probe_begin("k_sort_small", s);
     if (key_bits + 10 <= 32) k_sort_small<<<grid(g.small, npacks), 256, 0, s>>>(d, key_bits);
-    else k_sort_small64<<<grid(g.small64, npacks), 256, 0, s>>>(d, key_bits);
+    else k_sort_small64<<<grid(g.small64, npacks), 256, 0, s>>>(d, key_bits, 0);
+    // round 1's packs sorted by composite text keys (returns in other rounds)
+    if (!first) k_sort_small64<<<grid(g.small64, npacks), 256, 0, s>>>(d, 32, 1);
     probe_end("k_sort_small", s);
   }
   if (exact && ntiles == 0) return;

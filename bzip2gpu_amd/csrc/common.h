@@ -62,6 +62,7 @@ struct BlockState {
   int64_t bit_off;     // bit offset of the block inside its stream
   int32_t err;         // nonzero if a device-side invariant failed
   int32_t rk;          // suffix-sort ranks: 0 not kept, 1 to be built this round, 2 kept
+  int32_t rk_rows;     // suffix sort, round 1: rows in packs sorted by composite keys
 };
 
 // Per-output-stream layout (device-written by the layout kernel).  A whole

@@ -119,6 +119,7 @@ enum : int {
   CTR_ROUND = 8,   // index of the current suffix-sort round (-1 before round 0)
   CTR_RAN = 9,     // rounds that had groups to sort
   CTR_RKN = 10,    // blocks whose ranks are built this round
+  CTR_KEPT = 11,   // blocks that keep ranks (never decreases during a sort)
   CTR_N = 16
 };
 

@@ -101,6 +101,21 @@ def test_bwt_identical_rotations(dev):
         np.testing.assert_array_equal(sa.cpu().numpy(), oracle.bwt(d)[0])
 
 
+def test_bwt_rank_state_between_calls(dev):
+    """Consecutive sorts on one engine: a block that kept its ranks (a
+    tile-long group in round 0) must not leave that state to the next sort,
+    whose ranks are then built only where that sort needs them."""
+    import bzip2gpu_amd as bz
+    import bzip2gpu_amd.datagen as dg
+    seq = [b"a" * 20000, dg.periodic(200000, 4, 64, 0.0005), bytes(range(7)) * 9000,
+           dg.markov(250000, 5, order=2, s=1.2), b"a" * 20000]
+    for i, d in enumerate(seq):
+        d = np.frombuffer(d, dtype=np.uint8) if isinstance(d, bytes) else d
+        _case(f"bwt sequence step={i} m={d.size}")
+        sa, _ = bz.bwt(_t(dev, d))
+        np.testing.assert_array_equal(sa.cpu().numpy(), oracle.bwt(d)[0])
+
+
 def test_symbols_table(dev):
     import bzip2gpu_amd as bz
     _case("symbols table")
