@@ -1300,6 +1300,9 @@ __device__ inline void lheads_item(const Dev& d, int cur, int fin, int t) {
   const uint32_t* S = fin ? d.SA2 : d.SA;
   const int n = min(kTile, g.len - pos);
   const int nxt = cur ^ 1;
+  // SA values are read only to move them (fin) or for the rank buckets
+  const bool kept = u32(d.st[g.blk].rk) == 2;
+  const bool need_sa = fin || kept;
   // carry: last boundary before this tile (largest lb of earlier tiles)
   {
     int lbv = -1;
@@ -1332,7 +1335,7 @@ __device__ inline void lheads_item(const Dev& d, int cur, int fin, int t) {
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
       const int e = tid + NT * q;
-      sv[q] = e < n ? S[base + pos + e] : 0u;
+      sv[q] = need_sa && e < n ? S[base + pos + e] : 0u;
     }
 #pragma unroll
     for (int q = 0; q < KL; q++) {
@@ -1342,12 +1345,14 @@ __device__ inline void lheads_item(const Dev& d, int cur, int fin, int t) {
         sk[ix + ix / RPT] = kv[q];
       }
     }
+    if (need_sa) {
 #pragma unroll
-    for (int q = 0; q < RPT; q++) {
-      const int e = tid + NT * q;
-      if (e < n) {
-        ss[e + e / RPT] = sv[q];
-        if (fin) d.SA[base + pos + e] = sv[q];
+      for (int q = 0; q < RPT; q++) {
+        const int e = tid + NT * q;
+        if (e < n) {
+          ss[e + e / RPT] = sv[q];
+          if (fin) d.SA[base + pos + e] = sv[q];
+        }
       }
     }
   }
@@ -1387,7 +1392,7 @@ __device__ inline void lheads_item(const Dev& d, int cur, int fin, int t) {
   // starts (k_rk_apply writes them), so that a bucket's updates land in one
   // 32 KiB window that is staged in LDS; only blocks that keep ranks (none
   // in round 0, whose rows are ranked later only if their block needs it).
-  if (u32(d.st[g.blk].rk) == 2) {
+  if (kept) {
     const int tile_c0 = u32(d.desc[g.blk].tile_c0);
     const int nbk = (u32(d.st[g.blk].m) + kRkBucket - 1) / kRkBucket;
     for (int j = tid; j < nbk; j += NT) bcnt[j] = 0;
