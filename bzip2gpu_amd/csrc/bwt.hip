@@ -1303,6 +1303,20 @@ __device__ inline void lheads_item(const Dev& d, int cur, int fin, int t) {
   // SA values are read only to move them (fin) or for the rank buckets
   const bool kept = u32(d.st[g.blk].rk) == 2;
   const bool need_sa = fin || kept;
+  // the tile's keys (with the row before and after) and SA values, loaded
+  // before the carry's loads so that both latencies overlap
+  uint32_t kv[KL], sv[RPT];
+#pragma unroll
+  for (int q = 0; q < KL; q++) {
+    const int e = tid - 1 + NT * q;
+    const int p = pos + e;
+    kv[q] = (e <= n && p >= 0 && p < g.len) ? K[base + p] : 0u;
+  }
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    const int e = tid + NT * q;
+    sv[q] = need_sa && e < n ? S[base + pos + e] : 0u;
+  }
   // carry: last boundary before this tile (largest lb of earlier tiles)
   {
     int lbv = -1;
@@ -1320,23 +1334,10 @@ __device__ inline void lheads_item(const Dev& d, int cur, int fin, int t) {
     }
     __syncthreads();
   }
-  // stage the tile's keys (with the row before and after) and SA values in
-  // LDS; row e sits at e + e/RPT (one word of skew per RPT rows), so the
-  // RPT-row runs the threads own below are read without bank conflicts
-  // (all loads are issued before the LDS writes so that their latencies overlap)
+  // stage the keys and SA values in LDS; row e sits at e + e/RPT (one word
+  // of skew per RPT rows), so the RPT-row runs the threads own below are
+  // read without bank conflicts
   {
-    uint32_t kv[KL], sv[RPT];
-#pragma unroll
-    for (int q = 0; q < KL; q++) {
-      const int e = tid - 1 + NT * q;
-      const int p = pos + e;
-      kv[q] = (e <= n && p >= 0 && p < g.len) ? K[base + p] : 0u;
-    }
-#pragma unroll
-    for (int q = 0; q < RPT; q++) {
-      const int e = tid + NT * q;
-      sv[q] = need_sa && e < n ? S[base + pos + e] : 0u;
-    }
 #pragma unroll
     for (int q = 0; q < KL; q++) {
       const int e = tid - 1 + NT * q;
@@ -1450,8 +1451,11 @@ __device__ inline void lheads_item(const Dev& d, int cur, int fin, int t) {
 }
 
 constexpr int kHeadsNT = 512;  // threads per group-head workgroup
+// (at most 80 VGPRs: three 512-thread workgroups per CU instead of two; the
+// pass is latency-bound, and the third workgroup took it from 8.4 to 7.1 ms
+// per batched step)
 template <int NT>
-__global__ void __launch_bounds__(NT) k_lheads(Dev d, int fin) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) k_lheads(Dev d, int fin) {
   const int cur = ctr_get(d, CTR_ROUND) & 1;
   for (ItemLoop it(ctr_get(d, CTR_LTILES)); it.i < it.end; it.i += it.step) {
     __syncthreads();  // the previous tile's LDS is no longer read
