@@ -561,7 +561,9 @@ __device__ inline void keys_item(const Dev& d, int cur, int round, int P, int np
   }
 }
 
-__global__ void __launch_bounds__(256) k_keys(Dev d) {
+// (at most 64 VGPRs: eight workgroups per CU instead of six for the random
+// rank gathers)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_keys(Dev d) {
   __shared__ PackLds L;
   const int round = ctr_get(d, CTR_ROUND), cur = round & 1, P = round_prefix(round);
   const int npacks = round_packs(d);
