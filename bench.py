@@ -7,7 +7,7 @@ U{1..9}, each cut into 100000*level-byte blocks, content = slices of a
 seeded 256 MiB pool of 1 MiB segments drawn 50/25/25 from order-2 Markov
 text (Zipf 1.2), uniform bytes and runs with geometric lengths of mean 64.
 A step compresses every input with the batched pipeline (all blocks of a
-sub-batch of at most 1 GiB in one device pipeline, streams assembled on the
+sub-batch of at most 1280 MiB in one device pipeline, streams assembled on the
 device).  Beside it, on the same inputs: the plain per-block path on one
 stream, the plain path with the reference's own concurrency (16 host
 threads, one HIP stream each, one block per pipeline run), and the CPU
@@ -47,7 +47,7 @@ def _args():
     p.add_argument("--workers", type=int, default=4, help="serve: pipeline workers (HIP streams)")
     p.add_argument("--mib", type=int, default=0, help="override input size (MiB)")
     p.add_argument("--no-plain", action="store_true")
-    p.add_argument("--max-batch-mib", type=int, default=1024, help="batch: raw MiB per device sub-batch")
+    p.add_argument("--max-batch-mib", type=int, default=1280, help="batch: raw MiB per device sub-batch")
     p.add_argument("--no-overlap", action="store_true", help="batch: run sub-batches one after another")
     p.add_argument("--tune-force", default="",
                    help="pin autotuner variants instead of timing them, e.g. 0=0,1=0 (knob=variant)")
