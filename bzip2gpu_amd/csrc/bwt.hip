@@ -212,6 +212,7 @@ __global__ void __launch_bounds__(kScanWG) k_plan_scan2(Dev d) {
     d.ctr[CTR_NSEG] = nseg;
     d.ctr[CTR_ROUND] = r;
     d.ctr[CTR_RKN] = 0;  // blocks flagged for rank building count again
+    d.ctr[CTR_RPK] = 0;
     if (nseg > 0) d.ctr[CTR_RAN] = r + 1;
     if (r < kMaxSortRounds) {  // rows in small groups, then in large groups
       d.round_rows[r] = nseg > 0 ? tots[0] : 0;
@@ -957,10 +958,12 @@ __device__ inline void sort_small_loop(const Dev& d, int keybits, int radix1) {
   __shared__ SortLds<KT> S;
   const int round = ctr_get(d, CTR_ROUND), cur = round & 1;
   if (radix1 && round != 1) return;
-  for (ItemLoop it(round_packs(d)); it.i < it.end; it.i += it.step) {
+  // (the second launch goes over the packs that k_pack_flags listed)
+  for (ItemLoop it(radix1 ? ctr_get(d, CTR_RPK) : round_packs(d)); it.i < it.end; it.i += it.step) {
     __syncthreads();  // the previous pack's LDS is no longer read
     if (round == 1)
-      sort_small_body<KT, true>(d, cur, true, 4, radix1 ? 32 : keybits, it.i, S, radix1 ? 2 : 1);
+      sort_small_body<KT, true>(d, cur, true, 4, radix1 ? 32 : keybits, radix1 ? u32(d.rpk[it.i]) : it.i, S,
+                                radix1 ? 2 : 1);
     else if (round == 2)
       sort_small_body<KT, true>(d, cur, true, 8, keybits, it.i, S, 0);
     else
@@ -1547,6 +1550,8 @@ __global__ void __launch_bounds__(256) k_pack_flags(Dev d) {
     if ((tid & 63) == 0) smax[tid >> 6] = mx;
     __syncthreads();
     if (max(max(smax[0], smax[1]), max(smax[2], smax[3])) > kCountMax) {
+      // (round 1's second sort launch takes only the listed packs)
+      if (round == 1 && tid == 0) d.rpk[atomicAdd(&d.ctr[CTR_RPK], 1)] = it.i;
       for (int base = i0; base < i1; base += 256) {  // whole waves (the sums)
         const int i = base + tid;
         Seg g{0, 0, 0, 0};

@@ -350,6 +350,7 @@ int Engine::setup(const uint8_t* d_in, hipStream_t s, uint8_t* out) {
     d.packs = c.take<int32_t>(pack_cap + 2);
     d.horder = c.take<int32_t>(B + 2);
     d.rk_list = c.take<int32_t>(B + 2);
+    d.rpk = c.take<int32_t>(pack_cap + 2);
     d.ltile_seg = c.take<int32_t>(ltile_cap);
     d.ltile_pos = c.take<int32_t>(ltile_cap);
     d.llist = c.take<int32_t>(ltile_cap);

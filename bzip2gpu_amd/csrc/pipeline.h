@@ -85,6 +85,7 @@ struct Dev {
   int32_t* packs;      // pack -> first seg index
   int32_t* horder;     // blocks in code-length build order (largest alphabet first)
   int32_t* rk_list;    // blocks whose suffix-sort ranks are built this round
+  int32_t* rpk;        // round 1: the packs sorted by composite keys
   int32_t* ltile_seg;  // large tile -> seg index
   int32_t* ltile_pos;  // large tile -> first element within its seg
   int32_t* llist;      // compact list of large seg indices
@@ -120,6 +121,7 @@ enum : int {
   CTR_RAN = 9,     // rounds that had groups to sort
   CTR_RKN = 10,    // blocks whose ranks are built this round
   CTR_KEPT = 11,   // blocks that keep ranks (never decreases during a sort)
+  CTR_RPK = 12,    // round 1: packs sorted by composite keys (listed in rpk)
   CTR_N = 16
 };
 
