@@ -478,14 +478,19 @@ __device__ inline uint64_t text_key8(const uint8_t* T, uint32_t q, uint32_t m) {
 __device__ inline bool text_keys(int P, const PackLds& L) {
   return L.max_len <= kCountMax ? (P == 4 || P == 8) : (P == 4 && L.no_rank);
 }
-__device__ inline void keys_item(const Dev& d, int cur, int round, int P, int npacks, int w, PackLds& L) {
+// Items w < npacks are packs (the listed ones, rpk[w], with `listed`), the
+// others large-group tiles.  (In rounds 1-2 the packs' blocks get their
+// round count from the sort, which loads every pack.)
+__device__ inline void keys_item(const Dev& d, int cur, int round, int P, int npacks, int w, PackLds& L,
+                                 bool listed) {
   const int tid_ = loop_tid();
   if (w < npacks) {
-    load_pack(d, cur, w, L, P == 4);
+    load_pack(d, cur, listed ? u32(d.rpk[w]) : w, L, P == 4);
     if (L.count == 0) return;
     const int total = L.total;
-    for (int j = tid_; j < L.count; j += 256)
-      if (j == 0 || L.blk[j] != L.blk[j - 1]) atomicMax(&d.st[L.blk[j]].rounds, round + 1);
+    if (!listed)
+      for (int j = tid_; j < L.count; j += 256)
+        if (j == 0 || L.blk[j] != L.blk[j - 1]) atomicMax(&d.st[L.blk[j]].rounds, round + 1);
     if (text_keys(P, L)) return;
     // rows' segment and block first (block data reloaded only when the block
     // changes), then all SA loads, then all rank loads, then the stores
@@ -567,10 +572,14 @@ __device__ inline void keys_item(const Dev& d, int cur, int round, int P, int np
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_keys(Dev d) {
   __shared__ PackLds L;
   const int round = ctr_get(d, CTR_ROUND), cur = round & 1, P = round_prefix(round);
-  const int npacks = round_packs(d);
+  // rounds 1-2: only packs sorted by composite keys can need rank keys
+  // (the others are sorted by counting, on text keys), and k_pack_flags
+  // lists them
+  const bool listed = round == 1 || round == 2;
+  const int npacks = listed ? ctr_get(d, CTR_RPK) : round_packs(d);
   for (ItemLoop it(npacks + ctr_get(d, CTR_LTILES)); it.i < it.end; it.i += it.step) {
     __syncthreads();  // the previous item's pack is no longer read
-    keys_item(d, cur, round, P, npacks, it.i, L);
+    keys_item(d, cur, round, P, npacks, it.i, L, listed);
   }
 }
 
@@ -649,6 +658,9 @@ __device__ inline void sort_small_body(const Dev& d, int cur, bool keep_first, i
   load_pack(d, cur, k, L, TEXT && P == 4);
   if (L.count == 0) return;
   const int total = L.total;
+  if (TEXT && only != 2)  // rounds 1-2 (P = 4, 8): the pack's blocks ran round P / 4
+    for (int j = tid_; j < L.count; j += 256)
+      if (j == 0 || L.blk[j] != L.blk[j - 1]) atomicMax(&d.st[L.blk[j]].rounds, P / 4 + 1);
   const bool counting = L.max_len <= kCountMax;
   // rounds 1-2: counting-sorted packs gather their keys from the text (in
   // the key fill below), 4 bytes at SA + 4 in round 1, 8 bytes at SA + 8 in
@@ -1550,8 +1562,9 @@ __global__ void __launch_bounds__(256) k_pack_flags(Dev d) {
     if ((tid & 63) == 0) smax[tid >> 6] = mx;
     __syncthreads();
     if (max(max(smax[0], smax[1]), max(smax[2], smax[3])) > kCountMax) {
-      // (round 1's second sort launch takes only the listed packs)
-      if (round == 1 && tid == 0) d.rpk[atomicAdd(&d.ctr[CTR_RPK], 1)] = it.i;
+      // (the key gather of rounds 1-2 and round 1's second sort launch take
+      // only the listed packs)
+      if (tid == 0) d.rpk[atomicAdd(&d.ctr[CTR_RPK], 1)] = it.i;
       for (int base = i0; base < i1; base += 256) {  // whole waves (the sums)
         const int i = base + tid;
         Seg g{0, 0, 0, 0};
